@@ -1,0 +1,259 @@
+"""Pins the CPU oracle (CPU only, no GPU).
+
+Each oracle function is checked against the reference's own outputs
+(golden vectors in tests/golden/, written by the reference's src/lsmps.cpp)
+and against an independent route that fixes the answer: brute-force
+neighbor search, closed-form derivatives of a quadratic, a library Cholesky
+and a library sparse solve, finite differences for the MLP gradient.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+from conftest import SMALL_FIXTURES, load_fixture
+
+import oracle_py as O
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module", params=SMALL_FIXTURES)
+def fx(request):
+    return load_fixture(request.param)
+
+
+def _oracle_from_fixture(d):
+    return O.Oracle(d["xyz"], d["type_in"])
+
+
+def test_generate_matches_reference(ref_configs):
+    for name, rec in ref_configs.items():
+        c = rec["config"]
+        xyz, t = O.generate_case(c["case"], int(c["particles"]), int(c["seed"]), float(c["l0"]),
+                                 int(c.get("film_layers", 4)))
+        assert sha(xyz) == rec["sha"]["xyz"], name
+        assert sha(t) == rec["sha"]["type_in"], name
+
+
+def test_generate_small_fixtures():
+    for name in SMALL_FIXTURES:
+        d = load_fixture(name)
+        m = d["meta"]
+        if m["case"] == "edge":
+            continue
+        xyz, t = O.generate_case(m["case"], m["particles"], m["seed"], m["l0"], m["film_layers"])
+        assert np.array_equal(xyz, d["xyz"]) and np.array_equal(t, d["type_in"]), name
+
+
+def test_neighbors_classify_wls_ppe_solve_bitexact(fx):
+    m = fx["meta"]
+    re = m["re_ratio"] * m["l0"]
+    o = _oracle_from_fixture(fx)
+    pairs, trunc = o.neighbors(re, m["max_neighbors"])
+    row, col = o.get_neighbors()
+    assert np.array_equal(row, fx["nbr_row"]) and np.array_equal(col, fx["nbr_col"])
+    assert trunc == m["truncated"]
+    o.classify(m["l0"])
+    assert np.array_equal(o.types(), fx["type_out"])
+    f = test_field(fx["xyz"])
+    g, lap, st = o.wls_all(f, re, m["l0"])
+    assert np.array_equal(st, fx["wls_status"])
+    assert np.array_equal(g, fx["wls_grad"]) and np.array_equal(lap, fx["wls_lap"])
+    ws = o.wls_test(re, m["l0"], m["wls_samples"])
+    for k, v in m["wls"].items():
+        assert ws[k] == v, k
+    ppe = o.assemble(re, m["max_unknowns"])
+    A = o.get_ppe()
+    for k in ("row", "col", "val", "b", "x_exact"):
+        assert np.array_equal(A[k], fx["ppe_" + k]), k
+    for k, v in m["ppe"].items():
+        assert ppe[k] == v, k
+    st, x = o.solve(m["max_iter"], m["tol"])
+    for k, v in m["solve"].items():
+        assert st[k] == v, k
+
+
+def test_full_reference_configs(ref_configs):
+    for name, rec in ref_configs.items():
+        c = rec["config"]
+        l0 = float(c["l0"])
+        re = 2.1 * l0
+        xyz, t = O.generate_case(c["case"], int(c["particles"]), int(c["seed"]), l0, int(c.get("film_layers", 4)))
+        o = O.Oracle(xyz, t)
+        pairs, trunc = o.neighbors(re, int(c["max_neighbors"]))
+        row, col = o.get_neighbors()
+        assert pairs == rec["pairs"] and trunc == rec["truncated"]
+        assert sha(row) == rec["sha"]["nbr_row"] and sha(col) == rec["sha"]["nbr_col"], name
+        o.classify(l0)
+        assert sha(o.types()) == rec["sha"]["type_out"], name
+        ws = o.wls_test(re, l0, int(c["wls_samples"]))
+        for k, v in rec["wls"].items():
+            assert ws[k] == v, (name, k)
+        ppe = o.assemble(re, int(c["ppe_max_unknowns"]))
+        A = o.get_ppe()
+        for k in ("row", "col", "val", "b", "x_exact"):
+            assert sha(A[k]) == rec["sha"]["ppe_" + k], (name, k)
+        st, _ = o.solve(int(c["ppe_max_iter"]), float(c["ppe_tol"]))
+        for k, v in rec["solve"].items():
+            assert st[k] == v, (name, k)
+
+
+def test_neighbors_match_brute_force():
+    xyz, t = O.generate_case("rain_injection", 1500, 4, 0.01, 3)
+    for maxn in (96, 14):
+        a = O.Oracle(xyz, t)
+        b = O.Oracle(xyz, t)
+        pa = a.neighbors(0.021, maxn)
+        pb = b.neighbors(0.021, maxn, brute=True)
+        assert pa == pb
+        ra, ca = a.get_neighbors()
+        rb, cb = b.get_neighbors()
+        assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
+    # Symmetric without truncation, never self.
+    a = O.Oracle(xyz, t)
+    a.neighbors(0.021, 1000)
+    row, col = a.get_neighbors()
+    src = np.repeat(np.arange(len(xyz)), np.diff(row))
+    assert not np.any(src == col)
+    fwd = set(zip(src.tolist(), col.tolist()))
+    assert fwd == set((j, i) for i, j in fwd)
+    # Completeness against an all-pairs distance test.
+    d2 = ((xyz[:, None, :] - xyz[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    assert int((d2 < 0.021 ** 2).sum()) == len(col)
+
+
+def test_field(xyz):
+    """The reference's manufactured quadratic (src/lsmps.cpp:28)."""
+    x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+    return (1 + 2 * x - 3 * y + .5 * z + .7 * x * x - .2 * y * y + .4 * z * z + .3 * x * y - .1 * x * z + .25 * y * z)
+
+
+test_field.__test__ = False
+
+
+def test_wls_reproduces_other_quadratic():
+    """Field 1+x+2y-z+x^2+yz+z^2: grad (1+2x, 2+z, -1+y+2z), Laplacian 4."""
+    xyz, t = O.generate_case("bulk_uniform", 3000, 5, 0.01)
+    o = O.Oracle(xyz, t)
+    o.neighbors(0.021, 96)
+    o.classify(0.01)
+    x, y, z = xyz.T
+    f = 1 + x + 2 * y - z + x * x + y * z + z * z
+    g, lap, st = o.wls_all(f, 0.021, 0.01)
+    ok = st <= 1
+    assert ok.sum() > 2000
+    ge = np.stack([1 + 2 * x, 2 + z, -1 + y + 2 * z], 1)
+    assert np.abs(g[ok] - ge[ok]).max() < 1e-9
+    assert np.abs(lap[ok] - 4.0).max() < 1e-6
+    assert np.sqrt(np.mean((lap[ok] - 4.0) ** 2)) < 1e-8
+
+
+def test_wls_against_library_cholesky():
+    d = load_fixture("film_1200")
+    m = d["meta"]
+    re = m["re_ratio"] * m["l0"]
+    o = _oracle_from_fixture(d)
+    o.neighbors(re, m["max_neighbors"])
+    o.classify(m["l0"])
+    f = test_field(d["xyz"])
+    g, lap, st = o.wls_all(f, re, m["l0"])
+    for i in np.flatnonzero(st == 0)[::37]:
+        ws, M, b = o.wls_moments(f, re, m["l0"], int(i))
+        # Lower triangle is what the factorisation reads.
+        Ml = np.tril(M) + np.tril(M, -1).T
+        L = np.linalg.cholesky(Ml)
+        xs = np.linalg.solve(L.T, np.linalg.solve(L, b))
+        assert np.allclose(xs[:3], g[i], rtol=1e-9, atol=1e-9)
+        assert abs((xs[3] + xs[4] + xs[5]) / m["l0"] - lap[i]) < 1e-6
+
+
+def test_jacobi_bicgstab_against_spsolve():
+    sp = pytest.importorskip("scipy.sparse")
+    spl = pytest.importorskip("scipy.sparse.linalg")
+    d = load_fixture("film_1200")
+    m = d["meta"]
+    re = m["re_ratio"] * m["l0"]
+    o = _oracle_from_fixture(d)
+    o.neighbors(re, m["max_neighbors"])
+    o.classify(m["l0"])
+    o.assemble(re, m["max_unknowns"], xmode=1, xseed=3)
+    A = o.get_ppe()
+    n = len(A["b"])
+    M = sp.csr_matrix((A["val"], A["col"], A["row"]), shape=(n, n))
+    xd = spl.spsolve(M.tocsc(), A["b"])
+    for pc in (0, 1):
+        st, x = o.solve(5000, 1e-12, precond=pc)
+        assert st["converged"] == 1
+        assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+        assert st["final_residual"] <= 1e-12 * max(1, np.linalg.norm(A["b"])) * 10
+    # Jacobi helps on the graph Laplacian of a wall film: fewer iterations.
+    s0, _ = o.solve(5000, 1e-10, precond=0)
+    s1, _ = o.solve(5000, 1e-10, precond=1)
+    assert s1["iterations"] <= s0["iterations"] * 1.2
+    # A good start lowers the initial residual.
+    s2, _ = o.solve(5000, 1e-10, precond=1, x0=A["x_exact"] * 0.999)
+    assert s2["initial_residual"] < 0.01 * s1["initial_residual"]
+
+
+def test_diag_shift_and_random_rhs():
+    d = load_fixture("film_1200")
+    m = d["meta"]
+    re = m["re_ratio"] * m["l0"]
+    o = _oracle_from_fixture(d)
+    o.neighbors(re, m["max_neighbors"])
+    o.classify(m["l0"])
+    o.assemble(re, m["max_unknowns"], xmode=1, xseed=5, shift=0.05)
+    A = o.get_ppe()
+    n = len(A["b"])
+    # Row 0 pinned; other rows: shift*offdiag-sum on the diagonal.
+    assert A["row"][1] == 1 and A["val"][0] == 1.0
+    for r in range(1, n, 97):
+        s, e = A["row"][r], A["row"][r + 1]
+        assert A["col"][e - 1] == r
+        off = -A["val"][s:e - 1].sum()
+        assert abs(A["val"][e - 1] - 1.05 * off) <= 1e-12 * abs(off)
+    assert np.all(np.abs(A["x_exact"]) <= 1.0)
+    # b = A x* recomputed by numpy.
+    y = np.zeros(n)
+    for r in range(n):
+        s, e = A["row"][r], A["row"][r + 1]
+        y[r] = np.dot(A["val"][s:e], A["x_exact"][A["col"][s:e]])
+    assert np.allclose(y, A["b"], rtol=1e-12, atol=1e-12)
+
+
+def _np_mlp(P, X):
+    o = 0
+    W1 = P[o:o + 64 * 16].reshape(64, 16); o += 64 * 16
+    b1 = P[o:o + 64]; o += 64
+    W2 = P[o:o + 64 * 64].reshape(64, 64); o += 64 * 64
+    b2 = P[o:o + 64]; o += 64
+    W3 = P[o:o + 2 * 64].reshape(2, 64); o += 128
+    b3 = P[o:o + 2]
+    h1 = np.tanh(X @ W1.T + b1)
+    h2 = np.tanh(h1 @ W2.T + b2)
+    return h2 @ W3.T + b3
+
+
+def test_mlp_forward_and_gradient():
+    rng = np.random.default_rng(0)
+    P = (rng.standard_normal(5378) * 0.2).astype(np.float32)
+    X = rng.standard_normal((40, 16)).astype(np.float32)
+    T = rng.standard_normal((40, 2)).astype(np.float32)
+    wk = np.array([1.0, 0.5], np.float32)
+    y = O.mlp_forward(P, X)
+    assert np.allclose(y, _np_mlp(P.astype(np.float64), X.astype(np.float64)), atol=1e-5)
+    loss, g = O.mlp_loss_grad(P, X, T, wk)
+    ref = 0.5 * np.mean(((_np_mlp(P.astype(np.float64), X.astype(np.float64)) - T) ** 2 * wk).sum(1))
+    assert abs(loss - ref) < 1e-5 * max(1, ref)
+    # Central differences in float64 on a numpy copy of the network.
+    Pd = P.astype(np.float64)
+    for k in list(rng.choice(5378, 25, replace=False)) + [0, 1087, 5377]:
+        e = np.zeros_like(Pd)
+        e[k] = 1e-6
+        lp = 0.5 * np.mean(((_np_mlp(Pd + e, X) - T) ** 2 * wk).sum(1))
+        lm = 0.5 * np.mean(((_np_mlp(Pd - e, X) - T) ** 2 * wk).sum(1))
+        fd = (lp - lm) / 2e-6
+        assert abs(fd - g[k]) <= 2e-4 * max(1.0, abs(fd)), (k, fd, g[k])
