@@ -1,0 +1,166 @@
+/*
+ * C ABI of the MI355X (gfx950) LS-MPS per-step particle pipeline.
+ *
+ * Plain C: opaque context, raw host pointers and sizes, int status codes.
+ * No exceptions, no C++ or torch types cross it. Each entry point replaces
+ * one internal stage of the reference's run_benchmark (src/lsmps.cpp:73);
+ * the C++ API of include/lsmps.hpp is implemented on top of it
+ * (ls-mps_amd/csrc/host/lsmps_api.cpp), so the reference's callers keep
+ * working unchanged.
+ *
+ * Data stays in HBM between calls: upload positions and types once, run the
+ * stages, download only what a caller asks for. A context owns one HIP
+ * stream and its device buffers; use one context per host thread.
+ * Stage times are HIP-event times on the context's stream, in ms.
+ */
+#ifndef LSMPS_HIP_H
+#define LSMPS_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSMPS_HIP_ABI_VERSION 1
+
+enum {
+  LSMPS_OK = 0,
+  LSMPS_E_ARG = -1,    /* invalid argument or size */
+  LSMPS_E_HIP = -2,    /* HIP runtime error (message in lsmps_last_error) */
+  LSMPS_E_STATE = -3,  /* stage called before the stage it depends on */
+  LSMPS_E_LIMIT = -4,  /* size beyond a documented limit (int32 indices, grid) */
+};
+
+/* Particle types, numbered as the reference's ParticleType (include/lsmps.hpp:8). */
+enum { LSMPS_INTERIOR = 0, LSMPS_FREE_SURFACE = 1, LSMPS_SPLASH = 2, LSMPS_NEAR_WALL = 3, LSMPS_WALL = 4 };
+/* Per-particle WLS status. */
+enum { LSMPS_WLS_OK = 0, LSMPS_WLS_REGULARIZED = 1, LSMPS_WLS_FAILED = 2, LSMPS_WLS_SKIPPED = 3 };
+
+typedef struct lsmps_ctx lsmps_ctx;
+
+int lsmps_abi_version(void);
+/* Context on HIP device `device`. */
+int lsmps_create(int device, lsmps_ctx** out);
+void lsmps_destroy(lsmps_ctx* ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char* lsmps_last_error(const lsmps_ctx* ctx);
+/* Waits for all work queued on the context's stream. */
+int lsmps_synchronize(lsmps_ctx* ctx);
+
+/* Copies n particles (xyz interleaved, 3n doubles; type n bytes) into HBM.
+ * Replaces the std::vector<Particle> handed to run_benchmark
+ * (include/lsmps.hpp:10,29). Positions are stored SoA in FP64. */
+int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* type, uint64_t n);
+
+/* ---- cell-list neighbors: replaces neighbors() (src/lsmps.cpp:26) ------ */
+typedef struct {
+  uint64_t pairs;        /* sum of kept list lengths (NeighborData::pairs) */
+  uint64_t truncated;    /* dropped pairs beyond max_neighbors (NeighborData::truncated) */
+  double mean;           /* mean kept count (Metrics::neighbor_mean) */
+  int32_t p50, p90, p99; /* quantiles as run_benchmark takes them (src/lsmps.cpp:73) */
+  int32_t max_count;
+  int32_t grid[3];       /* cells per axis of the dense grid */
+  float ms;              /* GPU time of the whole stage */
+} lsmps_cnl_stats;
+/* Lists are CSR (row_ptr int32[n+1], col int32[pairs]) in the reference's
+ * visiting order, truncated to max_neighbors (<= 1024). */
+int lsmps_build_neighbors(lsmps_ctx* ctx, double re, int32_t max_neighbors, lsmps_cnl_stats* out);
+
+/* ---- classification: replaces classify() applied in run_benchmark
+ * (src/lsmps.cpp:27,73). Overwrites the device types in place. ---------- */
+typedef struct {
+  uint64_t count[5]; /* per type, in LSMPS_* order */
+  float ms;
+} lsmps_type_stats;
+int lsmps_classify(lsmps_ctx* ctx, double l0, lsmps_type_stats* out);
+
+/* ---- WLS 9x9 operator: per-particle form of wls_test (src/lsmps.cpp:22-25,28) */
+/* Field sampled at the particles. Quadratic form: coefficients of
+ * 1, x, y, z, x^2, y^2, z^2, xy, xz, yz and its exact Laplacian (used for
+ * error statistics). The reference's test field is
+ * {1, 2, -3, .5, .7, -.2, .4, .3, -.1, .25} with Laplacian 1.8. */
+typedef struct {
+  double c[10];
+  double lap_exact;
+} lsmps_quad_field;
+int lsmps_set_field_quadratic(lsmps_ctx* ctx, const lsmps_quad_field* f);
+/* Arbitrary field values (n doubles). Error statistics are then unavailable. */
+int lsmps_set_field_values(lsmps_ctx* ctx, const double* f, uint64_t n);
+
+enum { LSMPS_WLS_FP64 = 0, LSMPS_WLS_MIXED = 1 };
+typedef struct {
+  double re, l0;
+  double reg;        /* regularization base; the reference always uses 1e-8 */
+  int32_t precision; /* LSMPS_WLS_FP64: bit-exact with the reference;
+                        LSMPS_WLS_MIXED: FP32 moments, FP64 redo of weak rows */
+} lsmps_wls_params;
+typedef struct {
+  uint64_t ok, regularized, failed, skipped;
+  uint64_t fp64_redo; /* MIXED: rows redone in FP64 */
+  float ms;
+} lsmps_wls_result;
+/* Gradient and Laplacian for every particle except Wall and Splash. */
+int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out);
+
+/* The reference's sampled statistics (WlsStats, include/lsmps.hpp:20):
+ * stride max(1, n / wls_samples) from particle 0 until wls_samples are
+ * counted, Wall/Splash skipped (src/lsmps.cpp:28). Needs a quadratic field. */
+typedef struct {
+  uint64_t sampled, ok, regularized, failed, virtual_points;
+  double max_gradient_error, rms_gradient_error, max_laplacian_error, rms_laplacian_error;
+} lsmps_wls_stats;
+int lsmps_wls_sampled_stats(lsmps_ctx* ctx, int32_t wls_samples, lsmps_wls_stats* out);
+
+/* ---- PPE assembly: replaces assemble_ppe (src/lsmps.cpp:29-64) --------- */
+enum { LSMPS_XSTAR_QUADRATIC = 0, LSMPS_XSTAR_RANDOM = 1 };
+typedef struct {
+  double re;
+  int32_t max_unknowns; /* Config::ppe_max_unknowns */
+  int32_t xmode;        /* x* = x^2 + y^2 (reference) or U(-1,1) from mt19937(xseed) */
+  uint32_t xseed;
+  double shift;         /* diagonal *= 1 + shift after assembly; 0 = reference */
+} lsmps_ppe_params;
+typedef struct {
+  uint64_t unknowns, nnz, dirichlet, neumann, failed_rows; /* PpeStats fields */
+  float ms;
+} lsmps_ppe_result;
+int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_result* out);
+
+/* ---- BiCGStab: replaces solve_ppe (src/lsmps.cpp:65-67) ---------------- */
+enum {
+  LSMPS_SOLVE_PLAIN = 0,   /* FP64, unpreconditioned, the reference's stopping rule */
+  LSMPS_SOLVE_JACOBI = 1,  /* FP64, right Jacobi, true-residual check */
+  LSMPS_SOLVE_MIXED = 2,   /* FP32 Krylov + FP64 reliable updates, right Jacobi */
+  LSMPS_SOLVE_LEARNED = 3, /* MIXED with the MLP diagonal and initial guess,
+                              Jacobi fallback when the guess raises the residual */
+};
+typedef struct {
+  int32_t method;
+  int32_t max_iter;
+  double tol;          /* ||r|| <= tol * max(1, ||b||) */
+  int32_t use_x0;      /* start from the vector given by lsmps_upload_x0 */
+  int32_t chunk;       /* iterations per captured HIP graph (0 = default 16) */
+} lsmps_solve_params;
+typedef struct {
+  int32_t iterations, converged, state, replacements;
+  int32_t fallback;     /* LEARNED: 1 when the Jacobi path was used */
+  double initial_residual, final_residual, relative_l2_error, max_error;
+  float ms;             /* GPU time from setup to the final residual */
+} lsmps_solve_result;
+int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_result* out);
+int lsmps_upload_x0(lsmps_ctx* ctx, const double* x0, uint64_t n);
+
+/* ---- downloads (tests, parity, callers that want the arrays) ----------- */
+int lsmps_download_types(lsmps_ctx* ctx, uint8_t* type);
+int lsmps_download_neighbors(lsmps_ctx* ctx, int32_t* row_ptr, int32_t* col);
+int lsmps_download_wls(lsmps_ctx* ctx, double* grad3, double* lap, uint8_t* status);
+/* Any pointer may be NULL. rev[k] = particle index of unknown k. */
+int lsmps_download_ppe(lsmps_ctx* ctx, int32_t* row, int32_t* col, double* val, double* b, double* x_exact,
+                       int32_t* rev);
+int lsmps_download_solution(lsmps_ctx* ctx, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

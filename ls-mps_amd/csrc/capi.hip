@@ -1,0 +1,811 @@
+// C ABI of the gfx950 LS-MPS pipeline (declared in include/lsmps_hip.h).
+//
+// A context owns a HIP stream, events, and grow-only device buffers. Each
+// entry point queues its stage's kernels on the stream, times them with HIP
+// events, and synchronises once to return the stage's statistics. Nothing
+// here runs a stage on the CPU: every number a stage reports is computed by
+// the kernels; the host only walks the small arrays the reference's
+// statistics are defined over (neighbor-count histogram, sampled WLS
+// errors).
+#include "lsmps_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "solver.hpp"
+
+using namespace lsmps_hip;
+
+namespace {
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = n > 0 ? n : 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  const void* key[6] = {};
+  int m = -1, chunk = 0, with_replace = -1, fp32 = -1;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    m = -1;
+  }
+};
+
+}  // namespace
+
+struct lsmps_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::string err;
+
+  // particles (SoA FP64 positions, types)
+  uint64_t n = 0;
+  DBuf<double> xyz_in, x, y, z;
+  DBuf<uint8_t> type;
+
+  // cell list and neighbor CSR
+  bool have_cnl = false;
+  double re = 0.0;
+  int maxn = 0, gx = 0, gy = 0, gz = 0;
+  uint64_t pairs = 0;
+  DBuf<int> cx, cy, cz, box, cell, slot, ccount, cstart, perm, ncount, row_ptr, col, hist;
+  DBuf<double> sx, sy, sz;
+  DBuf<int64_t> scan_scr;
+  DBuf<unsigned long long> counters;  // 16 general-purpose counters
+
+  // field and WLS
+  bool have_field = false, field_quadratic = false, have_wls = false;
+  QuadField qf{};
+  DBuf<double> f, grad, lap, ge, le;
+  DBuf<uint8_t> wstatus, st_s, ty_s;
+  DBuf<int> redo, nredo;
+
+  // PPE
+  bool have_ppe = false;
+  uint64_t m = 0, nnz = 0;
+  DBuf<int> flag, pos, map, rev, rcnt, prow, pcol;
+  DBuf<double> pval, b, xe, diag;
+
+  // solver
+  DBuf<double> scale, yv, xsol, x0, val64s, vec64, partials;
+  DBuf<float> val32, vec32;
+  DBuf<SolverScalars> sc;
+  SolverScalars* host_sc = nullptr;  // pinned
+  int* host_small = nullptr;         // pinned scratch for small readbacks (64 ints)
+  bool have_x0 = false, have_solution = false;
+  GraphCache graph;
+};
+
+namespace {
+
+int fail(lsmps_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                                    \
+  do {                                                                                                 \
+    hipError_t _e = (call);                                                                            \
+    if (_e != hipSuccess) return fail(ctx, LSMPS_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                                    \
+  }                                                                  \
+  catch (const std::bad_alloc&) {                                    \
+    return fail(ctx, LSMPS_E_HIP, "host allocation failed");         \
+  }                                                                  \
+  catch (const std::exception& ex) {                                 \
+    return fail(ctx, LSMPS_E_HIP, std::string("exception: ") + ex.what()); \
+  }
+
+int check_launch(lsmps_ctx* ctx, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, LSMPS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return LSMPS_OK;
+}
+
+float elapsed(lsmps_ctx* ctx) {
+  float ms = 0.0f;
+  if (hipEventSynchronize(ctx->e1) != hipSuccess) return -1.0f;
+  if (hipEventElapsedTime(&ms, ctx->e0, ctx->e1) != hipSuccess) return -1.0f;
+  return ms;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+__global__ void k_split_xyz(const double* __restrict__ in, int64_t n, double* __restrict__ x, double* __restrict__ y,
+                            double* __restrict__ z) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] = in[3 * i];
+    y[i] = in[3 * i + 1];
+    z[i] = in[3 * i + 2];
+  }
+}
+
+void zero_counters(lsmps_ctx* ctx, int k) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->st, ctx->counters.p, k);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsmps_abi_version(void) { return LSMPS_HIP_ABI_VERSION; }
+
+int lsmps_create(int device, lsmps_ctx** out) {
+  if (!out) return LSMPS_E_ARG;
+  *out = nullptr;
+  lsmps_ctx* ctx = new (std::nothrow) lsmps_ctx;
+  if (!ctx) return LSMPS_E_HIP;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    delete ctx;
+    return LSMPS_E_ARG;
+  }
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->e0) != hipSuccess || hipEventCreate(&ctx->e1) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&ctx->host_sc), sizeof(SolverScalars)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&ctx->host_small), 64 * sizeof(int64_t)) != hipSuccess ||
+      ctx->counters.ensure(16) != hipSuccess || ctx->sc.ensure(1) != hipSuccess ||
+      ctx->partials.ensure(4 * SOLVER_MAX_GRID) != hipSuccess || ctx->box.ensure(8) != hipSuccess ||
+      ctx->nredo.ensure(1) != hipSuccess) {
+    lsmps_destroy(ctx);
+    return LSMPS_E_HIP;
+  }
+  *out = ctx;
+  return LSMPS_OK;
+}
+
+void lsmps_destroy(lsmps_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+  ctx->graph.reset();
+  DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x, &ctx->y, &ctx->z, &ctx->sx, &ctx->sy, &ctx->sz, &ctx->f,
+                       &ctx->grad, &ctx->lap, &ctx->ge, &ctx->le, &ctx->pval, &ctx->b, &ctx->xe, &ctx->diag,
+                       &ctx->scale, &ctx->yv, &ctx->xsol, &ctx->x0, &ctx->val64s, &ctx->vec64, &ctx->partials};
+  for (auto* p : d) p->release();
+  DBuf<int>* i[] = {&ctx->cx, &ctx->cy, &ctx->cz, &ctx->box, &ctx->cell, &ctx->slot, &ctx->ccount, &ctx->cstart,
+                    &ctx->perm, &ctx->ncount, &ctx->row_ptr, &ctx->col, &ctx->hist, &ctx->redo, &ctx->nredo,
+                    &ctx->flag, &ctx->pos, &ctx->map, &ctx->rev, &ctx->rcnt, &ctx->prow, &ctx->pcol};
+  for (auto* p : i) p->release();
+  ctx->type.release();
+  ctx->wstatus.release();
+  ctx->st_s.release();
+  ctx->ty_s.release();
+  ctx->scan_scr.release();
+  ctx->counters.release();
+  ctx->val32.release();
+  ctx->vec32.release();
+  ctx->sc.release();
+  if (ctx->host_sc) (void)hipHostFree(ctx->host_sc);
+  if (ctx->host_small) (void)hipHostFree(ctx->host_small);
+  if (ctx->e0) (void)hipEventDestroy(ctx->e0);
+  if (ctx->e1) (void)hipEventDestroy(ctx->e1);
+  if (ctx->st) (void)hipStreamDestroy(ctx->st);
+  delete ctx;
+}
+
+const char* lsmps_last_error(const lsmps_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int lsmps_synchronize(lsmps_ctx* ctx) {
+  if (!ctx) return LSMPS_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return LSMPS_OK;
+}
+
+int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* type, uint64_t n) {
+  if (!ctx) return LSMPS_E_ARG;
+  if ((n && (!xyz || !type))) return fail(ctx, LSMPS_E_ARG, "null particle arrays");
+  if (n >= (uint64_t)INT_MAX / 4) return fail(ctx, LSMPS_E_LIMIT, "particle count beyond int32 indexing");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ctx->xyz_in.ensure(3 * n));
+  HIPCHK(ctx->x.ensure(n));
+  HIPCHK(ctx->y.ensure(n));
+  HIPCHK(ctx->z.ensure(n));
+  HIPCHK(ctx->type.ensure(n));
+  if (n) {
+    HIPCHK(hipMemcpyAsync(ctx->xyz_in.p, xyz, 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->type.p, type, n, hipMemcpyHostToDevice, ctx->st));
+    hipLaunchKernelGGL(k_split_xyz, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->st, ctx->xyz_in.p,
+                       (int64_t)n, ctx->x.p, ctx->y.p, ctx->z.p);
+    if (int rc = check_launch(ctx, "split_xyz")) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->n = n;
+  ctx->have_cnl = ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_x0 = false;
+  ctx->have_field = false;
+  ctx->graph.reset();
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_build_neighbors(lsmps_ctx* ctx, double re, int32_t maxn, lsmps_cnl_stats* out) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (!(re > 0) || !std::isfinite(re)) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
+  if (maxn < 0 || maxn > 1024) return fail(ctx, LSMPS_E_LIMIT, "max_neighbors must be in [0, 1024]");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = (int)ctx->n;
+  hipStream_t st = ctx->st;
+  HIPCHK(ctx->cx.ensure(n));
+  HIPCHK(ctx->cy.ensure(n));
+  HIPCHK(ctx->cz.ensure(n));
+  HIPCHK(ctx->cell.ensure(n));
+  HIPCHK(ctx->slot.ensure(n));
+  HIPCHK(ctx->perm.ensure(n));
+  HIPCHK(ctx->ncount.ensure(n));
+  HIPCHK(ctx->row_ptr.ensure((size_t)n + 1));
+  HIPCHK(ctx->sx.ensure(n));
+  HIPCHK(ctx->sy.ensure(n));
+  HIPCHK(ctx->sz.ensure(n));
+  HIPCHK(ctx->hist.ensure((size_t)maxn + 1));
+  HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  launch_cell_coords(ctx->x.p, ctx->y.p, ctx->z.p, n, re, ctx->cx.p, ctx->cy.p, ctx->cz.p, ctx->box.p, st);
+  if (int rc = check_launch(ctx, "cell_coords")) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->host_small, ctx->box.p, 6 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  int bx[6];
+  std::memcpy(bx, ctx->host_small, sizeof(bx));
+  int64_t g[3] = {1, 1, 1};
+  if (n > 0)
+    for (int k = 0; k < 3; ++k) g[k] = (int64_t)bx[3 + k] - bx[k] + 1;
+  else
+    bx[0] = bx[1] = bx[2] = 0;
+  const int64_t ncells = g[0] * g[1] * g[2];
+  // Dense grid: bounded so a sparse outlier cannot blow up memory.
+  if (ncells > ((int64_t)1 << 30) || ncells > 64 * (int64_t)n + ((int64_t)1 << 24))
+    return fail(ctx, LSMPS_E_LIMIT, "particle bounding box spans too many cells for the dense grid");
+  ctx->gx = (int)g[0];
+  ctx->gy = (int)g[1];
+  ctx->gz = (int)g[2];
+  HIPCHK(ctx->ccount.ensure((size_t)ncells));
+  HIPCHK(ctx->cstart.ensure((size_t)ncells + 1));
+  HIPCHK(ctx->scan_scr.ensure(std::max(scan_scratch_elems(ncells), scan_scratch_elems((int64_t)n + 1))));
+  launch_cell_sort(ctx->cx.p, ctx->cy.p, ctx->cz.p, n, bx, ctx->gx, ctx->gy, ctx->gz, ctx->cell.p, ctx->slot.p,
+                   ctx->ccount.p, ctx->cstart.p, ctx->scan_scr.p, ctx->perm.p, ctx->x.p, ctx->y.p, ctx->z.p, ctx->sx.p,
+                   ctx->sy.p, ctx->sz.p, st);
+  if (int rc = check_launch(ctx, "cell_sort")) return rc;
+  zero_counters(ctx, 16);
+  launch_fill_int(ctx->hist.p, maxn + 1, 0, st);
+  const double re2 = re * re;
+  launch_nbr_count(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->perm.p, ctx->cstart.p, ctx->cell.p, n, ctx->gx, ctx->gy,
+                   ctx->gz, re2, maxn, ctx->ncount.p, ctx->counters.p, ctx->hist.p, st);
+  if (int rc = check_launch(ctx, "nbr_count")) return rc;
+  launch_exclusive_scan(ctx->ncount.p, n, ctx->row_ptr.p, ctx->scan_scr.p, st);
+  if (int rc = check_launch(ctx, "scan")) return rc;
+  const int ntiles = (int)((n + 2047) / 2048);
+  int64_t* hs = reinterpret_cast<int64_t*>(ctx->host_small);
+  HIPCHK(hipMemcpyAsync(hs, ctx->scan_scr.p + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const int64_t total = hs[0];
+  if (total > (int64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "neighbor pairs beyond int32 indexing");
+  HIPCHK(ctx->col.ensure((size_t)total));
+  launch_nbr_fill(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->perm.p, ctx->cstart.p, ctx->cell.p, n, ctx->gx, ctx->gy,
+                  ctx->gz, re2, maxn, ctx->row_ptr.p, ctx->col.p, st);
+  if (int rc = check_launch(ctx, "nbr_fill")) return rc;
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  std::vector<int> hist((size_t)maxn + 1);
+  unsigned long long trunc = 0;
+  HIPCHK(hipMemcpyAsync(hist.data(), ctx->hist.p, hist.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&trunc, ctx->counters.p, sizeof(trunc), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->re = re;
+  ctx->maxn = maxn;
+  ctx->pairs = (uint64_t)total;
+  ctx->have_cnl = true;
+  ctx->have_wls = ctx->have_ppe = ctx->have_solution = false;
+  if (out) {
+    std::memset(out, 0, sizeof(*out));
+    out->pairs = (uint64_t)total;
+    out->truncated = trunc;
+    out->grid[0] = ctx->gx;
+    out->grid[1] = ctx->gy;
+    out->grid[2] = ctx->gz;
+    out->ms = elapsed(ctx);
+    if (n > 0) {
+      double sum = 0.0;
+      for (size_t k = 0; k < hist.size(); ++k) {
+        sum += (double)k * hist[k];
+        if (hist[k]) out->max_count = (int32_t)k;
+      }
+      out->mean = sum / n;
+      // Quantile as run_benchmark takes it: sorted[(size_t)(q*(n-1))].
+      auto quant = [&](double q) {
+        const uint64_t idx = (uint64_t)(q * (double)(n - 1));
+        uint64_t cum = 0;
+        for (size_t k = 0; k < hist.size(); ++k) {
+          cum += (uint64_t)hist[k];
+          if (cum > idx) return (int32_t)k;
+        }
+        return (int32_t)(hist.size() - 1);
+      };
+      out->p50 = quant(.5);
+      out->p90 = quant(.9);
+      out->p99 = quant(.99);
+    }
+  }
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_classify(lsmps_ctx* ctx, double l0, lsmps_type_stats* out) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "classify needs neighbors");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->st;
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  zero_counters(ctx, 16);
+  launch_classify(ctx->x.p, ctx->y.p, ctx->z.p, ctx->row_ptr.p, ctx->col.p, (int)ctx->n, 1.5 * l0, ctx->type.p,
+                  ctx->counters.p, st);
+  if (int rc = check_launch(ctx, "classify")) return rc;
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  unsigned long long c[5];
+  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->have_wls = ctx->have_ppe = ctx->have_solution = false;
+  if (out) {
+    for (int k = 0; k < 5; ++k) out->count[k] = c[k];
+    out->ms = elapsed(ctx);
+  }
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_set_field_quadratic(lsmps_ctx* ctx, const lsmps_quad_field* q) {
+  if (!ctx || !q) return LSMPS_E_ARG;
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ctx->f.ensure(ctx->n));
+  std::memcpy(ctx->qf.c, q->c, sizeof(ctx->qf.c));
+  ctx->qf.lap_exact = q->lap_exact;
+  launch_field_quadratic(ctx->x.p, ctx->y.p, ctx->z.p, (int)ctx->n, ctx->qf, ctx->f.p, ctx->st);
+  if (int rc = check_launch(ctx, "field")) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->have_field = true;
+  ctx->field_quadratic = true;
+  ctx->have_wls = false;
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_set_field_values(lsmps_ctx* ctx, const double* f, uint64_t n) {
+  if (!ctx || (n && !f)) return LSMPS_E_ARG;
+  if (n != ctx->n) return fail(ctx, LSMPS_E_ARG, "field length differs from the particle count");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ctx->f.ensure(n));
+  if (n) HIPCHK(hipMemcpy(ctx->f.p, f, n * sizeof(double), hipMemcpyHostToDevice));
+  ctx->have_field = true;
+  ctx->field_quadratic = false;
+  ctx->have_wls = false;
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) {
+  if (!ctx || !p) return LSMPS_E_ARG;
+  if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "WLS needs neighbors");
+  if (!ctx->have_field) return fail(ctx, LSMPS_E_STATE, "WLS needs a field (lsmps_set_field_*)");
+  if (p->precision != LSMPS_WLS_FP64 && p->precision != LSMPS_WLS_MIXED)
+    return fail(ctx, LSMPS_E_ARG, "unknown WLS precision");
+  if (!(p->re > 0) || !(p->l0 > 0)) return fail(ctx, LSMPS_E_ARG, "re and l0 must be positive");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = (int)ctx->n;
+  hipStream_t st = ctx->st;
+  HIPCHK(ctx->grad.ensure(3 * (size_t)n));
+  HIPCHK(ctx->lap.ensure(n));
+  HIPCHK(ctx->wstatus.ensure(n));
+  if (p->precision == LSMPS_WLS_MIXED) HIPCHK(ctx->redo.ensure(n));
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  zero_counters(ctx, 16);
+  if (p->precision == LSMPS_WLS_FP64) {
+    launch_wls_exact(ctx->x.p, ctx->y.p, ctx->z.p, ctx->f.p, ctx->type.p, ctx->row_ptr.p, ctx->col.p, n, p->re, p->l0,
+                     p->reg, ctx->grad.p, ctx->lap.p, ctx->wstatus.p, ctx->counters.p, st);
+  } else {
+    launch_fill_int(ctx->nredo.p, 1, 0, st);
+    launch_wls_mixed(ctx->x.p, ctx->y.p, ctx->z.p, ctx->f.p, ctx->type.p, ctx->row_ptr.p, ctx->col.p, n, p->re,
+                     p->l0, p->reg, ctx->grad.p, ctx->lap.p, ctx->wstatus.p, ctx->redo.p, ctx->nredo.p,
+                     ctx->counters.p, st);
+  }
+  if (int rc = check_launch(ctx, "wls")) return rc;
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  unsigned long long c[6];
+  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->have_wls = true;
+  if (out) {
+    out->ok = c[WLS_OK];
+    out->regularized = c[WLS_REGULARIZED];
+    out->failed = c[WLS_FAILED] + c[WLS_NO_WEIGHT];
+    out->skipped = c[WLS_SKIPPED];
+    out->fp64_redo = c[WLS_REDO_SLOT];
+    out->ms = elapsed(ctx);
+  }
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_wls_sampled_stats(lsmps_ctx* ctx, int32_t samples, lsmps_wls_stats* out) {
+  if (!ctx || !out) return LSMPS_E_ARG;
+  if (!ctx->have_wls || !ctx->field_quadratic)
+    return fail(ctx, LSMPS_E_STATE, "sampled statistics need a WLS run on a quadratic field");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  std::memset(out, 0, sizeof(*out));
+  const int64_t n = (int64_t)ctx->n;
+  const int64_t stride = std::max<int64_t>(1, n / std::max(1, samples));
+  const int ns = n > 0 ? (int)((n + stride - 1) / stride) : 0;
+  HIPCHK(ctx->ge.ensure(ns));
+  HIPCHK(ctx->le.ensure(ns));
+  HIPCHK(ctx->st_s.ensure(ns));
+  HIPCHK(ctx->ty_s.ensure(ns));
+  launch_wls_sample_errors(ctx->x.p, ctx->y.p, ctx->z.p, ctx->type.p, ctx->grad.p, ctx->lap.p, ctx->wstatus.p, stride,
+                           ns, ctx->qf, ctx->ge.p, ctx->le.p, ctx->st_s.p, ctx->ty_s.p, ctx->st);
+  if (int rc = check_launch(ctx, "wls_sample_errors")) return rc;
+  std::vector<double> ge(ns), le(ns);
+  std::vector<uint8_t> sts(ns), tys(ns);
+  if (ns) {
+    HIPCHK(hipMemcpyAsync(ge.data(), ctx->ge.p, ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(le.data(), ctx->le.p, ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(sts.data(), ctx->st_s.p, ns, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(tys.data(), ctx->ty_s.p, ns, hipMemcpyDeviceToHost, ctx->st));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  // Same walk and accumulation order as the reference's wls_test.
+  double se_g = 0.0, se_l = 0.0;
+  for (int k = 0; k < ns && out->sampled < (uint64_t)samples; ++k) {
+    const uint8_t t = tys[k];
+    if (t == T_WALL || t == T_SPLASH) continue;
+    if (sts[k] == WLS_NO_WEIGHT) {
+      out->failed++;
+      continue;
+    }
+    if (t == T_NEARWALL || t == T_SURFACE) out->virtual_points++;
+    if (sts[k] == WLS_FAILED) {
+      out->failed++;
+      continue;
+    }
+    out->sampled++;
+    if (sts[k] == WLS_REGULARIZED) out->regularized++;
+    else out->ok++;
+    out->max_gradient_error = std::max(out->max_gradient_error, ge[k]);
+    out->max_laplacian_error = std::max(out->max_laplacian_error, le[k]);
+    se_g += ge[k] * ge[k];
+    se_l += le[k] * le[k];
+  }
+  if (out->sampled) {
+    out->rms_gradient_error = std::sqrt(se_g / out->sampled);
+    out->rms_laplacian_error = std::sqrt(se_l / out->sampled);
+  }
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_result* out) {
+  if (!ctx || !p) return LSMPS_E_ARG;
+  if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "PPE assembly needs neighbors");
+  if (p->xmode != LSMPS_XSTAR_QUADRATIC && p->xmode != LSMPS_XSTAR_RANDOM) return fail(ctx, LSMPS_E_ARG, "bad xmode");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = (int)ctx->n;
+  hipStream_t st = ctx->st;
+  HIPCHK(ctx->flag.ensure(n));
+  HIPCHK(ctx->pos.ensure((size_t)n + 1));
+  HIPCHK(ctx->map.ensure(n));
+  HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  launch_unknown_flags(ctx->type.p, n, ctx->flag.p, st);
+  launch_exclusive_scan(ctx->flag.p, n, ctx->pos.p, ctx->scan_scr.p, st);
+  if (int rc = check_launch(ctx, "unknown_scan")) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->host_small, ctx->pos.p + n, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const int total = ctx->host_small[0];
+  const int cap = std::max(0, std::min(p->max_unknowns, n));
+  const int m = std::min(total, cap);
+  HIPCHK(ctx->rev.ensure(m));
+  HIPCHK(ctx->rcnt.ensure(m));
+  HIPCHK(ctx->prow.ensure((size_t)m + 1));
+  HIPCHK(ctx->xe.ensure(m));
+  HIPCHK(ctx->b.ensure(m));
+  HIPCHK(ctx->diag.ensure(m));
+  const uint64_t nnz_cap = ctx->pairs + (uint64_t)m;
+  if (nnz_cap > (uint64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "PPE nonzeros beyond int32 indexing");
+  HIPCHK(ctx->pcol.ensure(nnz_cap));
+  HIPCHK(ctx->pval.ensure(nnz_cap));
+  HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)m + 1)));
+  launch_unknown_map(ctx->flag.p, ctx->pos.p, n, cap, ctx->map.p, ctx->rev.p, st);
+  if (p->xmode == LSMPS_XSTAR_RANDOM && m > 0) {
+    // Seeded manufactured solution: one mt19937 draw per unknown, in order.
+    std::vector<double> xs(m);
+    std::mt19937 rng(p->xseed);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    for (int k = 0; k < m; ++k) xs[k] = u(rng);
+    HIPCHK(hipMemcpyAsync(ctx->xe.p, xs.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  zero_counters(ctx, 16);
+  launch_ppe_row_count(ctx->x.p, ctx->y.p, ctx->z.p, ctx->row_ptr.p, ctx->col.p, ctx->map.p, ctx->rev.p, m, p->re,
+                       ctx->rcnt.p, st);
+  launch_exclusive_scan(ctx->rcnt.p, m, ctx->prow.p, ctx->scan_scr.p, st);
+  launch_ppe_row_fill(ctx->x.p, ctx->y.p, ctx->z.p, ctx->type.p, ctx->row_ptr.p, ctx->col.p, ctx->map.p, ctx->rev.p, m,
+                      p->re, p->shift, p->xmode, ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->xe.p, ctx->diag.p,
+                      ctx->counters.p, st);
+  launch_ppe_rhs(ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->xe.p, m, ctx->b.p, st);
+  if (int rc = check_launch(ctx, "ppe")) return rc;
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  unsigned long long c[2];
+  int nnz = 0;
+  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&nnz, ctx->prow.p + m, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->m = (uint64_t)m;
+  ctx->nnz = (uint64_t)nnz;
+  ctx->have_ppe = true;
+  ctx->have_solution = ctx->have_x0 = false;
+  ctx->graph.reset();
+  if (out) {
+    out->unknowns = (uint64_t)m;
+    out->nnz = (uint64_t)nnz;
+    out->dirichlet = (uint64_t)n - (uint64_t)m;
+    out->neumann = c[1];
+    out->failed_rows = c[0];
+    out->ms = elapsed(ctx);
+  }
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_upload_x0(lsmps_ctx* ctx, const double* x0, uint64_t n) {
+  if (!ctx || (n && !x0)) return LSMPS_E_ARG;
+  if (!ctx->have_ppe) return fail(ctx, LSMPS_E_STATE, "x0 needs an assembled PPE");
+  if (n != ctx->m) return fail(ctx, LSMPS_E_ARG, "x0 length differs from the unknown count");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ctx->x0.ensure(n));
+  if (n) HIPCHK(hipMemcpy(ctx->x0.p, x0, n * sizeof(double), hipMemcpyHostToDevice));
+  ctx->have_x0 = true;
+  return LSMPS_OK;
+  GUARD_END
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_iter, double tol, int flags, int chunk,
+                 lsmps_solve_result* out) {
+  hipStream_t st = ctx->st;
+  const bool with_replace = (flags & (SOLVE_RELIABLE | SOLVE_TRUE_CHECK)) != 0;
+  launch_solver_init<T>(B, x0, ctx->sc.p, max_iter, tol, flags, 0.1, st);
+  if (int rc = check_launch(ctx, "solver_init")) return rc;
+  // Capture `chunk` iterations once per buffer set; replay until the device
+  // state leaves RUNNING (kernels past convergence return at once).
+  GraphCache& G = ctx->graph;
+  const void* key[6] = {B.val, B.r, B.y, B.scale, B.b, B.val64};
+  const bool hit = G.exec && G.m == B.m && G.chunk == chunk && G.with_replace == (int)with_replace &&
+                   G.fp32 == (int)(sizeof(T) == 4) && std::equal(key, key + 6, G.key);
+  if (!hit) {
+    G.reset();
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < chunk; ++k) launch_solver_iteration<T>(B, k & 1, with_replace, ctx->sc.p, st);
+    hipError_t ec = hipStreamEndCapture(st, &G.graph);
+    if (ec != hipSuccess) return fail(ctx, LSMPS_E_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+    HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+    G.m = B.m;
+    G.chunk = chunk;
+    G.with_replace = with_replace;
+    G.fp32 = sizeof(T) == 4;
+    std::copy(key, key + 6, G.key);
+  }
+  const int max_launch = max_iter / chunk + 3;
+  HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int l = 0; l < max_launch && ctx->host_sc->state == SOLVE_RUNNING; ++l) {
+    HIPCHK(hipGraphLaunch(G.exec, st));
+    HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  launch_solver_final<T>(B, ctx->xe.p, ctx->xsol.p, ctx->sc.p, st);
+  if (int rc = check_launch(ctx, "solver_final")) return rc;
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const SolverScalars& s = *ctx->host_sc;
+  out->iterations = s.iterations;
+  out->converged = s.state == SOLVE_CONVERGED ? 1 : 0;
+  out->state = s.state;
+  out->replacements = s.n_replace;
+  out->initial_residual = s.init_res;
+  out->final_residual = s.final_res;
+  out->relative_l2_error = s.rel_l2;
+  out->max_error = s.max_err;
+  return LSMPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_result* out) {
+  if (!ctx || !p || !out) return LSMPS_E_ARG;
+  if (!ctx->have_ppe) return fail(ctx, LSMPS_E_STATE, "solve needs an assembled PPE");
+  if (p->use_x0 && !ctx->have_x0) return fail(ctx, LSMPS_E_STATE, "use_x0 without lsmps_upload_x0");
+  if (p->method < LSMPS_SOLVE_PLAIN || p->method > LSMPS_SOLVE_MIXED)
+    return fail(ctx, LSMPS_E_ARG, "unknown solve method");
+  int chunk = p->chunk > 0 ? p->chunk : 16;
+  chunk += chunk & 1;  // p and v ping-pong: an even count returns to parity 0
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  std::memset(out, 0, sizeof(*out));
+  const int m = (int)ctx->m;
+  const int64_t nnz = (int64_t)ctx->nnz;
+  hipStream_t st = ctx->st;
+  HIPCHK(ctx->xsol.ensure(m));
+  HIPCHK(ctx->yv.ensure(m));
+  HIPCHK(ctx->scale.ensure(m));
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  const double* x0 = p->use_x0 ? ctx->x0.p : nullptr;
+  int rc = LSMPS_OK;
+  if (p->method == LSMPS_SOLVE_MIXED) {
+    HIPCHK(ctx->val32.ensure(nnz));
+    HIPCHK(ctx->vec32.ensure(8 * (size_t)m));
+    launch_inv_diag(ctx->diag.p, m, ctx->scale.p, st);
+    launch_scale_matrix<float>(ctx->pcol.p, ctx->pval.p, ctx->scale.p, nnz, ctx->val32.p, st);
+    SolverBufs<float> B;
+    B.m = m;
+    B.rowp = ctx->prow.p;
+    B.col = ctx->pcol.p;
+    B.val = ctx->val32.p;
+    B.val64 = ctx->pval.p;
+    B.b = ctx->b.p;
+    B.scale = ctx->scale.p;
+    B.y = ctx->yv.p;
+    float* v = ctx->vec32.p;
+    B.r = v;
+    B.r0 = v + m;
+    B.p[0] = v + 2 * (size_t)m;
+    B.p[1] = v + 3 * (size_t)m;
+    B.v[0] = v + 4 * (size_t)m;
+    B.v[1] = v + 5 * (size_t)m;
+    B.s = v + 6 * (size_t)m;
+    B.t = v + 7 * (size_t)m;
+    B.partials = ctx->partials.p;
+    rc = run_bicgstab<float>(ctx, B, x0, p->max_iter, p->tol, SOLVE_RELIABLE | SOLVE_TRUE_CHECK, chunk, out);
+  } else {
+    const bool jac = p->method == LSMPS_SOLVE_JACOBI;
+    HIPCHK(ctx->vec64.ensure(8 * (size_t)m));
+    SolverBufs<double> B;
+    B.m = m;
+    B.rowp = ctx->prow.p;
+    B.col = ctx->pcol.p;
+    B.val64 = ctx->pval.p;
+    B.b = ctx->b.p;
+    if (jac) {
+      HIPCHK(ctx->val64s.ensure(nnz));
+      launch_inv_diag(ctx->diag.p, m, ctx->scale.p, st);
+      launch_scale_matrix<double>(ctx->pcol.p, ctx->pval.p, ctx->scale.p, nnz, ctx->val64s.p, st);
+      B.val = ctx->val64s.p;
+      B.scale = ctx->scale.p;
+    } else {
+      B.val = ctx->pval.p;
+      B.scale = nullptr;
+    }
+    B.y = ctx->yv.p;
+    double* v = ctx->vec64.p;
+    B.r = v;
+    B.r0 = v + m;
+    B.p[0] = v + 2 * (size_t)m;
+    B.p[1] = v + 3 * (size_t)m;
+    B.v[0] = v + 4 * (size_t)m;
+    B.v[1] = v + 5 * (size_t)m;
+    B.s = v + 6 * (size_t)m;
+    B.t = v + 7 * (size_t)m;
+    B.partials = ctx->partials.p;
+    rc = run_bicgstab<double>(ctx, B, x0, p->max_iter, p->tol, jac ? SOLVE_TRUE_CHECK : SOLVE_S_EXIT, chunk, out);
+  }
+  if (rc != LSMPS_OK) return rc;
+  out->ms = elapsed(ctx);
+  ctx->have_solution = true;
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_download_types(lsmps_ctx* ctx, uint8_t* type) {
+  if (!ctx || (ctx->n && !type)) return LSMPS_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  if (ctx->n) HIPCHK(hipMemcpyAsync(type, ctx->type.p, ctx->n, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return LSMPS_OK;
+}
+
+int lsmps_download_neighbors(lsmps_ctx* ctx, int32_t* row_ptr, int32_t* col) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "no neighbor lists");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (row_ptr) HIPCHK(hipMemcpyAsync(row_ptr, ctx->row_ptr.p, (ctx->n + 1) * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  if (col && ctx->pairs) HIPCHK(hipMemcpyAsync(col, ctx->col.p, ctx->pairs * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return LSMPS_OK;
+}
+
+int lsmps_download_wls(lsmps_ctx* ctx, double* grad3, double* lap, uint8_t* status) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (!ctx->have_wls) return fail(ctx, LSMPS_E_STATE, "no WLS results");
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t n = ctx->n;
+  if (grad3 && n) HIPCHK(hipMemcpyAsync(grad3, ctx->grad.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  if (lap && n) HIPCHK(hipMemcpyAsync(lap, ctx->lap.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  if (status && n) HIPCHK(hipMemcpyAsync(status, ctx->wstatus.p, n, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return LSMPS_OK;
+}
+
+int lsmps_download_ppe(lsmps_ctx* ctx, int32_t* row, int32_t* col, double* val, double* b, double* x_exact,
+                       int32_t* rev) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (!ctx->have_ppe) return fail(ctx, LSMPS_E_STATE, "no PPE");
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t m = ctx->m, nnz = ctx->nnz;
+  hipStream_t st = ctx->st;
+  if (row) HIPCHK(hipMemcpyAsync(row, ctx->prow.p, (m + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+  if (col && nnz) HIPCHK(hipMemcpyAsync(col, ctx->pcol.p, nnz * sizeof(int), hipMemcpyDeviceToHost, st));
+  if (val && nnz) HIPCHK(hipMemcpyAsync(val, ctx->pval.p, nnz * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (b && m) HIPCHK(hipMemcpyAsync(b, ctx->b.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (x_exact && m) HIPCHK(hipMemcpyAsync(x_exact, ctx->xe.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (rev && m) HIPCHK(hipMemcpyAsync(rev, ctx->rev.p, m * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return LSMPS_OK;
+}
+
+int lsmps_download_solution(lsmps_ctx* ctx, double* x) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (!ctx->have_solution) return fail(ctx, LSMPS_E_STATE, "no solution");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (x && ctx->m) HIPCHK(hipMemcpyAsync(x, ctx->xsol.p, ctx->m * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return LSMPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// CSR pressure-Poisson (PPE) assembly on gfx950.
+//
+// Semantics of the reference's assemble_ppe (src/lsmps.cpp:29-64):
+// unknowns are the first `cap` particles, in index order, of type Interior
+// or NearWall; row 0 is the identity pin; every other row holds, in
+// neighbor-list order, -a_ij = -w/(r*r) for each unknown neighbor with
+// w > 0, then the diagonal sum(a_ij) last (1 with a failed-row count when
+// it is not positive and finite); b = A x* summed in row order. Optional
+// extension: the diagonal is scaled by (1 + shift) after the check
+// (shift = 0 is the reference).
+//
+// Two passes (count, scan, fill), one thread per row, each row's
+// diagonal summed sequentially in list order with no contraction (this file
+// is built with -ffp-contract=off), so values, b and structure are
+// bit-identical to the reference's.
+#include "kernels.hpp"
+
+namespace lsmps_hip {
+
+__global__ __launch_bounds__(256) void k_unknown_flags(const uint8_t* __restrict__ type, int n, int* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (type[i] == T_INTERIOR || type[i] == T_NEARWALL) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_unknown_map(const int* __restrict__ flag, const int* __restrict__ pos, int n,
+                                                     int cap, int* __restrict__ map, int* __restrict__ rev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int p = pos[i];
+  const bool u = flag[i] && p < cap;
+  map[i] = u ? p : -1;
+  if (u) rev[p] = i;
+}
+
+// Weight test of one pair, as the reference computes it (src/lsmps.cpp:22,45).
+__device__ __forceinline__ bool pair_coef(double dx, double dy, double dz, double re, double* a) {
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  if (r <= 1e-12 || r >= re) return false;
+  const double t = re / r - 1.0;
+  const double w = t * t;
+  if (w <= 0.0) return false;
+  *a = w / (r * r);
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_ppe_row_count(const double* __restrict__ x, const double* __restrict__ y,
+                                                       const double* __restrict__ z, const int* __restrict__ nrow,
+                                                       const int* __restrict__ ncol, const int* __restrict__ map,
+                                                       const int* __restrict__ rev, int m, double re,
+                                                       int* __restrict__ cnt) {
+  const int rr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rr >= m) return;
+  if (rr == 0) {
+    cnt[0] = 1;
+    return;
+  }
+  const int i = rev[rr];
+  const double xi = x[i], yi = y[i], zi = z[i];
+  int c = 1;
+  for (int k = nrow[i]; k < nrow[i + 1]; ++k) {
+    const int j = ncol[k];
+    if (map[j] < 0) continue;
+    double a;
+    if (pair_coef(x[j] - xi, y[j] - yi, z[j] - zi, re, &a)) ++c;
+  }
+  cnt[rr] = c;
+}
+
+// counters: [0] failed rows, [1] NearWall rows except row 0.
+__global__ __launch_bounds__(256) void k_ppe_row_fill(const double* __restrict__ x, const double* __restrict__ y,
+                                                      const double* __restrict__ z, const uint8_t* __restrict__ type,
+                                                      const int* __restrict__ nrow, const int* __restrict__ ncol,
+                                                      const int* __restrict__ map, const int* __restrict__ rev, int m,
+                                                      double re, double shift, int xmode, const int* __restrict__ prow,
+                                                      int* __restrict__ pcol, double* __restrict__ pval,
+                                                      double* __restrict__ xe, double* __restrict__ diag_out,
+                                                      unsigned long long* __restrict__ counters) {
+  const int rr = blockIdx.x * blockDim.x + threadIdx.x;
+  int failed = 0, neumann = 0;
+  if (rr < m) {
+    const int i = rev[rr];
+    const double xi = x[i], yi = y[i], zi = z[i];
+    if (xmode == 0) xe[rr] = xi * xi + yi * yi;  // x* = x^2 + y^2 (src/lsmps.cpp:29)
+    int at = prow[rr];
+    if (rr == 0) {
+      pcol[at] = 0;
+      pval[at] = 1.0;
+      diag_out[0] = 1.0;
+    } else {
+      double d = 0.0;
+      for (int k = nrow[i]; k < nrow[i + 1]; ++k) {
+        const int j = ncol[k];
+        const int cj = map[j];
+        if (cj < 0) continue;
+        double a;
+        if (!pair_coef(x[j] - xi, y[j] - yi, z[j] - zi, re, &a)) continue;
+        d += a;
+        pcol[at] = cj;
+        pval[at] = -a;
+        ++at;
+      }
+      if (d <= 0 || !isfinite(d)) {
+        failed = 1;
+        d = 1.0;
+      }
+      if (shift != 0.0) d += shift * d;
+      pcol[at] = rr;
+      pval[at] = d;
+      diag_out[rr] = d;
+      neumann = type[i] == T_NEARWALL ? 1 : 0;
+    }
+  }
+  const unsigned long long fs = wave_sum((unsigned long long)failed), ns = wave_sum((unsigned long long)neumann);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    if (fs) atomicAdd(&counters[0], fs);
+    if (ns) atomicAdd(&counters[1], ns);
+  }
+}
+
+// b = A x* with each row summed left to right (src/lsmps.cpp:59-61).
+__global__ __launch_bounds__(256) void k_ppe_rhs(const int* __restrict__ prow, const int* __restrict__ pcol,
+                                                 const double* __restrict__ pval, const double* __restrict__ xe, int m,
+                                                 double* __restrict__ b) {
+  const int rr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rr >= m) return;
+  double s = 0.0;
+  for (int k = prow[rr]; k < prow[rr + 1]; ++k) s += pval[k] * xe[pcol[k]];
+  b[rr] = s;
+}
+
+static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+void launch_unknown_flags(const uint8_t* type, int n, int* flag, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_unknown_flags, dim3(blocks(n, 256)), dim3(256), 0, st, type, n, flag);
+}
+void launch_unknown_map(const int* flag, const int* pos, int n, int cap, int* map, int* rev, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_unknown_map, dim3(blocks(n, 256)), dim3(256), 0, st, flag, pos, n, cap, map, rev);
+}
+void launch_ppe_row_count(const double* x, const double* y, const double* z, const int* nrow, const int* ncol,
+                          const int* map, const int* rev, int m, double re, int* cnt, hipStream_t st) {
+  if (m > 0)
+    hipLaunchKernelGGL(k_ppe_row_count, dim3(blocks(m, 256)), dim3(256), 0, st, x, y, z, nrow, ncol, map, rev, m, re,
+                       cnt);
+}
+void launch_ppe_row_fill(const double* x, const double* y, const double* z, const uint8_t* type, const int* nrow,
+                         const int* ncol, const int* map, const int* rev, int m, double re, double shift, int xmode,
+                         const int* prow, int* pcol, double* pval, double* xe, double* diag,
+                         unsigned long long* counters, hipStream_t st) {
+  if (m > 0)
+    hipLaunchKernelGGL(k_ppe_row_fill, dim3(blocks(m, 256)), dim3(256), 0, st, x, y, z, type, nrow, ncol, map, rev, m,
+                       re, shift, xmode, prow, pcol, pval, xe, diag, counters);
+}
+void launch_ppe_rhs(const int* prow, const int* pcol, const double* pval, const double* xe, int m, double* b,
+                    hipStream_t st) {
+  if (m > 0) hipLaunchKernelGGL(k_ppe_rhs, dim3(blocks(m, 256)), dim3(256), 0, st, prow, pcol, pval, xe, m, b);
+}
+
+}  // namespace lsmps_hip

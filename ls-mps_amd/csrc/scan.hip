@@ -1,0 +1,107 @@
+// Exclusive prefix sum of int32 counts into offsets (CSR row pointers, cell
+// starts). Three launches: tile sums, one-block scan of the tile sums, tile
+// rescans. Each tile is 256 threads x 8 consecutive items.
+#include "kernels.hpp"
+
+namespace lsmps_hip {
+
+constexpr int SCAN_THREADS = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile_sums(const int* __restrict__ in, int64_t n,
+                                                                 int64_t* __restrict__ tile_sum) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  int64_t s = 0;
+  for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_THREADS) {
+    const int64_t i = base + k;
+    if (i < n) s += in[i];
+  }
+  __shared__ int64_t lds[SCAN_THREADS / WAVE];
+  s = wave_sum(s);
+  if ((threadIdx.x & (WAVE - 1)) == 0) lds[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int w = 0; w < SCAN_THREADS / WAVE; ++w) t += lds[w];
+    tile_sum[blockIdx.x] = t;
+  }
+}
+
+// One block: exclusive scan of the tile sums in place; writes the grand
+// total to out[n] and to *total.
+__global__ __launch_bounds__(1024) void k_scan_tile_offsets(int64_t* __restrict__ tile_sum, int ntiles,
+                                                            int* __restrict__ out_last, int64_t* __restrict__ total) {
+  __shared__ int64_t wsum[1024 / WAVE];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  for (int base = 0; base < ntiles; base += 1024) {
+    const int i = base + threadIdx.x;
+    int64_t v = i < ntiles ? tile_sum[i] : 0;
+    int64_t inc = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      int64_t u = __shfl_up(inc, o);
+      if (lane >= o) inc += u;
+    }
+    if (lane == WAVE - 1) wsum[wid] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t acc = 0;
+      for (int w = 0; w < 1024 / WAVE; ++w) {
+        int64_t t = wsum[w];
+        wsum[w] = acc;
+        acc += t;
+      }
+    }
+    __syncthreads();
+    const int64_t excl = carry + wsum[wid] + inc - v;
+    if (i < ntiles) tile_sum[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = excl + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *out_last = (int)carry;
+    *total = carry;
+  }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int* __restrict__ in, int64_t n,
+                                                             const int64_t* __restrict__ tile_off,
+                                                             int* __restrict__ out) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+  int tsum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    tsum += v[k];
+  }
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  int inc = wave_incl_scan(tsum);
+  __shared__ int wsum[SCAN_THREADS / WAVE];
+  if (lane == WAVE - 1) wsum[wid] = inc;
+  __syncthreads();
+  int wofs = 0;
+  for (int w = 0; w < wid; ++w) wofs += wsum[w];
+  int64_t run = tile_off[blockIdx.x] + wofs + inc - tsum;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = (int)run;
+    run += v[k];
+  }
+}
+
+size_t scan_scratch_elems(int64_t n) { return (size_t)((n + SCAN_TILE - 1) / SCAN_TILE) + 2; }
+
+// out[0..n] = exclusive prefix sums of in[0..n-1], out[n] = total; the
+// 64-bit total also lands in scratch[ntiles] for overflow checks.
+void launch_exclusive_scan(const int* in, int64_t n, int* out, int64_t* scratch, hipStream_t st) {
+  const int ntiles = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
+  if (ntiles > 0) hipLaunchKernelGGL(k_scan_tile_sums, dim3(ntiles), dim3(SCAN_THREADS), 0, st, in, n, scratch);
+  hipLaunchKernelGGL(k_scan_tile_offsets, dim3(1), dim3(1024), 0, st, scratch, ntiles, out + n, scratch + ntiles);
+  if (ntiles > 0) hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(SCAN_THREADS), 0, st, in, n, scratch, out);
+}
+
+}  // namespace lsmps_hip

@@ -1,0 +1,68 @@
+// Device-resident BiCGStab state and launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lsmps_hip {
+
+enum : int {
+  SOLVE_RUNNING = 0,
+  SOLVE_S_CONVERGED = 1,  // ||s|| met the tolerance; x update pending
+  SOLVE_CONVERGED = 2,
+  SOLVE_BREAKDOWN = 3,
+  SOLVE_MAXITER = 4,
+};
+enum : int {
+  SOLVE_S_EXIT = 1,      // stop on ||s|| as the reference does
+  SOLVE_RELIABLE = 2,    // replace the recurrence residual by b - A x (FP64) on 10x drops
+  SOLVE_TRUE_CHECK = 4,  // confirm convergence with the true FP64 residual
+};
+
+// Read by every block at kernel start, written only by the last block of
+// a kernel (after all others have finished), so no intra-kernel races.
+struct SolverScalars {
+  double rho, rho_prev, alpha, omega, beta;
+  double tol, bnorm, thresh, init_res;
+  double sn, rn, rn_max, delta;
+  double final_res, rel_l2, max_err;
+  int it, max_iter, state, iterations;
+  int flags, replace, n_replace, pad;
+  unsigned ticket[8];
+};
+
+constexpr unsigned SOLVER_MAX_GRID = 2048;  // 8 blocks of 256 per CU on 256 CUs
+
+template <typename T>
+struct SolverBufs {
+  int m = 0;
+  const int* rowp = nullptr;
+  const int* col = nullptr;
+  const T* val = nullptr;          // A S in precision T
+  const double* val64 = nullptr;   // A in FP64 (true residuals)
+  const double* b = nullptr;
+  const double* scale = nullptr;   // S (nullptr = identity)
+  double* y = nullptr;             // FP64 iterate of A S y = b
+  T* r = nullptr;
+  T* r0 = nullptr;
+  T* p[2] = {nullptr, nullptr};
+  T* v[2] = {nullptr, nullptr};
+  T* s = nullptr;
+  T* t = nullptr;
+  double* partials = nullptr;      // >= 3 * SOLVER_MAX_GRID doubles
+};
+
+unsigned solver_grid(int m);
+void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st);
+template <typename T>
+void launch_scale_matrix(const int* col, const double* val, const double* scale, int64_t nnz, T* out, hipStream_t st);
+template <typename T>
+void launch_solver_init(const SolverBufs<T>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,
+                        int flags, double delta, hipStream_t st);
+template <typename T>
+void launch_solver_iteration(const SolverBufs<T>& B, int parity, bool with_replace, SolverScalars* sc,
+                             hipStream_t st);
+template <typename T>
+void launch_solver_final(const SolverBufs<T>& B, const double* xe, double* x, SolverScalars* sc, hipStream_t st);
+
+}  // namespace lsmps_hip

@@ -317,7 +317,7 @@ uint8_t wls_one(const orc_state& s, const double* f, double re, double l0, doubl
   const double wsum = moments(s, f, re, l0, i, M, rhs);
   g[0] = g[1] = g[2] = 0.0;
   *lap = 0.0;
-  if (!(wsum > 0)) return ORC_WLS_FAILED;
+  if (wsum <= 0) return ORC_WLS_NO_WEIGHT;
   bool shifted = false;
   if (!chol_ladder(M, reg, shifted)) return ORC_WLS_FAILED;
   const std::array<double, NB> x = chol_solve(M, rhs);

@@ -83,7 +83,7 @@ void orc_get_types(const orc_state* s, uint8_t* type);
 void orc_set_types(orc_state* s, const uint8_t* type);
 
 /* Status codes of a per-particle WLS solve. */
-enum { ORC_WLS_OK = 0, ORC_WLS_REGULARIZED = 1, ORC_WLS_FAILED = 2, ORC_WLS_SKIPPED = 3 };
+enum { ORC_WLS_OK = 0, ORC_WLS_REGULARIZED = 1, ORC_WLS_FAILED = 2, ORC_WLS_SKIPPED = 3, ORC_WLS_NO_WEIGHT = 4 };
 
 /* WLS gradient and Laplacian of a field sampled at particles, for every
  * particle that is not Wall or Splash (src/lsmps.cpp:22-25,28). */

@@ -107,7 +107,7 @@ void ref_wls_test(uint64_t* counts5, double* err4) {
 
 // Per-particle gradient/Laplacian of the reference's test field computed
 // with the reference's own weight/basis/cholesky/solve; status 0 ok,
-// 1 regularized, 2 failed (weight sum or factorisation), 3 skipped type.
+// 1 regularized, 2 factorisation failed, 3 skipped type, 4 zero weight sum.
 void ref_wls_particles(const uint64_t* idx, uint64_t count, double* grad3, double* lap, uint8_t* status) {
   const double re = g.cfg.re_ratio * g.cfg.l0, l0 = g.cfg.l0;
   auto f = [](lsmps::Vec3 r) {
@@ -137,7 +137,7 @@ void ref_wls_particles(const uint64_t* idx, uint64_t count, double* grad3, doubl
       }
     }
     if (ws <= 0) {
-      status[s] = 2;
+      status[s] = 4;
       continue;
     }
     for (double& x : M) x /= ws;

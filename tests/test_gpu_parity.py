@@ -1,0 +1,235 @@
+"""GPU parity of the HIP path against the reference's golden vectors and
+the CPU oracle. Every call goes through the C ABI (lsmps_amd -> liblsmps_hip.so).
+
+Bar: bit-exact for neighbor lists, types, PPE structure/values/b and the FP64
+WLS operator; BiCGStab by tolerance (the GPU's parallel dot products round
+differently from the reference's sequential ones, so the iteration count may
+differ by a few): same convergence flag, iterations within 10%, final
+relative L2 error against x* within a factor 2 of the reference's.
+"""
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+from conftest import SMALL_FIXTURES, load_fixture
+
+import lsmps_amd as L
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def pipe():
+    p = L.Pipeline(0)
+    yield p
+    p.close()
+
+
+def check_solve(st, ref, what):
+    assert st["converged"] == ref["converged"], (what, st, ref)
+    if ref["converged"]:
+        assert abs(st["iterations"] - ref["iterations"]) <= max(3, 0.1 * ref["iterations"]), (what, st, ref)
+        r = st["relative_l2_error"] / max(ref["relative_l2_error"], 1e-300)
+        assert 0.5 <= r <= 2.0, (what, st["relative_l2_error"], ref["relative_l2_error"])
+
+
+@pytest.mark.parametrize("name", SMALL_FIXTURES)
+def test_small_fixture_bitexact(pipe, name):
+    d = load_fixture(name)
+    m = d["meta"]
+    re = m["re_ratio"] * m["l0"]
+    say("fixture", name, "n", len(d["xyz"]))
+    pipe.upload(d["xyz"], d["type_in"])
+    cs = pipe.neighbors(re, m["max_neighbors"])
+    row, col = pipe.neighbor_lists()
+    assert np.array_equal(row, d["nbr_row"]) and np.array_equal(col, d["nbr_col"])
+    assert cs["truncated"] == m["truncated"]
+    pipe.classify(m["l0"])
+    assert np.array_equal(pipe.types(), d["type_out"])
+    pipe.set_field(L.REFERENCE_FIELD)
+    pipe.wls(re, m["l0"], L.WLS_FP64)
+    g, lap, st = pipe.wls_results()
+    assert np.array_equal(st, d["wls_status"])
+    assert np.array_equal(g, d["wls_grad"]) and np.array_equal(lap, d["wls_lap"])
+    ws = pipe.wls_sampled_stats(m["wls_samples"])
+    for k, v in m["wls"].items():
+        assert ws[k] == v, (k, ws[k], v)
+    pr = pipe.assemble(re, m["max_unknowns"])
+    for k, v in m["ppe"].items():
+        assert pr[k] == v, k
+    A = pipe.ppe()
+    for k in ("row", "col", "val", "b", "x_exact"):
+        assert np.array_equal(A[k], d["ppe_" + k]), k
+    if pr["unknowns"]:
+        st = pipe.solve(L.SOLVE_PLAIN, m["max_iter"], m["tol"])
+        check_solve(st, m["solve"], name)
+
+
+def test_reference_configs(pipe, ref_configs):
+    for name, rec in ref_configs.items():
+        c = rec["config"]
+        l0 = float(c["l0"])
+        re = 2.1 * l0
+        say("config", name)
+        xyz, t = L.generate_case(c["case"], int(c["particles"]), int(c["seed"]), l0, int(c.get("film_layers", 4)))
+        pipe.upload(xyz, t)
+        cs = pipe.neighbors(re, int(c["max_neighbors"]))
+        assert cs["pairs"] == rec["pairs"] and cs["truncated"] == rec["truncated"]
+        row, col = pipe.neighbor_lists()
+        assert sha(row) == rec["sha"]["nbr_row"] and sha(col) == rec["sha"]["nbr_col"], name
+        assert np.bincount(np.diff(row)).tolist() == rec["neighbor_count_hist"]
+        ts = pipe.classify(l0)
+        assert ts["count"] == rec["type_counts"]
+        assert sha(pipe.types()) == rec["sha"]["type_out"]
+        pipe.set_field(L.REFERENCE_FIELD)
+        pipe.wls(re, l0, L.WLS_FP64)
+        ws = pipe.wls_sampled_stats(int(c["wls_samples"]))
+        for k, v in rec["wls"].items():
+            assert ws[k] == v, (name, k, ws[k], v)
+        pr = pipe.assemble(re, int(c["ppe_max_unknowns"]))
+        for k, v in rec["ppe"].items():
+            assert pr[k] == v, (name, k)
+        A = pipe.ppe()
+        for k in ("row", "col", "val", "b", "x_exact"):
+            assert sha(A[k]) == rec["sha"]["ppe_" + k], (name, k)
+        st = pipe.solve(L.SOLVE_PLAIN, int(c["ppe_max_iter"]), float(c["ppe_tol"]))
+        check_solve(st, rec["solve"], name)
+
+
+def test_neighbors_against_oracle_random_sets(pipe):
+    rng = np.random.default_rng(11)
+    for n, maxn, spread in [(1, 8, 0.1), (2, 8, 0.001), (700, 96, 0.1), (3000, 10, 0.12), (5000, 40, 0.3)]:
+        xyz = rng.uniform(-spread, spread, (n, 3))
+        if n > 100:
+            xyz[:20] = xyz[20:40]  # coincident pairs
+        t = rng.integers(0, 5, n).astype(np.uint8)
+        say("random set", n, maxn)
+        pipe.upload(xyz, t)
+        cs = pipe.neighbors(0.021, maxn)
+        row, col = pipe.neighbor_lists()
+        o = O.Oracle(xyz, t)
+        pairs, trunc = o.neighbors(0.021, maxn)
+        orow, ocol = o.get_neighbors()
+        assert np.array_equal(row, orow) and np.array_equal(col, ocol)
+        assert cs["truncated"] == trunc
+        pipe.classify(0.01)
+        o.classify(0.01)
+        assert np.array_equal(pipe.types(), o.types())
+
+
+def test_empty_input(pipe):
+    pipe.upload(np.zeros((0, 3)), np.zeros(0, np.uint8))
+    cs = pipe.neighbors(0.021, 96)
+    assert cs["pairs"] == 0
+    pipe.classify(0.01)
+    pr = pipe.assemble(0.021, 100)
+    assert pr["unknowns"] == 0 and pr["nnz"] == 0
+
+
+def test_wls_other_field_and_mixed(pipe):
+    """FP64 operator reproduces the C1 quadratic; FP32 mixed mode close."""
+    xyz, t = L.generate_case("bulk_lattice", 20000, 3, 0.01)
+    say("wls fields", len(xyz))
+    pipe.upload(xyz, t)
+    pipe.neighbors(0.021, 96)
+    pipe.classify(0.01)
+    pipe.set_field(L.C1_FIELD)
+    r64 = pipe.wls(0.021, 0.01, L.WLS_FP64)
+    g64, l64, s64 = pipe.wls_results()
+    ok = s64 <= 1
+    ge = L.cases.field_gradient(xyz, L.C1_FIELD)
+    assert np.sqrt(np.mean((g64[ok] - ge[ok]) ** 2)) < 1e-10
+    assert np.sqrt(np.mean((l64[ok] - 4.0) ** 2)) < 1e-8
+    # Oracle agreement (bit-exact FP64).
+    o = O.Oracle(xyz, t)
+    o.neighbors(0.021, 96)
+    o.classify(0.01)
+    og, ol, ost = o.wls_all(L.cases.field_values(xyz, L.C1_FIELD), 0.021, 0.01)
+    f_dev = L.cases.field_values(xyz, L.C1_FIELD)
+    assert np.array_equal(ost, s64)
+    assert np.array_equal(og, g64) and np.array_equal(ol, l64), "FP64 WLS must be bit-exact"
+    assert f_dev.shape[0] == len(xyz)
+    r32 = pipe.wls(0.021, 0.01, L.WLS_MIXED)
+    g32, l32, s32 = pipe.wls_results()
+    say("mixed wls", r32)
+    assert np.array_equal(s32 == L.WLS_SKIPPED, s64 == L.WLS_SKIPPED)
+    okm = ok & (s32 <= 1)
+    # FP32 moments: relative accuracy ~1e-5 of the field scale.
+    assert np.sqrt(np.mean((g32[okm] - ge[okm]) ** 2)) < 1e-3
+    assert np.sqrt(np.mean((l32[okm] - 4.0) ** 2)) < 5e-2
+
+
+def test_solvers_reach_tolerance_and_agree(pipe):
+    xyz, t = L.generate_case("bulk_lattice", 50000, 7, 0.01)
+    say("solvers C1", len(xyz))
+    pipe.upload(xyz, t)
+    pipe.neighbors(0.021, 96)
+    pipe.classify(0.01)
+    pr = pipe.assemble(0.021, 6000, L.XSTAR_RANDOM, 5)
+    A = pipe.ppe()
+    o = O.Oracle(xyz, t)
+    o.neighbors(0.021, 96)
+    o.classify(0.01)
+    o.assemble(0.021, 6000, 1, 5)
+    B = o.get_ppe()
+    for k in ("row", "col", "val", "b", "x_exact"):
+        assert np.array_equal(A[k], B[k]), k
+    ost, ox = o.solve(20000, 1e-10, precond=1)
+    bnorm = np.linalg.norm(A["b"])
+    for method in (L.SOLVE_JACOBI, L.SOLVE_MIXED, L.SOLVE_PLAIN):
+        st = pipe.solve(method, 20000, 1e-10)
+        x = pipe.solution()
+        say("method", method, st)
+        assert st["converged"] == 1
+        # True residual recomputed on the host in FP64.
+        n = len(A["b"])
+        ax = np.zeros(n)
+        rows = np.repeat(np.arange(n), np.diff(A["row"]))
+        np.add.at(ax, rows, A["val"] * x[A["col"]])
+        res = np.linalg.norm(ax - A["b"])
+        assert res <= 1e-10 * max(1.0, bnorm) * (1.0 if method != L.SOLVE_PLAIN else 10.0)
+        assert np.linalg.norm(x - ox) <= 1e-6 * np.linalg.norm(ox)
+        if method == L.SOLVE_JACOBI:
+            assert abs(st["iterations"] - ost["iterations"]) <= max(5, 0.15 * ost["iterations"])
+
+
+def test_solver_max_iter_and_breakdown_paths(pipe):
+    d = load_fixture("film_1200")
+    m = d["meta"]
+    re = m["re_ratio"] * m["l0"]
+    pipe.upload(d["xyz"], d["type_in"])
+    pipe.neighbors(re, m["max_neighbors"])
+    pipe.classify(m["l0"])
+    pipe.assemble(re, m["max_unknowns"])
+    st = pipe.solve(L.SOLVE_PLAIN, 1, 1e-7)
+    assert st["converged"] == 0 and st["iterations"] == 1
+    st = pipe.solve(L.SOLVE_PLAIN, 0, 1e-7)
+    assert st["converged"] == 0 and st["iterations"] == 0
+
+
+def test_deterministic(pipe):
+    xyz, t = L.generate_case("rain_injection", 30000, 2, 0.01)
+    outs = []
+    for _ in range(2):
+        pipe.upload(xyz, t)
+        pipe.neighbors(0.021, 96)
+        pipe.classify(0.01)
+        pipe.set_field(L.REFERENCE_FIELD)
+        pipe.wls(0.021, 0.01, L.WLS_MIXED)
+        g, lap, s = pipe.wls_results()
+        pipe.assemble(0.021, 10 ** 9)
+        st = pipe.solve(L.SOLVE_MIXED, 5000, 1e-8)
+        outs.append((g.copy(), lap.copy(), st["iterations"], pipe.solution().copy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2] and np.array_equal(outs[0][3], outs[1][3])
