@@ -102,8 +102,7 @@ def main():
     p.set_field(L.C1_FIELD)
 
     def step():
-        cs = p.neighbors(re, maxn)
-        ts = p.classify(l0)
+        cs, ts = p.neighbors_classify(re, maxn, l0)
         wr = p.wls(re, l0, L.WLS_MIXED)
         return cs, ts, wr
 

@@ -35,7 +35,13 @@ enum {
 /* Particle types, numbered as the reference's ParticleType (include/lsmps.hpp:8). */
 enum { LSMPS_INTERIOR = 0, LSMPS_FREE_SURFACE = 1, LSMPS_SPLASH = 2, LSMPS_NEAR_WALL = 3, LSMPS_WALL = 4 };
 /* Per-particle WLS status. */
-enum { LSMPS_WLS_OK = 0, LSMPS_WLS_REGULARIZED = 1, LSMPS_WLS_FAILED = 2, LSMPS_WLS_SKIPPED = 3 };
+enum {
+  LSMPS_WLS_OK = 0,
+  LSMPS_WLS_REGULARIZED = 1, /* factorised after a diagonal shift of the ladder */
+  LSMPS_WLS_FAILED = 2,      /* every rung of the ladder failed */
+  LSMPS_WLS_SKIPPED = 3,     /* Wall or Splash particle */
+  LSMPS_WLS_NO_WEIGHT = 4,   /* weight sum <= 0 (no usable neighbor) */
+};
 
 typedef struct lsmps_ctx lsmps_ctx;
 
@@ -67,12 +73,20 @@ typedef struct {
  * visiting order, truncated to max_neighbors (<= 1024). */
 int lsmps_build_neighbors(lsmps_ctx* ctx, double re, int32_t max_neighbors, lsmps_cnl_stats* out);
 
+/* Neighbors and classification in one pass over the candidates (the
+ * classification reads exactly the offsets the neighbor test computes):
+ * replaces neighbors() + the classify loop of run_benchmark
+ * (src/lsmps.cpp:26-27,73). types->ms is 0 (time is in out->ms). */
+typedef struct lsmps_type_stats lsmps_type_stats;
+int lsmps_build_neighbors_classify(lsmps_ctx* ctx, double re, int32_t max_neighbors, double l0,
+                                   lsmps_cnl_stats* out, lsmps_type_stats* types);
+
 /* ---- classification: replaces classify() applied in run_benchmark
  * (src/lsmps.cpp:27,73). Overwrites the device types in place. ---------- */
-typedef struct {
+struct lsmps_type_stats {
   uint64_t count[5]; /* per type, in LSMPS_* order */
   float ms;
-} lsmps_type_stats;
+};
 int lsmps_classify(lsmps_ctx* ctx, double l0, lsmps_type_stats* out);
 
 /* ---- WLS 9x9 operator: per-particle form of wls_test (src/lsmps.cpp:22-25,28) */
@@ -117,7 +131,7 @@ enum { LSMPS_XSTAR_QUADRATIC = 0, LSMPS_XSTAR_RANDOM = 1 };
 typedef struct {
   double re;
   int32_t max_unknowns; /* Config::ppe_max_unknowns */
-  int32_t xmode;        /* x* = x^2 + y^2 (reference) or U(-1,1) from mt19937(xseed) */
+  int32_t xmode;        /* x* = x^2 + y^2 (reference) or U(-1,1) from a counter hash of (xseed, row) */
   uint32_t xseed;
   double shift;         /* diagonal *= 1 + shift after assembly; 0 = reference */
 } lsmps_ppe_params;

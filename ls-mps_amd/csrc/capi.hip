@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <random>
@@ -74,27 +76,29 @@ struct lsmps_ctx {
   DBuf<double> xyz_in, x, y, z;
   DBuf<uint8_t> type;
 
-  // cell list and neighbor CSR
+  // cell list: particles in cell-major ("sorted") order and the ELL
+  // neighbor table in sorted-slot indices; CSR only for downloads
   bool have_cnl = false;
   double re = 0.0;
   int maxn = 0, gx = 0, gy = 0, gz = 0;
   uint64_t pairs = 0;
-  DBuf<int> cx, cy, cz, box, cell, slot, ccount, cstart, perm, ncount, row_ptr, col, hist;
+  DBuf<int> cx, cy, cz, box, cell, slot, ccount, cstart, perm, iperm, scell, nbr, cnt, cnt_orig, row_ptr, col, hist;
   DBuf<double> sx, sy, sz;
+  DBuf<uint8_t> type_s;
   DBuf<int64_t> scan_scr;
   DBuf<unsigned long long> counters;  // 16 general-purpose counters
 
-  // field and WLS
+  // field and WLS (results in sorted order; original order on download)
   bool have_field = false, field_quadratic = false, have_wls = false;
   QuadField qf{};
-  DBuf<double> f, grad, lap, ge, le;
-  DBuf<uint8_t> wstatus, st_s, ty_s;
+  DBuf<double> f, fs, grad_s, lap_s, grad, lap, ge, le;
+  DBuf<uint8_t> wstatus_s, wstatus, st_s, ty_s;
   DBuf<int> redo, nredo;
 
   // PPE
   bool have_ppe = false;
   uint64_t m = 0, nnz = 0;
-  DBuf<int> flag, pos, map, rev, rcnt, prow, pcol;
+  DBuf<int> flag, pos, smap, rev, rcnt, prow, pcol;
   DBuf<double> pval, b, xe, diag;
 
   // solver
@@ -197,15 +201,19 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->st) (void)hipStreamSynchronize(ctx->st);
   ctx->graph.reset();
-  DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x, &ctx->y, &ctx->z, &ctx->sx, &ctx->sy, &ctx->sz, &ctx->f,
-                       &ctx->grad, &ctx->lap, &ctx->ge, &ctx->le, &ctx->pval, &ctx->b, &ctx->xe, &ctx->diag,
-                       &ctx->scale, &ctx->yv, &ctx->xsol, &ctx->x0, &ctx->val64s, &ctx->vec64, &ctx->partials};
+  DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x, &ctx->y, &ctx->z, &ctx->sx, &ctx->sy, &ctx->sz, &ctx->f, &ctx->fs,
+                       &ctx->grad_s, &ctx->lap_s, &ctx->grad, &ctx->lap, &ctx->ge, &ctx->le, &ctx->pval, &ctx->b,
+                       &ctx->xe, &ctx->diag, &ctx->scale, &ctx->yv, &ctx->xsol, &ctx->x0, &ctx->val64s, &ctx->vec64,
+                       &ctx->partials};
   for (auto* p : d) p->release();
   DBuf<int>* i[] = {&ctx->cx, &ctx->cy, &ctx->cz, &ctx->box, &ctx->cell, &ctx->slot, &ctx->ccount, &ctx->cstart,
-                    &ctx->perm, &ctx->ncount, &ctx->row_ptr, &ctx->col, &ctx->hist, &ctx->redo, &ctx->nredo,
-                    &ctx->flag, &ctx->pos, &ctx->map, &ctx->rev, &ctx->rcnt, &ctx->prow, &ctx->pcol};
+                    &ctx->perm, &ctx->iperm, &ctx->scell, &ctx->nbr, &ctx->cnt, &ctx->cnt_orig, &ctx->row_ptr,
+                    &ctx->col, &ctx->hist, &ctx->redo, &ctx->nredo, &ctx->flag, &ctx->pos, &ctx->smap, &ctx->rev,
+                    &ctx->rcnt, &ctx->prow, &ctx->pcol};
   for (auto* p : i) p->release();
   ctx->type.release();
+  ctx->type_s.release();
+  ctx->wstatus_s.release();
   ctx->wstatus.release();
   ctx->st_s.release();
   ctx->ty_s.release();
@@ -257,25 +265,89 @@ int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   GUARD_END
 }
 
-int lsmps_build_neighbors(lsmps_ctx* ctx, double re, int32_t maxn, lsmps_cnl_stats* out) {
-  if (!ctx) return LSMPS_E_ARG;
+}  // extern "C"
+
+namespace {
+
+void fill_cnl_stats(lsmps_ctx* ctx, const std::vector<int>& hist, unsigned long long trunc, lsmps_cnl_stats* out) {
+  const int n = (int)ctx->n;
+  std::memset(out, 0, sizeof(*out));
+  uint64_t pairs = 0;
+  for (size_t k = 0; k < hist.size(); ++k) pairs += (uint64_t)k * (uint64_t)hist[k];
+  out->pairs = pairs;
+  out->truncated = trunc;
+  out->grid[0] = ctx->gx;
+  out->grid[1] = ctx->gy;
+  out->grid[2] = ctx->gz;
+  if (n > 0) {
+    double sum = 0.0;
+    for (size_t k = 0; k < hist.size(); ++k) {
+      sum += (double)k * hist[k];
+      if (hist[k]) out->max_count = (int32_t)k;
+    }
+    out->mean = sum / n;
+    // Quantile as run_benchmark takes it: sorted[(size_t)(q*(n-1))].
+    auto quant = [&](double q) {
+      const uint64_t idx = (uint64_t)(q * (double)(n - 1));
+      uint64_t cum = 0;
+      for (size_t k = 0; k < hist.size(); ++k) {
+        cum += (uint64_t)hist[k];
+        if (cum > idx) return (int32_t)k;
+      }
+      return (int32_t)(hist.size() - 1);
+    };
+    out->p50 = quant(.5);
+    out->p90 = quant(.9);
+    out->p99 = quant(.99);
+  }
+}
+
+CnlBufs cnl_bufs(lsmps_ctx* ctx) {
+  CnlBufs B;
+  B.gx = ctx->gx;
+  B.gy = ctx->gy;
+  B.gz = ctx->gz;
+  B.x = ctx->x.p;
+  B.y = ctx->y.p;
+  B.z = ctx->z.p;
+  B.cx = ctx->cx.p;
+  B.cy = ctx->cy.p;
+  B.cz = ctx->cz.p;
+  B.cell = ctx->cell.p;
+  B.slot = ctx->slot.p;
+  B.ccount = ctx->ccount.p;
+  B.cstart = ctx->cstart.p;
+  B.scan_scratch = ctx->scan_scr.p;
+  B.perm = ctx->perm.p;
+  B.iperm = ctx->iperm.p;
+  B.scell = ctx->scell.p;
+  B.sx = ctx->sx.p;
+  B.sy = ctx->sy.p;
+  B.sz = ctx->sz.p;
+  B.nbr = ctx->nbr.p;
+  B.cnt = ctx->cnt.p;
+  B.cnt_orig = ctx->cnt_orig.p;
+  return B;
+}
+
+// Cell sort + one neighbor pass (+ classification when classify).
+int build_cell_list(lsmps_ctx* ctx, double re, int32_t maxn, bool classify, double l0, lsmps_cnl_stats* out,
+                    lsmps_type_stats* tout) {
   if (!(re > 0) || !std::isfinite(re)) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
   if (maxn < 0 || maxn > 1024) return fail(ctx, LSMPS_E_LIMIT, "max_neighbors must be in [0, 1024]");
-  GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   const int n = (int)ctx->n;
   hipStream_t st = ctx->st;
-  HIPCHK(ctx->cx.ensure(n));
-  HIPCHK(ctx->cy.ensure(n));
-  HIPCHK(ctx->cz.ensure(n));
-  HIPCHK(ctx->cell.ensure(n));
-  HIPCHK(ctx->slot.ensure(n));
-  HIPCHK(ctx->perm.ensure(n));
-  HIPCHK(ctx->ncount.ensure(n));
-  HIPCHK(ctx->row_ptr.ensure((size_t)n + 1));
+  if ((uint64_t)n * (uint64_t)std::max(1, maxn) > ((uint64_t)1 << 40))
+    return fail(ctx, LSMPS_E_LIMIT, "neighbor table too large");
+  DBuf<int>* per_particle[] = {&ctx->cx,   &ctx->cy,    &ctx->cz,    &ctx->cell, &ctx->slot,
+                               &ctx->perm, &ctx->iperm, &ctx->scell, &ctx->cnt,  &ctx->cnt_orig};
+  for (auto* b : per_particle) HIPCHK(b->ensure(n));
   HIPCHK(ctx->sx.ensure(n));
   HIPCHK(ctx->sy.ensure(n));
   HIPCHK(ctx->sz.ensure(n));
+  HIPCHK(ctx->type_s.ensure(n));
+  HIPCHK(ctx->nbr.ensure((size_t)n * (size_t)std::max(1, maxn)));
   HIPCHK(ctx->hist.ensure((size_t)maxn + 1));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
@@ -300,70 +372,54 @@ int lsmps_build_neighbors(lsmps_ctx* ctx, double re, int32_t maxn, lsmps_cnl_sta
   HIPCHK(ctx->ccount.ensure((size_t)ncells));
   HIPCHK(ctx->cstart.ensure((size_t)ncells + 1));
   HIPCHK(ctx->scan_scr.ensure(std::max(scan_scratch_elems(ncells), scan_scratch_elems((int64_t)n + 1))));
-  launch_cell_sort(ctx->cx.p, ctx->cy.p, ctx->cz.p, n, bx, ctx->gx, ctx->gy, ctx->gz, ctx->cell.p, ctx->slot.p,
-                   ctx->ccount.p, ctx->cstart.p, ctx->scan_scr.p, ctx->perm.p, ctx->x.p, ctx->y.p, ctx->z.p, ctx->sx.p,
-                   ctx->sy.p, ctx->sz.p, st);
+  const CnlBufs B = cnl_bufs(ctx);
+  launch_cell_sort(B, n, bx, st);
+  launch_gather_types(ctx->perm.p, ctx->type.p, n, ctx->type_s.p, st);
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
   zero_counters(ctx, 16);
   launch_fill_int(ctx->hist.p, maxn + 1, 0, st);
-  const double re2 = re * re;
-  launch_nbr_count(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->perm.p, ctx->cstart.p, ctx->cell.p, n, ctx->gx, ctx->gy,
-                   ctx->gz, re2, maxn, ctx->ncount.p, ctx->counters.p, ctx->hist.p, st);
-  if (int rc = check_launch(ctx, "nbr_count")) return rc;
-  launch_exclusive_scan(ctx->ncount.p, n, ctx->row_ptr.p, ctx->scan_scr.p, st);
-  if (int rc = check_launch(ctx, "scan")) return rc;
-  const int ntiles = (int)((n + 2047) / 2048);
-  int64_t* hs = reinterpret_cast<int64_t*>(ctx->host_small);
-  HIPCHK(hipMemcpyAsync(hs, ctx->scan_scr.p + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  const int64_t total = hs[0];
-  if (total > (int64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "neighbor pairs beyond int32 indexing");
-  HIPCHK(ctx->col.ensure((size_t)total));
-  launch_nbr_fill(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->perm.p, ctx->cstart.p, ctx->cell.p, n, ctx->gx, ctx->gy,
-                  ctx->gz, re2, maxn, ctx->row_ptr.p, ctx->col.p, st);
-  if (int rc = check_launch(ctx, "nbr_fill")) return rc;
+  launch_cnl_pass(B, n, re * re, maxn, classify, 1.5 * l0, ctx->type.p, ctx->type_s.p, ctx->counters.p, ctx->hist.p,
+                  ctx->counters.p + 8, st);
+  if (int rc = check_launch(ctx, "cnl_pass")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
   std::vector<int> hist((size_t)maxn + 1);
-  unsigned long long trunc = 0;
+  unsigned long long c[16];
   HIPCHK(hipMemcpyAsync(hist.data(), ctx->hist.p, hist.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&trunc, ctx->counters.p, sizeof(trunc), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  lsmps_cnl_stats cs;
+  fill_cnl_stats(ctx, hist, c[0], &cs);
+  if (cs.pairs > (uint64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "neighbor pairs beyond int32 indexing");
+  cs.ms = elapsed(ctx);
   ctx->re = re;
   ctx->maxn = maxn;
-  ctx->pairs = (uint64_t)total;
+  ctx->pairs = cs.pairs;
   ctx->have_cnl = true;
   ctx->have_wls = ctx->have_ppe = ctx->have_solution = false;
-  if (out) {
-    std::memset(out, 0, sizeof(*out));
-    out->pairs = (uint64_t)total;
-    out->truncated = trunc;
-    out->grid[0] = ctx->gx;
-    out->grid[1] = ctx->gy;
-    out->grid[2] = ctx->gz;
-    out->ms = elapsed(ctx);
-    if (n > 0) {
-      double sum = 0.0;
-      for (size_t k = 0; k < hist.size(); ++k) {
-        sum += (double)k * hist[k];
-        if (hist[k]) out->max_count = (int32_t)k;
-      }
-      out->mean = sum / n;
-      // Quantile as run_benchmark takes it: sorted[(size_t)(q*(n-1))].
-      auto quant = [&](double q) {
-        const uint64_t idx = (uint64_t)(q * (double)(n - 1));
-        uint64_t cum = 0;
-        for (size_t k = 0; k < hist.size(); ++k) {
-          cum += (uint64_t)hist[k];
-          if (cum > idx) return (int32_t)k;
-        }
-        return (int32_t)(hist.size() - 1);
-      };
-      out->p50 = quant(.5);
-      out->p90 = quant(.9);
-      out->p99 = quant(.99);
-    }
+  if (out) *out = cs;
+  if (tout) {
+    for (int k = 0; k < 5; ++k) tout->count[k] = c[8 + k];
+    tout->ms = 0.0f;  // fused into the neighbor pass
   }
   return LSMPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsmps_build_neighbors(lsmps_ctx* ctx, double re, int32_t maxn, lsmps_cnl_stats* out) {
+  if (!ctx) return LSMPS_E_ARG;
+  GUARD_BEGIN
+  return build_cell_list(ctx, re, maxn, false, 0.0, out, nullptr);
+  GUARD_END
+}
+
+int lsmps_build_neighbors_classify(lsmps_ctx* ctx, double re, int32_t maxn, double l0, lsmps_cnl_stats* out,
+                                   lsmps_type_stats* types) {
+  if (!ctx) return LSMPS_E_ARG;
+  GUARD_BEGIN
+  return build_cell_list(ctx, re, maxn, true, l0, out, types);
   GUARD_END
 }
 
@@ -375,8 +431,7 @@ int lsmps_classify(lsmps_ctx* ctx, double l0, lsmps_type_stats* out) {
   hipStream_t st = ctx->st;
   HIPCHK(hipEventRecord(ctx->e0, st));
   zero_counters(ctx, 16);
-  launch_classify(ctx->x.p, ctx->y.p, ctx->z.p, ctx->row_ptr.p, ctx->col.p, (int)ctx->n, 1.5 * l0, ctx->type.p,
-                  ctx->counters.p, st);
+  launch_classify(cnl_bufs(ctx), (int)ctx->n, ctx->maxn, 1.5 * l0, ctx->type.p, ctx->type_s.p, ctx->counters.p, st);
   if (int rc = check_launch(ctx, "classify")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
   unsigned long long c[5];
@@ -395,12 +450,8 @@ int lsmps_set_field_quadratic(lsmps_ctx* ctx, const lsmps_quad_field* q) {
   if (!ctx || !q) return LSMPS_E_ARG;
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
-  HIPCHK(ctx->f.ensure(ctx->n));
   std::memcpy(ctx->qf.c, q->c, sizeof(ctx->qf.c));
   ctx->qf.lap_exact = q->lap_exact;
-  launch_field_quadratic(ctx->x.p, ctx->y.p, ctx->z.p, (int)ctx->n, ctx->qf, ctx->f.p, ctx->st);
-  if (int rc = check_launch(ctx, "field")) return rc;
-  HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->have_field = true;
   ctx->field_quadratic = true;
   ctx->have_wls = false;
@@ -433,19 +484,27 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   HIPCHK(hipSetDevice(ctx->device));
   const int n = (int)ctx->n;
   hipStream_t st = ctx->st;
-  HIPCHK(ctx->grad.ensure(3 * (size_t)n));
-  HIPCHK(ctx->lap.ensure(n));
-  HIPCHK(ctx->wstatus.ensure(n));
+  HIPCHK(ctx->grad_s.ensure(3 * (size_t)n));
+  HIPCHK(ctx->lap_s.ensure(n));
+  HIPCHK(ctx->wstatus_s.ensure(n));
+  HIPCHK(ctx->fs.ensure(n));
   if (p->precision == LSMPS_WLS_MIXED) HIPCHK(ctx->redo.ensure(n));
   HIPCHK(hipEventRecord(ctx->e0, st));
+  // Field at the sorted slots: evaluated there for a quadratic, gathered for values.
+  if (ctx->field_quadratic)
+    launch_field_quadratic(ctx->sx.p, ctx->sy.p, ctx->sz.p, n, ctx->qf, ctx->fs.p, st);
+  else
+    launch_gather_double(ctx->perm.p, ctx->f.p, n, ctx->fs.p, st);
   zero_counters(ctx, 16);
+  const int* nb = ctx->nbr.p;
+  const int* nc = ctx->cnt.p;
   if (p->precision == LSMPS_WLS_FP64) {
-    launch_wls_exact(ctx->x.p, ctx->y.p, ctx->z.p, ctx->f.p, ctx->type.p, ctx->row_ptr.p, ctx->col.p, n, p->re, p->l0,
-                     p->reg, ctx->grad.p, ctx->lap.p, ctx->wstatus.p, ctx->counters.p, st);
+    launch_wls_exact(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->fs.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0,
+                     p->reg, ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, ctx->counters.p, st);
   } else {
     launch_fill_int(ctx->nredo.p, 1, 0, st);
-    launch_wls_mixed(ctx->x.p, ctx->y.p, ctx->z.p, ctx->f.p, ctx->type.p, ctx->row_ptr.p, ctx->col.p, n, p->re,
-                     p->l0, p->reg, ctx->grad.p, ctx->lap.p, ctx->wstatus.p, ctx->redo.p, ctx->nredo.p,
+    launch_wls_mixed(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->fs.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0,
+                     p->reg, ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p,
                      ctx->counters.p, st);
   }
   if (int rc = check_launch(ctx, "wls")) return rc;
@@ -480,8 +539,9 @@ int lsmps_wls_sampled_stats(lsmps_ctx* ctx, int32_t samples, lsmps_wls_stats* ou
   HIPCHK(ctx->le.ensure(ns));
   HIPCHK(ctx->st_s.ensure(ns));
   HIPCHK(ctx->ty_s.ensure(ns));
-  launch_wls_sample_errors(ctx->x.p, ctx->y.p, ctx->z.p, ctx->type.p, ctx->grad.p, ctx->lap.p, ctx->wstatus.p, stride,
-                           ns, ctx->qf, ctx->ge.p, ctx->le.p, ctx->st_s.p, ctx->ty_s.p, ctx->st);
+  launch_wls_sample_errors(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->type.p, ctx->grad_s.p, ctx->lap_s.p,
+                           ctx->wstatus_s.p, ctx->iperm.p, stride, ns, ctx->qf, ctx->ge.p, ctx->le.p, ctx->st_s.p,
+                           ctx->ty_s.p, ctx->st);
   if (int rc = check_launch(ctx, "wls_sample_errors")) return rc;
   std::vector<double> ge(ns), le(ns);
   std::vector<uint8_t> sts(ns), tys(ns);
@@ -532,7 +592,7 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
   hipStream_t st = ctx->st;
   HIPCHK(ctx->flag.ensure(n));
   HIPCHK(ctx->pos.ensure((size_t)n + 1));
-  HIPCHK(ctx->map.ensure(n));
+  HIPCHK(ctx->smap.ensure(n));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
   launch_unknown_flags(ctx->type.p, n, ctx->flag.p, st);
@@ -554,22 +614,13 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
   HIPCHK(ctx->pcol.ensure(nnz_cap));
   HIPCHK(ctx->pval.ensure(nnz_cap));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)m + 1)));
-  launch_unknown_map(ctx->flag.p, ctx->pos.p, n, cap, ctx->map.p, ctx->rev.p, st);
-  if (p->xmode == LSMPS_XSTAR_RANDOM && m > 0) {
-    // Seeded manufactured solution: one mt19937 draw per unknown, in order.
-    std::vector<double> xs(m);
-    std::mt19937 rng(p->xseed);
-    std::uniform_real_distribution<double> u(-1.0, 1.0);
-    for (int k = 0; k < m; ++k) xs[k] = u(rng);
-    HIPCHK(hipMemcpyAsync(ctx->xe.p, xs.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-  }
+  launch_unknown_map(ctx->flag.p, ctx->pos.p, n, cap, ctx->iperm.p, ctx->smap.p, ctx->rev.p, st);
   zero_counters(ctx, 16);
-  launch_ppe_row_count(ctx->x.p, ctx->y.p, ctx->z.p, ctx->row_ptr.p, ctx->col.p, ctx->map.p, ctx->rev.p, m, p->re,
-                       ctx->rcnt.p, st);
+  launch_ppe_row_count(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, ctx->smap.p, ctx->rev.p,
+                       ctx->iperm.p, m, p->re, ctx->rcnt.p, st);
   launch_exclusive_scan(ctx->rcnt.p, m, ctx->prow.p, ctx->scan_scr.p, st);
-  launch_ppe_row_fill(ctx->x.p, ctx->y.p, ctx->z.p, ctx->type.p, ctx->row_ptr.p, ctx->col.p, ctx->map.p, ctx->rev.p, m,
-                      p->re, p->shift, p->xmode, ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->xe.p, ctx->diag.p,
+  launch_ppe_row_fill(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->type.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, ctx->smap.p,
+                      ctx->rev.p, ctx->iperm.p, m, p->re, p->shift, p->xmode, p->xseed, ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->xe.p, ctx->diag.p,
                       ctx->counters.p, st);
   launch_ppe_rhs(ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->xe.p, m, ctx->b.p, st);
   if (int rc = check_launch(ctx, "ppe")) return rc;
@@ -642,10 +693,16 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
   const int max_launch = max_iter / chunk + 3;
   HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  const bool trace = std::getenv("LSMPS_SOLVER_TRACE") != nullptr;
   for (int l = 0; l < max_launch && ctx->host_sc->state == SOLVE_RUNNING; ++l) {
     HIPCHK(hipGraphLaunch(G.exec, st));
     HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (trace) {
+      const SolverScalars& s = *ctx->host_sc;
+      std::fprintf(stderr, "[solver] it=%d state=%d rn=%.6e rn/thr=%.3e rho=%.6e alpha=%.6e omega=%.6e beta=%.6e repl=%d\n",
+                   s.it, s.state, s.rn, s.rn / s.thresh, s.rho, s.alpha, s.omega, s.beta, s.n_replace);
+    }
   }
   launch_solver_final<T>(B, ctx->xe.p, ctx->xsol.p, ctx->sc.p, st);
   if (int rc = check_launch(ctx, "solver_final")) return rc;
@@ -713,6 +770,44 @@ int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_res
     B.t = v + 7 * (size_t)m;
     B.partials = ctx->partials.p;
     rc = run_bicgstab<float>(ctx, B, x0, p->max_iter, p->tol, SOLVE_RELIABLE | SOLVE_TRUE_CHECK, chunk, out);
+    if (rc == LSMPS_OK && !out->converged) {
+      // Fallback and judge: FP64 right-Jacobi BiCGStab, continuing from the
+      // mixed iterate when it lowered the residual, else from zero.
+      const lsmps_solve_result first = *out;
+      const bool resume = std::isfinite(first.final_residual) && first.final_residual < first.initial_residual;
+      if (resume) {
+        HIPCHK(ctx->x0.ensure(m));
+        HIPCHK(hipMemcpyAsync(ctx->x0.p, ctx->xsol.p, (size_t)m * sizeof(double), hipMemcpyDeviceToDevice, st));
+      }
+      HIPCHK(ctx->vec64.ensure(8 * (size_t)m));
+      HIPCHK(ctx->val64s.ensure(nnz));
+      launch_scale_matrix<double>(ctx->pcol.p, ctx->pval.p, ctx->scale.p, nnz, ctx->val64s.p, st);
+      SolverBufs<double> D;
+      D.m = m;
+      D.rowp = ctx->prow.p;
+      D.col = ctx->pcol.p;
+      D.val = ctx->val64s.p;
+      D.val64 = ctx->pval.p;
+      D.b = ctx->b.p;
+      D.scale = ctx->scale.p;
+      D.y = ctx->yv.p;
+      double* v = ctx->vec64.p;
+      D.r = v;
+      D.r0 = v + m;
+      D.p[0] = v + 2 * (size_t)m;
+      D.p[1] = v + 3 * (size_t)m;
+      D.v[0] = v + 4 * (size_t)m;
+      D.v[1] = v + 5 * (size_t)m;
+      D.s = v + 6 * (size_t)m;
+      D.t = v + 7 * (size_t)m;
+      D.partials = ctx->partials.p;
+      rc = run_bicgstab<double>(ctx, D, resume ? ctx->x0.p : nullptr, p->max_iter, p->tol, SOLVE_TRUE_CHECK, chunk,
+                                out);
+      out->iterations += first.iterations;
+      out->replacements += first.replacements;
+      out->initial_residual = first.initial_residual;
+      out->fallback = 1;
+    }
   } else {
     const bool jac = p->method == LSMPS_SOLVE_JACOBI;
     HIPCHK(ctx->vec64.ensure(8 * (size_t)m));
@@ -763,11 +858,20 @@ int lsmps_download_types(lsmps_ctx* ctx, uint8_t* type) {
 int lsmps_download_neighbors(lsmps_ctx* ctx, int32_t* row_ptr, int32_t* col) {
   if (!ctx) return LSMPS_E_ARG;
   if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "no neighbor lists");
+  GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
-  if (row_ptr) HIPCHK(hipMemcpyAsync(row_ptr, ctx->row_ptr.p, (ctx->n + 1) * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  const int n = (int)ctx->n;
+  HIPCHK(ctx->row_ptr.ensure((size_t)n + 1));
+  HIPCHK(ctx->col.ensure(ctx->pairs));
+  HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
+  launch_exclusive_scan(ctx->cnt_orig.p, n, ctx->row_ptr.p, ctx->scan_scr.p, ctx->st);
+  launch_ell_to_csr(cnl_bufs(ctx), n, ctx->maxn, ctx->row_ptr.p, ctx->col.p, ctx->st);
+  if (int rc = check_launch(ctx, "ell_to_csr")) return rc;
+  if (row_ptr) HIPCHK(hipMemcpyAsync(row_ptr, ctx->row_ptr.p, ((size_t)n + 1) * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
   if (col && ctx->pairs) HIPCHK(hipMemcpyAsync(col, ctx->col.p, ctx->pairs * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return LSMPS_OK;
+  GUARD_END
 }
 
 int lsmps_download_wls(lsmps_ctx* ctx, double* grad3, double* lap, uint8_t* status) {
@@ -775,6 +879,12 @@ int lsmps_download_wls(lsmps_ctx* ctx, double* grad3, double* lap, uint8_t* stat
   if (!ctx->have_wls) return fail(ctx, LSMPS_E_STATE, "no WLS results");
   HIPCHK(hipSetDevice(ctx->device));
   const size_t n = ctx->n;
+  HIPCHK(ctx->grad.ensure(3 * n));
+  HIPCHK(ctx->lap.ensure(n));
+  HIPCHK(ctx->wstatus.ensure(n));
+  launch_wls_unsort(ctx->perm.p, ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, (int)n, ctx->grad.p, ctx->lap.p,
+                    ctx->wstatus.p, ctx->st);
+  if (int rc = check_launch(ctx, "wls_unsort")) return rc;
   if (grad3 && n) HIPCHK(hipMemcpyAsync(grad3, ctx->grad.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
   if (lap && n) HIPCHK(hipMemcpyAsync(lap, ctx->lap.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
   if (status && n) HIPCHK(hipMemcpyAsync(status, ctx->wstatus.p, n, hipMemcpyDeviceToHost, ctx->st));

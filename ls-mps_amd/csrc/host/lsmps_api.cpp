@@ -119,8 +119,10 @@ Metrics run_benchmark(const Config& c, std::vector<Particle>& ps) {
   check(ctx, lsmps_upload_particles(ctx, xyz.data(), type.data(), n), "upload");
   const double re = c.re_ratio * c.l0;
 
+  // Neighbors and classification share one pass over the candidates.
   lsmps_cnl_stats cs;
-  check(ctx, lsmps_build_neighbors(ctx, re, c.max_neighbors, &cs), "neighbors");
+  lsmps_type_stats ts;
+  check(ctx, lsmps_build_neighbors_classify(ctx, re, c.max_neighbors, c.l0, &cs, &ts), "neighbors");
   m.neighbor_pairs = cs.pairs;
   m.truncated_particles = cs.truncated;
   m.neighbor_ms = cs.ms;
@@ -131,8 +133,6 @@ Metrics run_benchmark(const Config& c, std::vector<Particle>& ps) {
     m.neighbor_p99 = cs.p99;
   }
 
-  lsmps_type_stats ts;
-  check(ctx, lsmps_classify(ctx, c.l0, &ts), "classify");
   m.classify_ms = ts.ms;
   check(ctx, lsmps_download_types(ctx, type.data()), "download types");
   for (size_t i = 0; i < n; ++i) ps[i].type = (ParticleType)type[i];

@@ -25,49 +25,59 @@ struct QuadField {
 size_t scan_scratch_elems(int64_t n);
 void launch_exclusive_scan(const int* in, int64_t n, int* out, int64_t* scratch, hipStream_t st);
 
-// cnl.hip
+// cnl.hip: buffers of the cell list (device pointers).
+struct CnlBufs {
+  int gx = 0, gy = 0, gz = 0;
+  const double *x = nullptr, *y = nullptr, *z = nullptr;  // original order
+  int *cx = nullptr, *cy = nullptr, *cz = nullptr, *cell = nullptr, *slot = nullptr;
+  int *ccount = nullptr, *cstart = nullptr;               // per cell
+  int64_t* scan_scratch = nullptr;
+  int *perm = nullptr, *iperm = nullptr, *scell = nullptr;  // sorted slot <-> particle
+  double *sx = nullptr, *sy = nullptr, *sz = nullptr;       // sorted positions
+  int *nbr = nullptr, *cnt = nullptr, *cnt_orig = nullptr;  // ELL table
+};
 void launch_cell_coords(const double* x, const double* y, const double* z, int n, double re, int* cx, int* cy,
                         int* cz, int* box, hipStream_t st);
 void launch_fill_int(int* p, int64_t n, int v, hipStream_t st);
-void launch_cell_sort(const int* cx, const int* cy, const int* cz, int n, const int* box_host, int gx, int gy,
-                      int gz, int* cell, int* slot, int* count, int* start, int64_t* scan_scratch, int* perm,
-                      const double* x, const double* y, const double* z, double* sx, double* sy, double* sz,
-                      hipStream_t st);
-void launch_nbr_count(const double* sx, const double* sy, const double* sz, const int* perm, const int* start,
-                      const int* cell, int n, int gx, int gy, int gz, double re2, int maxn, int* count,
-                      unsigned long long* trunc_total, int* hist, hipStream_t st);
-void launch_nbr_fill(const double* sx, const double* sy, const double* sz, const int* perm, const int* start,
-                     const int* cell, int n, int gx, int gy, int gz, double re2, int maxn, const int* row_ptr, int* col,
+void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t st);
+void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classify, double zlim, uint8_t* type,
+                     uint8_t* type_s, unsigned long long* trunc_total, int* hist, unsigned long long* tcount,
                      hipStream_t st);
-void launch_classify(const double* x, const double* y, const double* z, const int* row_ptr, const int* col, int n,
-                     double zlim, uint8_t* type, unsigned long long* tcount, hipStream_t st);
+void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* type, uint8_t* type_s,
+                     unsigned long long* tcount, hipStream_t st);
+void launch_ell_to_csr(const CnlBufs& B, int n, int maxn, const int* row_ptr, int* col, hipStream_t st);
+void launch_gather_types(const int* perm, const uint8_t* type, int n, uint8_t* type_s, hipStream_t st);
 
-// wls.hip
+// wls.hip (positions, field, types, neighbor rows and outputs in sorted order)
 void launch_field_quadratic(const double* x, const double* y, const double* z, int n, const QuadField& q, double* f,
                             hipStream_t st);
 void launch_wls_exact(const double* x, const double* y, const double* z, const double* f, const uint8_t* type,
-                      const int* row_ptr, const int* col, int n, double re, double l0, double reg, double* grad,
+                      const int* nbr, const int* cnt, int maxn, int n, double re, double l0, double reg, double* grad,
                       double* lap, uint8_t* status, unsigned long long* counts, hipStream_t st);
 void launch_wls_mixed(const double* x, const double* y, const double* z, const double* f, const uint8_t* type,
-                      const int* row_ptr, const int* col, int n, double re, double l0, double reg, double* grad,
+                      const int* nbr, const int* cnt, int maxn, int n, double re, double l0, double reg, double* grad,
                       double* lap, uint8_t* status, int* redo, int* nredo, unsigned long long* counts,
                       hipStream_t st);
 void launch_wls_sample_errors(const double* x, const double* y, const double* z, const uint8_t* type,
-                              const double* grad, const double* lap, const uint8_t* status, int64_t stride, int ns,
-                              const QuadField& q, double* ge, double* le, uint8_t* st_out, uint8_t* ty_out,
-                              hipStream_t st);
+                              const double* grad, const double* lap, const uint8_t* status, const int* iperm,
+                              int64_t stride, int ns, const QuadField& q, double* ge, double* le, uint8_t* st_out,
+                              uint8_t* ty_out, hipStream_t st);
+void launch_wls_unsort(const int* perm, const double* gs, const double* ls, const uint8_t* ss, int n, double* g,
+                       double* l, uint8_t* st, hipStream_t stream);
+void launch_gather_double(const int* perm, const double* v, int n, double* out, hipStream_t st);
 
-// ppe.hip
+// ppe.hip (rows in unknown order; positions and neighbor rows in sorted order)
 void launch_unknown_flags(const uint8_t* type, int n, int* flag, hipStream_t st);
-void launch_unknown_map(const int* flag, const int* pos, int n, int cap, int* map, int* rev, hipStream_t st);
-void launch_ppe_row_count(const double* x, const double* y, const double* z, const int* nrow, const int* ncol,
-                          const int* map, const int* rev, int m, double re, int* cnt, hipStream_t st);
-void launch_ppe_row_fill(const double* x, const double* y, const double* z, const uint8_t* type, const int* nrow,
-                         const int* ncol, const int* map, const int* rev, int m, double re, double shift, int xmode,
-                         const int* prow, int* pcol, double* pval, double* xe, double* diag,
-                         unsigned long long* counters, hipStream_t st);
+void launch_unknown_map(const int* flag, const int* pos, int n, int cap, const int* iperm, int* smap, int* rev,
+                        hipStream_t st);
+void launch_ppe_row_count(const double* x, const double* y, const double* z, const int* nbr, const int* ncnt, int maxn,
+                          const int* smap, const int* rev, const int* iperm, int m, double re, int* cnt,
+                          hipStream_t st);
+void launch_ppe_row_fill(const double* x, const double* y, const double* z, const uint8_t* type, const int* nbr,
+                         const int* ncnt, int maxn, const int* smap, const int* rev, const int* iperm, int m, double re,
+                         double shift, int xmode, uint32_t xseed, const int* prow, int* pcol, double* pval, double* xe,
+                         double* diag, unsigned long long* counters, hipStream_t st);
 void launch_ppe_rhs(const int* prow, const int* pcol, const double* pval, const double* xe, int m, double* b,
                     hipStream_t st);
-
 
 }  // namespace lsmps_hip

@@ -22,13 +22,16 @@ __global__ __launch_bounds__(256) void k_unknown_flags(const uint8_t* __restrict
   if (i < n) flag[i] = (type[i] == T_INTERIOR || type[i] == T_NEARWALL) ? 1 : 0;
 }
 
+// Unknown numbering in particle index order; smap is the same map indexed
+// by sorted slot (what the row kernels look up for each neighbor).
 __global__ __launch_bounds__(256) void k_unknown_map(const int* __restrict__ flag, const int* __restrict__ pos, int n,
-                                                     int cap, int* __restrict__ map, int* __restrict__ rev) {
+                                                     int cap, const int* __restrict__ iperm, int* __restrict__ smap,
+                                                     int* __restrict__ rev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int p = pos[i];
   const bool u = flag[i] && p < cap;
-  map[i] = u ? p : -1;
+  smap[iperm[i]] = u ? p : -1;
   if (u) rev[p] = i;
 }
 
@@ -43,10 +46,12 @@ __device__ __forceinline__ bool pair_coef(double dx, double dy, double dz, doubl
   return true;
 }
 
+// Positions (x, y, z), neighbor rows and smap are in sorted-slot order.
 __global__ __launch_bounds__(256) void k_ppe_row_count(const double* __restrict__ x, const double* __restrict__ y,
-                                                       const double* __restrict__ z, const int* __restrict__ nrow,
-                                                       const int* __restrict__ ncol, const int* __restrict__ map,
-                                                       const int* __restrict__ rev, int m, double re,
+                                                       const double* __restrict__ z, const int* __restrict__ nbr,
+                                                       const int* __restrict__ ncnt, int maxn,
+                                                       const int* __restrict__ smap, const int* __restrict__ rev,
+                                                       const int* __restrict__ iperm, int m, double re,
                                                        int* __restrict__ cnt) {
   const int rr = blockIdx.x * blockDim.x + threadIdx.x;
   if (rr >= m) return;
@@ -54,24 +59,39 @@ __global__ __launch_bounds__(256) void k_ppe_row_count(const double* __restrict_
     cnt[0] = 1;
     return;
   }
-  const int i = rev[rr];
-  const double xi = x[i], yi = y[i], zi = z[i];
+  const int s = iperm[rev[rr]];
+  const double xi = x[s], yi = y[s], zi = z[s];
+  const int* row = nbr + (size_t)s * maxn;
+  const int k = ncnt[s];
   int c = 1;
-  for (int k = nrow[i]; k < nrow[i + 1]; ++k) {
-    const int j = ncol[k];
-    if (map[j] < 0) continue;
+  for (int q = 0; q < k; ++q) {
+    const int s2 = row[q];
+    if (smap[s2] < 0) continue;
     double a;
-    if (pair_coef(x[j] - xi, y[j] - yi, z[j] - zi, re, &a)) ++c;
+    if (pair_coef(x[s2] - xi, y[s2] - yi, z[s2] - zi, re, &a)) ++c;
   }
   cnt[rr] = c;
+}
+
+// Seeded manufactured solution x*_k ~ U(-1, 1): the top 53 bits of the
+// splitmix64 finaliser of seed * 2^64/phi + k, scaled to [0, 1) and mapped
+// to [-1, 1). Counter-based, so every row draws its value independently.
+__device__ __forceinline__ double xstar_random(uint32_t seed, int k) {
+  uint64_t z = (uint64_t)seed * 0x9E3779B97F4A7C15ull + (uint64_t)k;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return -1.0 + 2.0 * ((double)(z >> 11) * 0x1.0p-53);
 }
 
 // counters: [0] failed rows, [1] NearWall rows except row 0.
 __global__ __launch_bounds__(256) void k_ppe_row_fill(const double* __restrict__ x, const double* __restrict__ y,
                                                       const double* __restrict__ z, const uint8_t* __restrict__ type,
-                                                      const int* __restrict__ nrow, const int* __restrict__ ncol,
-                                                      const int* __restrict__ map, const int* __restrict__ rev, int m,
-                                                      double re, double shift, int xmode, const int* __restrict__ prow,
+                                                      const int* __restrict__ nbr, const int* __restrict__ ncnt,
+                                                      int maxn, const int* __restrict__ smap,
+                                                      const int* __restrict__ rev, const int* __restrict__ iperm, int m,
+                                                      double re, double shift, int xmode, uint32_t xseed,
+                                                      const int* __restrict__ prow,
                                                       int* __restrict__ pcol, double* __restrict__ pval,
                                                       double* __restrict__ xe, double* __restrict__ diag_out,
                                                       unsigned long long* __restrict__ counters) {
@@ -79,8 +99,10 @@ __global__ __launch_bounds__(256) void k_ppe_row_fill(const double* __restrict__
   int failed = 0, neumann = 0;
   if (rr < m) {
     const int i = rev[rr];
-    const double xi = x[i], yi = y[i], zi = z[i];
-    if (xmode == 0) xe[rr] = xi * xi + yi * yi;  // x* = x^2 + y^2 (src/lsmps.cpp:29)
+    const int s = iperm[i];
+    const double xi = x[s], yi = y[s], zi = z[s];
+    xe[rr] = xmode == 0 ? xi * xi + yi * yi  // x* = x^2 + y^2 (src/lsmps.cpp:29)
+                        : xstar_random(xseed, rr);
     int at = prow[rr];
     if (rr == 0) {
       pcol[at] = 0;
@@ -88,12 +110,14 @@ __global__ __launch_bounds__(256) void k_ppe_row_fill(const double* __restrict__
       diag_out[0] = 1.0;
     } else {
       double d = 0.0;
-      for (int k = nrow[i]; k < nrow[i + 1]; ++k) {
-        const int j = ncol[k];
-        const int cj = map[j];
+      const int* row = nbr + (size_t)s * maxn;
+      const int kn = ncnt[s];
+      for (int q = 0; q < kn; ++q) {
+        const int s2 = row[q];
+        const int cj = smap[s2];
         if (cj < 0) continue;
         double a;
-        if (!pair_coef(x[j] - xi, y[j] - yi, z[j] - zi, re, &a)) continue;
+        if (!pair_coef(x[s2] - xi, y[s2] - yi, z[s2] - zi, re, &a)) continue;
         d += a;
         pcol[at] = cj;
         pval[at] = -a;
@@ -133,22 +157,25 @@ static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) 
 void launch_unknown_flags(const uint8_t* type, int n, int* flag, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_unknown_flags, dim3(blocks(n, 256)), dim3(256), 0, st, type, n, flag);
 }
-void launch_unknown_map(const int* flag, const int* pos, int n, int cap, int* map, int* rev, hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_unknown_map, dim3(blocks(n, 256)), dim3(256), 0, st, flag, pos, n, cap, map, rev);
+void launch_unknown_map(const int* flag, const int* pos, int n, int cap, const int* iperm, int* smap, int* rev,
+                        hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_unknown_map, dim3(blocks(n, 256)), dim3(256), 0, st, flag, pos, n, cap, iperm, smap, rev);
 }
-void launch_ppe_row_count(const double* x, const double* y, const double* z, const int* nrow, const int* ncol,
-                          const int* map, const int* rev, int m, double re, int* cnt, hipStream_t st) {
+void launch_ppe_row_count(const double* x, const double* y, const double* z, const int* nbr, const int* ncnt, int maxn,
+                          const int* smap, const int* rev, const int* iperm, int m, double re, int* cnt,
+                          hipStream_t st) {
   if (m > 0)
-    hipLaunchKernelGGL(k_ppe_row_count, dim3(blocks(m, 256)), dim3(256), 0, st, x, y, z, nrow, ncol, map, rev, m, re,
-                       cnt);
+    hipLaunchKernelGGL(k_ppe_row_count, dim3(blocks(m, 256)), dim3(256), 0, st, x, y, z, nbr, ncnt, maxn, smap, rev,
+                       iperm, m, re, cnt);
 }
-void launch_ppe_row_fill(const double* x, const double* y, const double* z, const uint8_t* type, const int* nrow,
-                         const int* ncol, const int* map, const int* rev, int m, double re, double shift, int xmode,
-                         const int* prow, int* pcol, double* pval, double* xe, double* diag,
-                         unsigned long long* counters, hipStream_t st) {
+void launch_ppe_row_fill(const double* x, const double* y, const double* z, const uint8_t* type, const int* nbr,
+                         const int* ncnt, int maxn, const int* smap, const int* rev, const int* iperm, int m, double re,
+                         double shift, int xmode, uint32_t xseed, const int* prow, int* pcol, double* pval, double* xe,
+                         double* diag, unsigned long long* counters, hipStream_t st) {
   if (m > 0)
-    hipLaunchKernelGGL(k_ppe_row_fill, dim3(blocks(m, 256)), dim3(256), 0, st, x, y, z, type, nrow, ncol, map, rev, m,
-                       re, shift, xmode, prow, pcol, pval, xe, diag, counters);
+    hipLaunchKernelGGL(k_ppe_row_fill, dim3(blocks(m, 256)), dim3(256), 0, st, x, y, z, type, nbr, ncnt, maxn, smap,
+                       rev, iperm, m, re, shift, xmode, xseed, prow, pcol, pval, xe, diag, counters);
 }
 void launch_ppe_rhs(const int* prow, const int* pcol, const double* pval, const double* xe, int m, double* b,
                     hipStream_t st) {

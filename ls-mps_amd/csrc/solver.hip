@@ -48,6 +48,10 @@ __device__ void finish_iteration(SolverScalars* sc, double rr, double r0r) {
   sc->rn = rn;
   sc->it += 1;
   sc->iterations = sc->it;
+  if (!isfinite(rn) || !isfinite(r0r)) {
+    sc->state = SOLVE_BREAKDOWN;
+    return;
+  }
   if (rn <= sc->thresh) {
     if (sc->flags & SOLVE_TRUE_CHECK) {
       sc->rho_prev = sc->rho;
@@ -107,7 +111,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_pv(const int* __restric
   double v[1] = {dot}, tot[1];
   if (grid_sum_last_block<1>(v, partials, &sc->ticket[0], tot, lds) && threadIdx.x == 0) {
     const double den = tot[0];
-    if (fabs(den) < 1e-30) {
+    if (fabs(den) < 1e-30 || !isfinite(den)) {
       sc->state = SOLVE_BREAKDOWN;
     } else {
       sc->alpha = sc->rho / den;
@@ -150,7 +154,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_st(const int* __restric
     if ((sc->flags & SOLVE_S_EXIT) && sn <= sc->thresh) {
       sc->state = SOLVE_S_CONVERGED;
       sc->iterations = sc->it + 1;
-    } else if (tot[1] < 1e-30) {
+    } else if (tot[1] < 1e-30 || !isfinite(tot[1]) || !isfinite(tot[2])) {
       sc->state = SOLVE_BREAKDOWN;
     } else {
       sc->omega = tot[2] / tot[1];
@@ -256,6 +260,10 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_true_residual(
       sc->n_replace += 1;
       sc->rn = rn;
       sc->rn_max = rn;
+      if (!isfinite(rn) || !isfinite(tot[1])) {
+        sc->state = SOLVE_BREAKDOWN;
+        return;
+      }
       if (kind == 2 && rn <= sc->thresh) {
         sc->state = SOLVE_CONVERGED;
         return;

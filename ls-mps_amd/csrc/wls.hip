@@ -74,10 +74,11 @@ __device__ __forceinline__ void chol9_solve(const T (&L)[45], const T (&b)[9], T
 // the status; writes grad/lap.
 __device__ __noinline__ uint8_t wls_exact_one(int i, const double* __restrict__ x, const double* __restrict__ y,
                                               const double* __restrict__ z, const double* __restrict__ f,
-                                              const int* __restrict__ row_ptr, const int* __restrict__ col, double re,
-                                              double l0, double reg, double* g3, double* lap) {
+                                              const int* __restrict__ nbr, const int* __restrict__ cnt, int maxn,
+                                              double re, double l0, double reg, double* g3, double* lap) {
   const double xi = x[i], yi = y[i], zi = z[i], fi = f[i];
-  const int rb = row_ptr[i], rend = row_ptr[i + 1];
+  const int* col = nbr + (size_t)i * maxn;
+  const int rb = 0, rend = cnt[i];
   const double h2 = 2 * l0;
   for (int attempt = 0; attempt < 5; ++attempt) {
     double M[45], B[9], ws = 0.0;
@@ -161,8 +162,8 @@ __device__ __forceinline__ void store_result(int i, uint8_t st, const double* g3
 
 __global__ __launch_bounds__(128) void k_wls_exact(const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, const double* __restrict__ f,
-                                                   const uint8_t* __restrict__ type, const int* __restrict__ row_ptr,
-                                                   const int* __restrict__ col, int n, double re, double l0,
+                                                   const uint8_t* __restrict__ type, const int* __restrict__ nbr,
+                                                   const int* __restrict__ cnt, int maxn, int n, double re, double l0,
                                                    double reg, double* __restrict__ grad, double* __restrict__ lap,
                                                    uint8_t* __restrict__ status,
                                                    unsigned long long* __restrict__ counts) {
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(128) void k_wls_exact(const double* __restrict__ x,
     const uint8_t t = type[i];
     double g3[3] = {0.0, 0.0, 0.0}, L = 0.0;
     uint8_t st = WLS_SKIPPED;
-    if (t != T_WALL && t != T_SPLASH) st = wls_exact_one(i, x, y, z, f, row_ptr, col, re, l0, reg, g3, &L);
+    if (t != T_WALL && t != T_SPLASH) st = wls_exact_one(i, x, y, z, f, nbr, cnt, maxn, re, l0, reg, g3, &L);
     store_result(i, st, g3, L, grad, lap, status);
     slot = st;
   }
@@ -182,16 +183,17 @@ __global__ __launch_bounds__(128) void k_wls_exact(const double* __restrict__ x,
 // Redo in FP64 the particles the FP32 kernel could not trust.
 __global__ __launch_bounds__(128) void k_wls_exact_list(const double* __restrict__ x, const double* __restrict__ y,
                                                         const double* __restrict__ z, const double* __restrict__ f,
-                                                        const int* __restrict__ row_ptr, const int* __restrict__ col,
-                                                        const int* __restrict__ list, const int* __restrict__ nlist,
+                                                        const int* __restrict__ nbr, const int* __restrict__ cnt,
+                                                        int maxn, const int* __restrict__ list,
+                                                        const int* __restrict__ nlist,
                                                         double re, double l0, double reg, double* __restrict__ grad,
                                                         double* __restrict__ lap, uint8_t* __restrict__ status,
                                                         unsigned long long* __restrict__ counts) {
-  const int cnt = *nlist;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+  const int nl = *nlist;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x) {
     const int i = list[k];
     double g3[3] = {0.0, 0.0, 0.0}, L = 0.0;
-    const uint8_t st = wls_exact_one(i, x, y, z, f, row_ptr, col, re, l0, reg, g3, &L);
+    const uint8_t st = wls_exact_one(i, x, y, z, f, nbr, cnt, maxn, re, l0, reg, g3, &L);
     store_result(i, st, g3, L, grad, lap, status);
     atomicAdd(&counts[st], 1ull);
     atomicAdd(&counts[WLS_REDO_SLOT], 1ull);
@@ -204,8 +206,8 @@ __global__ __launch_bounds__(128) void k_wls_exact_list(const double* __restrict
 // Returns the status slot to count (-1 when the row went to the FP64 list).
 __device__ __forceinline__ int wls_mixed_one(int i, const double* __restrict__ x, const double* __restrict__ y,
                                              const double* __restrict__ z, const double* __restrict__ f,
-                                             const uint8_t* __restrict__ type, const int* __restrict__ row_ptr,
-                                             const int* __restrict__ col, double re, double l0,
+                                             const uint8_t* __restrict__ type, const int* __restrict__ nbr,
+                                             const int* __restrict__ cnt, int maxn, double re, double l0,
                                              double* __restrict__ grad, double* __restrict__ lap,
                                              uint8_t* __restrict__ status, int* __restrict__ redo,
                                              int* __restrict__ nredo) {
@@ -220,7 +222,8 @@ __device__ __forceinline__ int wls_mixed_one(int i, const double* __restrict__ x
   const float inv_h2 = 1.0f / (2.0f * l0s), inv_g = 1.0f / l0s;
   const float rmin = (float)(1e-12 * inv_re);
   const double xi = x[i], yi = y[i], zi = z[i], fi = f[i];
-  const int rb = row_ptr[i], rend = row_ptr[i + 1];
+  const int* col = nbr + (size_t)i * maxn;
+  const int rb = 0, rend = cnt[i];
   float M[45], B[9], ws = 0.0f;
 #pragma unroll
   for (int k = 0; k < 45; ++k) M[k] = 0.0f;
@@ -283,14 +286,14 @@ __device__ __forceinline__ int wls_mixed_one(int i, const double* __restrict__ x
 
 __global__ __launch_bounds__(128) void k_wls_mixed(const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, const double* __restrict__ f,
-                                                   const uint8_t* __restrict__ type, const int* __restrict__ row_ptr,
-                                                   const int* __restrict__ col, int n, double re, double l0,
+                                                   const uint8_t* __restrict__ type, const int* __restrict__ nbr,
+                                                   const int* __restrict__ cnt, int maxn, int n, double re, double l0,
                                                    double* __restrict__ grad, double* __restrict__ lap,
                                                    uint8_t* __restrict__ status, int* __restrict__ redo,
                                                    int* __restrict__ nredo, unsigned long long* __restrict__ counts) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int slot = -1;
-  if (i < n) slot = wls_mixed_one(i, x, y, z, f, type, row_ptr, col, re, l0, grad, lap, status, redo, nredo);
+  if (i < n) slot = wls_mixed_one(i, x, y, z, f, type, nbr, cnt, maxn, re, l0, grad, lap, status, redo, nredo);
   count_status(slot, counts);
 }
 
@@ -308,19 +311,22 @@ __global__ __launch_bounds__(256) void k_field_quadratic(const double* __restric
          q.c[7] * a * b + q.c[8] * a * c + q.c[9] * b * c;
 }
 
-// Errors of the sampled particles i = k*stride against the closed-form
-// gradient and Laplacian of the quadratic field (src/lsmps.cpp:28).
+// Errors of the sampled particles i = k*stride (original index) against the
+// closed-form gradient and Laplacian of the quadratic field
+// (src/lsmps.cpp:28); positions and results are read at the sorted slot.
 __global__ __launch_bounds__(256) void k_wls_sample_errors(const double* __restrict__ x, const double* __restrict__ y,
                                                            const double* __restrict__ z, const uint8_t* __restrict__ type,
                                                            const double* __restrict__ grad,
                                                            const double* __restrict__ lap,
-                                                           const uint8_t* __restrict__ status, int64_t stride, int ns,
+                                                           const uint8_t* __restrict__ status,
+                                                           const int* __restrict__ iperm, int64_t stride, int ns,
                                                            QuadField q, double* __restrict__ ge,
                                                            double* __restrict__ le, uint8_t* __restrict__ st_out,
                                                            uint8_t* __restrict__ ty_out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ns) return;
-  const int64_t i = (int64_t)k * stride;
+  const int64_t io = (int64_t)k * stride;  // original index; arrays below are in sorted order
+  const int64_t i = iperm[io];
   const double a = x[i], b = y[i], c = z[i];
   const double ex = q.c[1] + 2 * q.c[4] * a + q.c[7] * b + q.c[8] * c;
   const double ey = q.c[2] + 2 * q.c[5] * b + q.c[7] * a + q.c[9] * c;
@@ -329,10 +335,40 @@ __global__ __launch_bounds__(256) void k_wls_sample_errors(const double* __restr
   ge[k] = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
   le[k] = fabs(lap[i] - q.lap_exact);
   st_out[k] = status[i];
-  ty_out[k] = type[i];
+  ty_out[k] = type[io];
+}
+
+// WLS results from sorted order back to original particle order.
+__global__ __launch_bounds__(256) void k_wls_unsort(const int* __restrict__ perm, const double* __restrict__ gs,
+                                                    const double* __restrict__ ls, const uint8_t* __restrict__ ss, int n,
+                                                    double* __restrict__ g, double* __restrict__ l,
+                                                    uint8_t* __restrict__ st) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  g[3 * (size_t)i] = gs[3 * (size_t)s];
+  g[3 * (size_t)i + 1] = gs[3 * (size_t)s + 1];
+  g[3 * (size_t)i + 2] = gs[3 * (size_t)s + 2];
+  l[i] = ls[s];
+  st[i] = ss[s];
+}
+
+__global__ __launch_bounds__(256) void k_gather_double(const int* __restrict__ perm, const double* __restrict__ v, int n,
+                                                       double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) out[s] = v[perm[s]];
 }
 
 static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+void launch_wls_unsort(const int* perm, const double* gs, const double* ls, const uint8_t* ss, int n, double* g,
+                       double* l, uint8_t* st, hipStream_t stream) {
+  if (n > 0) hipLaunchKernelGGL(k_wls_unsort, dim3(blocks(n, 256)), dim3(256), 0, stream, perm, gs, ls, ss, n, g, l, st);
+}
+
+void launch_gather_double(const int* perm, const double* v, int n, double* out, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_double, dim3(blocks(n, 256)), dim3(256), 0, st, perm, v, n, out);
+}
 
 void launch_field_quadratic(const double* x, const double* y, const double* z, int n, const QuadField& q, double* f,
                             hipStream_t st) {
@@ -340,34 +376,34 @@ void launch_field_quadratic(const double* x, const double* y, const double* z, i
 }
 
 void launch_wls_exact(const double* x, const double* y, const double* z, const double* f, const uint8_t* type,
-                      const int* row_ptr, const int* col, int n, double re, double l0, double reg, double* grad,
+                      const int* nbr, const int* cnt, int maxn, int n, double re, double l0, double reg, double* grad,
                       double* lap, uint8_t* status, unsigned long long* counts, hipStream_t st) {
   if (n > 0)
-    hipLaunchKernelGGL(k_wls_exact, dim3(blocks(n, 128)), dim3(128), 0, st, x, y, z, f, type, row_ptr, col, n, re,
+    hipLaunchKernelGGL(k_wls_exact, dim3(blocks(n, 128)), dim3(128), 0, st, x, y, z, f, type, nbr, cnt, maxn, n, re,
                        l0, reg, grad, lap, status, counts);
 }
 
 void launch_wls_mixed(const double* x, const double* y, const double* z, const double* f, const uint8_t* type,
-                      const int* row_ptr, const int* col, int n, double re, double l0, double reg, double* grad,
+                      const int* nbr, const int* cnt, int maxn, int n, double re, double l0, double reg, double* grad,
                       double* lap, uint8_t* status, int* redo, int* nredo, unsigned long long* counts,
                       hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_wls_mixed, dim3(blocks(n, 128)), dim3(128), 0, st, x, y, z, f, type, row_ptr, col, n, re, l0,
+  hipLaunchKernelGGL(k_wls_mixed, dim3(blocks(n, 128)), dim3(128), 0, st, x, y, z, f, type, nbr, cnt, maxn, n, re, l0,
                      grad, lap, status, redo, nredo, counts);
   // The redo list length is only known on the device: a bounded grid
   // strides over it (typical lists are empty or a few hundred entries).
   const unsigned g = blocks(n, 128) < 512u ? blocks(n, 128) : 512u;
-  hipLaunchKernelGGL(k_wls_exact_list, dim3(g), dim3(128), 0, st, x, y, z, f, row_ptr, col, redo, nredo,
+  hipLaunchKernelGGL(k_wls_exact_list, dim3(g), dim3(128), 0, st, x, y, z, f, nbr, cnt, maxn, redo, nredo,
                      re, l0, reg, grad, lap, status, counts);
 }
 
 void launch_wls_sample_errors(const double* x, const double* y, const double* z, const uint8_t* type,
-                              const double* grad, const double* lap, const uint8_t* status, int64_t stride, int ns,
-                              const QuadField& q, double* ge, double* le, uint8_t* st_out, uint8_t* ty_out,
-                              hipStream_t st) {
+                              const double* grad, const double* lap, const uint8_t* status, const int* iperm,
+                              int64_t stride, int ns, const QuadField& q, double* ge, double* le, uint8_t* st_out,
+                              uint8_t* ty_out, hipStream_t st) {
   if (ns > 0)
     hipLaunchKernelGGL(k_wls_sample_errors, dim3(blocks(ns, 256)), dim3(256), 0, st, x, y, z, type, grad, lap, status,
-                       stride, ns, q, ge, le, st_out, ty_out);
+                       iperm, stride, ns, q, ge, le, st_out, ty_out);
 }
 
 }  // namespace lsmps_hip

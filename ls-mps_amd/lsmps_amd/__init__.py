@@ -96,6 +96,7 @@ ABI = {
     "lsmps_synchronize": (C.c_int, [_p]),
     "lsmps_upload_particles": (C.c_int, [_p, _p, _p, _u64]),
     "lsmps_build_neighbors": (C.c_int, [_p, _d, _i32, C.POINTER(CnlStats)]),
+    "lsmps_build_neighbors_classify": (C.c_int, [_p, _d, _i32, _d, C.POINTER(CnlStats), C.POINTER(TypeStats)]),
     "lsmps_classify": (C.c_int, [_p, _d, C.POINTER(TypeStats)]),
     "lsmps_set_field_quadratic": (C.c_int, [_p, C.POINTER(QuadField)]),
     "lsmps_set_field_values": (C.c_int, [_p, _p, _u64]),
@@ -175,6 +176,15 @@ class Pipeline:
         self._check(self.L.lsmps_build_neighbors(self.h, re, max_neighbors, C.byref(s)), "build_neighbors")
         self.pairs = s.pairs
         return as_dict(s)
+
+    def neighbors_classify(self, re, max_neighbors, l0):
+        """Neighbors and classification fused in one pass."""
+        s = CnlStats()
+        t = TypeStats()
+        self._check(self.L.lsmps_build_neighbors_classify(self.h, re, max_neighbors, l0, C.byref(s), C.byref(t)),
+                    "build_neighbors_classify")
+        self.pairs = s.pairs
+        return as_dict(s), as_dict(t)
 
     def classify(self, l0):
         s = TypeStats()

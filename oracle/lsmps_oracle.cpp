@@ -372,6 +372,17 @@ orc_wls_stats wls_sampled(const orc_state& s, double re, double l0, double reg, 
   return st;
 }
 
+// Seeded manufactured solution U(-1, 1) for row k (no reference code): the
+// splitmix64 finaliser (Steele, Lea, Flood 2014) of seed * 2^64/phi + k, top
+// 53 bits scaled to [0, 1), mapped to [-1, 1).
+double hashed_uniform(uint32_t seed, uint64_t k) {
+  uint64_t z = (uint64_t)seed * 0x9E3779B97F4A7C15ull + k;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return -1.0 + 2.0 * ((double)(z >> 11) * 0x1.0p-53);
+}
+
 // Graph-Laplacian PPE over the first `cap` Interior/NearWall particles with
 // row 0 pinned to the identity and b = A x* (src/lsmps.cpp:29-64).
 void assemble(orc_state& s, double re, int max_unknowns, int xmode, uint32_t xseed, double shift,
@@ -392,11 +403,9 @@ void assemble(orc_state& s, double re, int max_unknowns, int xmode, uint32_t xse
   A.row.push_back(0);
   A.b.assign(m, 0.0);
   A.xe.resize(m);
-  std::mt19937 rng(xseed);
-  std::uniform_real_distribution<double> ux(-1.0, 1.0);
   for (size_t k = 0; k < m; ++k) {
     const P3& p = s.r[s.rev[k]];
-    A.xe[k] = xmode == 0 ? p.x * p.x + p.y * p.y : ux(rng);
+    A.xe[k] = xmode == 0 ? p.x * p.x + p.y * p.y : hashed_uniform(xseed, k);
   }
   for (size_t k = 0; k < m; ++k) {
     const int i = s.rev[k];

@@ -97,7 +97,7 @@ double orc_wls_moments(const orc_state* s, const double* f, double re, double l0
 void orc_wls_test(const orc_state* s, double re, double l0, double reg, int wls_samples, orc_wls_stats* out);
 
 /* PPE assembly (src/lsmps.cpp:29-64). xmode 0: x* = x^2+y^2 (reference);
- * xmode 1: x* ~ U(-1,1) from mt19937(xseed). shift adds shift*diag to each
+ * xmode 1: x* ~ U(-1,1) from the splitmix64 hash of (xseed, row). shift adds shift*diag to each
  * diagonal after assembly (0 = reference). */
 void orc_assemble_ppe(orc_state* s, double re, int max_unknowns, int xmode, uint32_t xseed,
                       double shift, orc_ppe_stats* out);

@@ -187,7 +187,8 @@ def test_jacobi_bicgstab_against_spsolve():
     for pc in (0, 1):
         st, x = o.solve(5000, 1e-12, precond=pc)
         assert st["converged"] == 1
-        assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+        # error <= cond(A) * relative residual; cond ~ 1e4 here.
+        assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-7
         assert st["final_residual"] <= 1e-12 * max(1, np.linalg.norm(A["b"])) * 10
     # Jacobi helps on the graph Laplacian of a wall film: fewer iterations.
     s0, _ = o.solve(5000, 1e-10, precond=0)
