@@ -101,9 +101,11 @@ struct lsmps_ctx {
   DBuf<int> flag, pos, smap, rev, rcnt, prow, pcol;
   DBuf<double> pval, b, xe, diag;
 
-  // solver
-  DBuf<double> scale, yv, xsol, x0, val64s, vec64, partials;
-  DBuf<float> val32, vec32;
+  // solver (iteration matrix in sliced-ELL form)
+  DBuf<double> scale, yv, xsol, x0, vec64, partials, sell_v64, sell_vs64;
+  DBuf<float> vec32, sell_v32;
+  DBuf<int> sell_w, sell_ptr, sell_col;
+  uint64_t sell_nnz = 0;
   DBuf<SolverScalars> sc;
   SolverScalars* host_sc = nullptr;  // pinned
   int* host_small = nullptr;         // pinned scratch for small readbacks (64 ints)
@@ -203,13 +205,13 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->graph.reset();
   DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x, &ctx->y, &ctx->z, &ctx->sx, &ctx->sy, &ctx->sz, &ctx->f, &ctx->fs,
                        &ctx->grad_s, &ctx->lap_s, &ctx->grad, &ctx->lap, &ctx->ge, &ctx->le, &ctx->pval, &ctx->b,
-                       &ctx->xe, &ctx->diag, &ctx->scale, &ctx->yv, &ctx->xsol, &ctx->x0, &ctx->val64s, &ctx->vec64,
-                       &ctx->partials};
+                       &ctx->xe, &ctx->diag, &ctx->scale, &ctx->yv, &ctx->xsol, &ctx->x0, &ctx->vec64, &ctx->partials,
+                       &ctx->sell_v64, &ctx->sell_vs64};
   for (auto* p : d) p->release();
   DBuf<int>* i[] = {&ctx->cx, &ctx->cy, &ctx->cz, &ctx->box, &ctx->cell, &ctx->slot, &ctx->ccount, &ctx->cstart,
                     &ctx->perm, &ctx->iperm, &ctx->scell, &ctx->nbr, &ctx->cnt, &ctx->cnt_orig, &ctx->row_ptr,
                     &ctx->col, &ctx->hist, &ctx->redo, &ctx->nredo, &ctx->flag, &ctx->pos, &ctx->smap, &ctx->rev,
-                    &ctx->rcnt, &ctx->prow, &ctx->pcol};
+                    &ctx->rcnt, &ctx->prow, &ctx->pcol, &ctx->sell_w, &ctx->sell_ptr, &ctx->sell_col};
   for (auto* p : i) p->release();
   ctx->type.release();
   ctx->type_s.release();
@@ -219,7 +221,7 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->ty_s.release();
   ctx->scan_scr.release();
   ctx->counters.release();
-  ctx->val32.release();
+  ctx->sell_v32.release();
   ctx->vec32.release();
   ctx->sc.release();
   if (ctx->host_sc) (void)hipHostFree(ctx->host_sc);
@@ -674,13 +676,13 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
   // Capture `chunk` iterations once per buffer set; replay until the device
   // state leaves RUNNING (kernels past convergence return at once).
   GraphCache& G = ctx->graph;
-  const void* key[6] = {B.val, B.r, B.y, B.scale, B.b, B.val64};
+  const void* key[6] = {B.val, B.r, B.y, B.scale, B.sptr, B.val64};
   const bool hit = G.exec && G.m == B.m && G.chunk == chunk && G.with_replace == (int)with_replace &&
                    G.fp32 == (int)(sizeof(T) == 4) && std::equal(key, key + 6, G.key);
   if (!hit) {
     G.reset();
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < chunk; ++k) launch_solver_iteration<T>(B, k & 1, with_replace, ctx->sc.p, st);
+    for (int k = 0; k < chunk; ++k) launch_solver_iteration<T>(B, with_replace, ctx->sc.p, st);
     hipError_t ec = hipStreamEndCapture(st, &G.graph);
     if (ec != hipSuccess) return fail(ctx, LSMPS_E_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
     HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
@@ -725,51 +727,108 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+
+// Sliced-ELL structure of the assembled PPE (slice offsets, padded columns)
+// and its unscaled FP64 values.
+int build_sell_structure(lsmps_ctx* ctx) {
+  const int m = (int)ctx->m;
+  hipStream_t st = ctx->st;
+  const int ns = sell_slices(m);
+  HIPCHK(ctx->sell_w.ensure(ns));
+  HIPCHK(ctx->sell_ptr.ensure((size_t)ns + 1));
+  HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)ns + 1)));
+  launch_sell_widths(ctx->prow.p, m, ctx->sell_w.p, st);
+  launch_exclusive_scan(ctx->sell_w.p, ns, ctx->sell_ptr.p, ctx->scan_scr.p, st);
+  if (int rc = check_launch(ctx, "sell_widths")) return rc;
+  int64_t* hs = reinterpret_cast<int64_t*>(ctx->host_small);
+  const int ntiles = (ns + 2047) / 2048;
+  HIPCHK(hipMemcpyAsync(hs, ctx->scan_scr.p + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (hs[0] > (int64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "padded matrix beyond int32 indexing");
+  ctx->sell_nnz = (uint64_t)hs[0];
+  HIPCHK(ctx->sell_col.ensure(ctx->sell_nnz));
+  HIPCHK(ctx->sell_v64.ensure(ctx->sell_nnz));
+  return LSMPS_OK;
+}
+
+template <typename T>
+void bind_vectors(SolverBufs<T>& B, T* v, int m) {
+  B.r = v;
+  B.r0 = v + m;
+  B.p = v + 2 * (size_t)m;
+  B.v = v + 3 * (size_t)m;
+  B.s = v + 4 * (size_t)m;
+  B.t = v + 5 * (size_t)m;
+}
+
+// FP64 right-Jacobi solve (the judge and fallback), optionally from x0.
+int solve_fp64_jacobi(lsmps_ctx* ctx, const double* x0, int max_iter, double tol, int chunk, lsmps_solve_result* out) {
+  const int m = (int)ctx->m;
+  hipStream_t st = ctx->st;
+  HIPCHK(ctx->vec64.ensure(6 * (size_t)m));
+  HIPCHK(ctx->sell_vs64.ensure(ctx->sell_nnz));
+  launch_inv_diag(ctx->diag.p, m, ctx->scale.p, st);
+  launch_sell_fill<double>(ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->scale.p, m, ctx->maxn + 1, ctx->sell_ptr.p, ctx->sell_col.p,
+                           ctx->sell_vs64.p, ctx->sell_v64.p, st);
+  SolverBufs<double> B;
+  B.m = m;
+  B.sptr = ctx->sell_ptr.p;
+  B.col = ctx->sell_col.p;
+  B.val = ctx->sell_vs64.p;
+  B.val64 = ctx->sell_v64.p;
+  B.b = ctx->b.p;
+  B.scale = ctx->scale.p;
+  B.y = ctx->yv.p;
+  B.partials = ctx->partials.p;
+  bind_vectors(B, ctx->vec64.p, m);
+  return run_bicgstab<double>(ctx, B, x0, max_iter, tol, SOLVE_TRUE_CHECK, chunk, out);
+}
+
+}  // namespace
+
+extern "C" {
+
 int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_result* out) {
   if (!ctx || !p || !out) return LSMPS_E_ARG;
   if (!ctx->have_ppe) return fail(ctx, LSMPS_E_STATE, "solve needs an assembled PPE");
   if (p->use_x0 && !ctx->have_x0) return fail(ctx, LSMPS_E_STATE, "use_x0 without lsmps_upload_x0");
   if (p->method < LSMPS_SOLVE_PLAIN || p->method > LSMPS_SOLVE_MIXED)
     return fail(ctx, LSMPS_E_ARG, "unknown solve method");
-  int chunk = p->chunk > 0 ? p->chunk : 16;
-  chunk += chunk & 1;  // p and v ping-pong: an even count returns to parity 0
+  const int chunk = p->chunk > 0 ? p->chunk : 16;
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   std::memset(out, 0, sizeof(*out));
   const int m = (int)ctx->m;
-  const int64_t nnz = (int64_t)ctx->nnz;
   hipStream_t st = ctx->st;
   HIPCHK(ctx->xsol.ensure(m));
   HIPCHK(ctx->yv.ensure(m));
   HIPCHK(ctx->scale.ensure(m));
   HIPCHK(hipEventRecord(ctx->e0, st));
+  if (int rc = build_sell_structure(ctx)) return rc;
   const double* x0 = p->use_x0 ? ctx->x0.p : nullptr;
   int rc = LSMPS_OK;
   if (p->method == LSMPS_SOLVE_MIXED) {
-    HIPCHK(ctx->val32.ensure(nnz));
-    HIPCHK(ctx->vec32.ensure(8 * (size_t)m));
+    HIPCHK(ctx->sell_v32.ensure(ctx->sell_nnz));
+    HIPCHK(ctx->vec32.ensure(6 * (size_t)m));
     launch_inv_diag(ctx->diag.p, m, ctx->scale.p, st);
-    launch_scale_matrix<float>(ctx->pcol.p, ctx->pval.p, ctx->scale.p, nnz, ctx->val32.p, st);
+    launch_sell_fill<float>(ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->scale.p, m, ctx->maxn + 1, ctx->sell_ptr.p, ctx->sell_col.p,
+                            ctx->sell_v32.p, ctx->sell_v64.p, st);
     SolverBufs<float> B;
     B.m = m;
-    B.rowp = ctx->prow.p;
-    B.col = ctx->pcol.p;
-    B.val = ctx->val32.p;
-    B.val64 = ctx->pval.p;
+    B.sptr = ctx->sell_ptr.p;
+    B.col = ctx->sell_col.p;
+    B.val = ctx->sell_v32.p;
+    B.val64 = ctx->sell_v64.p;
     B.b = ctx->b.p;
     B.scale = ctx->scale.p;
     B.y = ctx->yv.p;
-    float* v = ctx->vec32.p;
-    B.r = v;
-    B.r0 = v + m;
-    B.p[0] = v + 2 * (size_t)m;
-    B.p[1] = v + 3 * (size_t)m;
-    B.v[0] = v + 4 * (size_t)m;
-    B.v[1] = v + 5 * (size_t)m;
-    B.s = v + 6 * (size_t)m;
-    B.t = v + 7 * (size_t)m;
     B.partials = ctx->partials.p;
-    rc = run_bicgstab<float>(ctx, B, x0, p->max_iter, p->tol, SOLVE_RELIABLE | SOLVE_TRUE_CHECK, chunk, out);
+    bind_vectors(B, ctx->vec32.p, m);
+    rc = run_bicgstab<float>(ctx, B, x0, p->max_iter, p->tol, SOLVE_RELIABLE | SOLVE_TRUE_CHECK | SOLVE_RESTART,
+                             chunk, out);
     if (rc == LSMPS_OK && !out->converged) {
       // Fallback and judge: FP64 right-Jacobi BiCGStab, continuing from the
       // mixed iterate when it lowered the residual, else from zero.
@@ -779,66 +838,30 @@ int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_res
         HIPCHK(ctx->x0.ensure(m));
         HIPCHK(hipMemcpyAsync(ctx->x0.p, ctx->xsol.p, (size_t)m * sizeof(double), hipMemcpyDeviceToDevice, st));
       }
-      HIPCHK(ctx->vec64.ensure(8 * (size_t)m));
-      HIPCHK(ctx->val64s.ensure(nnz));
-      launch_scale_matrix<double>(ctx->pcol.p, ctx->pval.p, ctx->scale.p, nnz, ctx->val64s.p, st);
-      SolverBufs<double> D;
-      D.m = m;
-      D.rowp = ctx->prow.p;
-      D.col = ctx->pcol.p;
-      D.val = ctx->val64s.p;
-      D.val64 = ctx->pval.p;
-      D.b = ctx->b.p;
-      D.scale = ctx->scale.p;
-      D.y = ctx->yv.p;
-      double* v = ctx->vec64.p;
-      D.r = v;
-      D.r0 = v + m;
-      D.p[0] = v + 2 * (size_t)m;
-      D.p[1] = v + 3 * (size_t)m;
-      D.v[0] = v + 4 * (size_t)m;
-      D.v[1] = v + 5 * (size_t)m;
-      D.s = v + 6 * (size_t)m;
-      D.t = v + 7 * (size_t)m;
-      D.partials = ctx->partials.p;
-      rc = run_bicgstab<double>(ctx, D, resume ? ctx->x0.p : nullptr, p->max_iter, p->tol, SOLVE_TRUE_CHECK, chunk,
-                                out);
+      rc = solve_fp64_jacobi(ctx, resume ? ctx->x0.p : nullptr, p->max_iter, p->tol, chunk, out);
       out->iterations += first.iterations;
       out->replacements += first.replacements;
       out->initial_residual = first.initial_residual;
       out->fallback = 1;
     }
+  } else if (p->method == LSMPS_SOLVE_JACOBI) {
+    rc = solve_fp64_jacobi(ctx, x0, p->max_iter, p->tol, chunk, out);
   } else {
-    const bool jac = p->method == LSMPS_SOLVE_JACOBI;
-    HIPCHK(ctx->vec64.ensure(8 * (size_t)m));
+    HIPCHK(ctx->vec64.ensure(6 * (size_t)m));
+    launch_sell_fill<double>(ctx->prow.p, ctx->pcol.p, ctx->pval.p, nullptr, m, ctx->maxn + 1, ctx->sell_ptr.p, ctx->sell_col.p,
+                             ctx->sell_v64.p, nullptr, st);
     SolverBufs<double> B;
     B.m = m;
-    B.rowp = ctx->prow.p;
-    B.col = ctx->pcol.p;
-    B.val64 = ctx->pval.p;
+    B.sptr = ctx->sell_ptr.p;
+    B.col = ctx->sell_col.p;
+    B.val = ctx->sell_v64.p;
+    B.val64 = ctx->sell_v64.p;
     B.b = ctx->b.p;
-    if (jac) {
-      HIPCHK(ctx->val64s.ensure(nnz));
-      launch_inv_diag(ctx->diag.p, m, ctx->scale.p, st);
-      launch_scale_matrix<double>(ctx->pcol.p, ctx->pval.p, ctx->scale.p, nnz, ctx->val64s.p, st);
-      B.val = ctx->val64s.p;
-      B.scale = ctx->scale.p;
-    } else {
-      B.val = ctx->pval.p;
-      B.scale = nullptr;
-    }
+    B.scale = nullptr;
     B.y = ctx->yv.p;
-    double* v = ctx->vec64.p;
-    B.r = v;
-    B.r0 = v + m;
-    B.p[0] = v + 2 * (size_t)m;
-    B.p[1] = v + 3 * (size_t)m;
-    B.v[0] = v + 4 * (size_t)m;
-    B.v[1] = v + 5 * (size_t)m;
-    B.s = v + 6 * (size_t)m;
-    B.t = v + 7 * (size_t)m;
     B.partials = ctx->partials.p;
-    rc = run_bicgstab<double>(ctx, B, x0, p->max_iter, p->tol, jac ? SOLVE_TRUE_CHECK : SOLVE_S_EXIT, chunk, out);
+    bind_vectors(B, ctx->vec64.p, m);
+    rc = run_bicgstab<double>(ctx, B, x0, p->max_iter, p->tol, SOLVE_S_EXIT, chunk, out);
   }
   if (rc != LSMPS_OK) return rc;
   out->ms = elapsed(ctx);

@@ -75,45 +75,40 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   __syncthreads();
 }
 
-// Grid-wide reduction of NV doubles without a second launch: every block
-// stores its partial sums, the last block to finish (found with an atomic
-// ticket) adds the partials in block order, so the result is the same on
-// every run. Returns true in all threads of the last block, which then find
-// the totals in `tot` (valid in every thread). `ticket` must be zero on
-// entry; the last block resets it.
+// Grid-wide reductions are two launches: every block stores its partial
+// sums (block_partials), then a one-block kernel adds them in block order
+// (finalize_sum), so the result is the same on every run. (A last-block
+// ticket with a device-scope fence in every block would save the second
+// launch, but on this GPU each such fence writes back the XCD's L2 and costs
+// far more than a launch.)
 template <int NV>
-__device__ __forceinline__ bool grid_sum_last_block(double (&v)[NV], double* partials, unsigned* ticket,
-                                                     double (&tot)[NV], double* lds) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials, double* lds) {
   block_sum<NV>(v, lds);
-  __shared__ int last;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+}
+
+// In a one-block kernel: totals of the NV partial arrays written by a grid
+// of `nblk` blocks, valid in every thread.
+template <int NV>
+__device__ __forceinline__ void finalize_sum(const double* partials, int nblk, double (&tot)[NV], double* lds) {
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     acc[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-      acc[k] += __hip_atomic_load(&partials[(size_t)k * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc[k] += partials[(size_t)k * nblk + b];
   }
   block_sum<NV>(acc, lds);
   __shared__ double bc[NV];
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) bc[k] = acc[k];
-    atomicExch(ticket, 0u);
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = bc[k];
-  return true;
 }
 
 }  // namespace lsmps_hip

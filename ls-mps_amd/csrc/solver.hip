@@ -11,24 +11,38 @@
 // identity (the reference), 1/diag(A) (right Jacobi) or a learned diagonal.
 // The scaled matrix is built once per solve in the solve precision T.
 //
-// One iteration is three launches, each ending in a deterministic grid
-// reduction finished by the last block (no host round trip):
-//   k_bicg_pv  p_new = r + beta (p_old - omega v_old) formed on the fly at
-//              every column it gathers, v = A p_new, partial r0.v -> alpha
-//   k_bicg_st  s = r - alpha v formed on the fly, t = A s, partial s.s,
-//              t.t, t.s -> omega (or the early stop on ||s||)
+// The iteration matrix is stored sliced-ELL: rows in slices of 64 (one
+// wavefront), each slice padded to its longest row and stored column-major,
+// so one thread owns one row, value and column loads are fully coalesced,
+// and neighboring rows (neighboring particles) gather neighboring vector
+// entries. Padding entries hold value 0 and the row's own column.
+//
+// One iteration is five streaming launches and three one-block launches
+// that add the block partials in fixed order and update the
+// device-resident scalars (no host round trip):
+//   k_bicg_p   p = r + beta (p - omega v)
+//   k_bicg_v   v = A p, partial r0.v                  -> k_fin_alpha
+//   k_bicg_s   s = r - alpha v
+//   k_bicg_t   t = A s, partial s.s, t.t, t.s         -> k_fin_omega
 //   k_bicg_xr  y += alpha p + omega s (FP64), r = s - omega t, partial
-//              r.r, r0.r -> convergence, next rho and beta
-// p and v are double-buffered because rows of k_bicg_pv read p_old/v_old
-// at other rows while they write p_new/v_new.
+//              r.r, r0.r                              -> k_fin_iter
+// so each matrix entry gathers one vector entry. Columns inside a row are
+// stored in ascending order, so the 64 rows of a slice (64 neighboring
+// particles) gather neighboring entries.
 //
 // Mixed precision (T = float): matrix and Krylov vectors in FP32, every dot
 // product accumulated in FP64, y in FP64, and reliable updates: when the
 // recurrence residual has dropped 10x since the last replacement, or when
-// it meets the tolerance, k_bicg_replace recomputes r = b - A S y from the
-// FP64 matrix and FP64 y, so the stopping test is on the true FP64
-// residual.
+// it meets the tolerance, k_bicg_true_residual recomputes r = b - A S y from
+// the FP64 matrix and FP64 y, so the stopping test is on the true FP64
+// residual. With SOLVE_RESTART the FP32 Krylov space then restarts from the
+// true residual (r0 = r, p = v = 0): each restart is one step of iterative
+// refinement whose inner FP32 solve reduces the residual 10x. (Without the
+// restart, the FP32 recurrences drift away from the replaced residual and
+// BiCGStab stalls on these ill-conditioned Laplacians.)
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
 
 #include "kernels.hpp"
 #include "solver.hpp"
@@ -36,6 +50,7 @@
 namespace lsmps_hip {
 
 constexpr int SOLVER_THREADS = 256;
+constexpr int SLICE = 64;
 
 __device__ __forceinline__ bool iter_active(const SolverScalars* sc) {
   return sc->state == SOLVE_RUNNING && sc->it < sc->max_iter;
@@ -81,35 +96,63 @@ __device__ void finish_iteration(SolverScalars* sc, double rr, double r0r) {
   if (sc->it >= sc->max_iter) sc->state = SOLVE_MAXITER;
 }
 
-template <typename T, int G>
-__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_pv(const int* __restrict__ rowp, const int* __restrict__ col,
-                                                            const T* __restrict__ val, int m, const T* __restrict__ r,
-                                                            const T* __restrict__ r0, const T* __restrict__ p_old,
-                                                            const T* __restrict__ v_old, T* __restrict__ p_new,
-                                                            T* __restrict__ v_new, SolverScalars* sc,
-                                                            double* __restrict__ partials) {
+// Row `row` of a sliced-ELL matrix: first element index and width.
+struct SellRow {
+  int base, width;
+};
+__device__ __forceinline__ SellRow sell_row(const int* __restrict__ sptr, int row) {
+  const int sl = row / SLICE;
+  const int b = sptr[sl];
+  return {b + (row - sl * SLICE), (sptr[sl + 1] - b) / SLICE};
+}
+
+template <typename T>
+__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_p(int m, const T* __restrict__ r, const T* __restrict__ v,
+                                                           T* __restrict__ p, const SolverScalars* sc) {
+  if (!iter_active(sc)) return;
+  const T beta = (T)sc->beta, omega = (T)sc->omega;
+  for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS)
+    p[i] = fma(beta, fma(-omega, v[i], p[i]), r[i]);
+}
+
+// y = A x on the sliced-ELL matrix, one thread per row; returns the row's value.
+template <typename T>
+__device__ __forceinline__ T sell_dot(const int* __restrict__ sptr, const int* __restrict__ col,
+                                      const T* __restrict__ val, const T* __restrict__ x, int row) {
+  const SellRow R = sell_row(sptr, row);
+  T acc = 0;
+#pragma unroll 4
+  for (int k = 0; k < R.width; ++k) {
+    const int e = R.base + k * SLICE;
+    acc = fma(val[e], x[col[e]], acc);
+  }
+  return acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_v(const int* __restrict__ sptr, const int* __restrict__ col,
+                                                           const T* __restrict__ val, int m, const T* __restrict__ p,
+                                                           const T* __restrict__ r0, T* __restrict__ v,
+                                                           SolverScalars* sc, double* __restrict__ partials) {
   if (!iter_active(sc)) return;
   __shared__ double lds[SOLVER_THREADS / WAVE];
-  const T beta = (T)sc->beta, omega = (T)sc->omega;
-  const int lane = threadIdx.x % G, grp = threadIdx.x / G, gpb = SOLVER_THREADS / G;
   double dot = 0.0;
-  for (int row = blockIdx.x * gpb + grp; row < m; row += gridDim.x * gpb) {
-    T acc = 0;
-    const int e = rowp[row + 1];
-    for (int k = rowp[row] + lane; k < e; k += G) {
-      const int c = col[k];
-      const T pj = fma(beta, fma(-omega, v_old[c], p_old[c]), r[c]);
-      acc = fma(val[k], pj, acc);
-    }
-    acc = group_sum<G>(acc);
-    if (lane == 0) {
-      p_new[row] = fma(beta, fma(-omega, v_old[row], p_old[row]), r[row]);
-      v_new[row] = acc;
-      dot += (double)r0[row] * (double)acc;
-    }
+  for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
+    const T acc = sell_dot(sptr, col, val, p, row);
+    v[row] = acc;
+    dot += (double)r0[row] * (double)acc;
   }
-  double v[1] = {dot}, tot[1];
-  if (grid_sum_last_block<1>(v, partials, &sc->ticket[0], tot, lds) && threadIdx.x == 0) {
+  double vv[1] = {dot};
+  block_partials<1>(vv, partials, lds);
+}
+
+__global__ __launch_bounds__(SOLVER_THREADS) void k_fin_alpha(const double* __restrict__ partials, int nblk,
+                                                              SolverScalars* sc) {
+  if (!iter_active(sc)) return;
+  __shared__ double lds[SOLVER_THREADS / WAVE];
+  double tot[1];
+  finalize_sum<1>(partials, nblk, tot, lds);
+  if (threadIdx.x == 0) {
     const double den = tot[0];
     if (fabs(den) < 1e-30 || !isfinite(den)) {
       sc->state = SOLVE_BREAKDOWN;
@@ -119,36 +162,42 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_pv(const int* __restric
   }
 }
 
-template <typename T, int G>
-__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_st(const int* __restrict__ rowp, const int* __restrict__ col,
-                                                            const T* __restrict__ val, int m, const T* __restrict__ r,
-                                                            const T* __restrict__ v, T* __restrict__ s,
-                                                            T* __restrict__ t, SolverScalars* sc,
-                                                            double* __restrict__ partials) {
+template <typename T>
+__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_s(int m, const T* __restrict__ r, const T* __restrict__ v,
+                                                           T* __restrict__ s, const SolverScalars* sc) {
+  if (!iter_active(sc)) return;
+  const T alpha = (T)sc->alpha;
+  for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS)
+    s[i] = fma(-alpha, v[i], r[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_t(const int* __restrict__ sptr, const int* __restrict__ col,
+                                                           const T* __restrict__ val, int m, const T* __restrict__ s,
+                                                           T* __restrict__ t, SolverScalars* sc,
+                                                           double* __restrict__ partials) {
   if (!iter_active(sc)) return;
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
-  const T alpha = (T)sc->alpha;
-  const int lane = threadIdx.x % G, grp = threadIdx.x / G, gpb = SOLVER_THREADS / G;
   double ss = 0.0, tt = 0.0, ts = 0.0;
-  for (int row = blockIdx.x * gpb + grp; row < m; row += gridDim.x * gpb) {
-    T acc = 0;
-    const int e = rowp[row + 1];
-    for (int k = rowp[row] + lane; k < e; k += G) {
-      const int c = col[k];
-      acc = fma(val[k], fma(-alpha, v[c], r[c]), acc);
-    }
-    acc = group_sum<G>(acc);
-    if (lane == 0) {
-      const T si = fma(-alpha, v[row], r[row]);
-      s[row] = si;
-      t[row] = acc;
-      ss += (double)si * (double)si;
-      tt += (double)acc * (double)acc;
-      ts += (double)acc * (double)si;
-    }
+  for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
+    const T acc = sell_dot(sptr, col, val, s, row);
+    const T si = s[row];
+    t[row] = acc;
+    ss += (double)si * (double)si;
+    tt += (double)acc * (double)acc;
+    ts += (double)acc * (double)si;
   }
-  double vv[3] = {ss, tt, ts}, tot[3];
-  if (grid_sum_last_block<3>(vv, partials, &sc->ticket[1], tot, lds) && threadIdx.x == 0) {
+  double vv[3] = {ss, tt, ts};
+  block_partials<3>(vv, partials, lds);
+}
+
+__global__ __launch_bounds__(SOLVER_THREADS) void k_fin_omega(const double* __restrict__ partials, int nblk,
+                                                              SolverScalars* sc) {
+  if (!iter_active(sc)) return;
+  __shared__ double lds[3 * SOLVER_THREADS / WAVE];
+  double tot[3];
+  finalize_sum<3>(partials, nblk, tot, lds);
+  if (threadIdx.x == 0) {
     const double sn = sqrt(tot[0]);
     sc->sn = sn;
     if ((sc->flags & SOLVE_S_EXIT) && sn <= sc->thresh) {
@@ -185,8 +234,18 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const T* __re
       r0r += (double)r0[i] * (double)ri;
     }
   }
-  double vv[2] = {rr, r0r}, tot[2];
-  if (grid_sum_last_block<2>(vv, partials, &sc->ticket[2], tot, lds) && threadIdx.x == 0) {
+  double vv[2] = {rr, r0r};
+  block_partials<2>(vv, partials, lds);
+}
+
+__global__ __launch_bounds__(SOLVER_THREADS) void k_fin_iter(const double* __restrict__ partials, int nblk,
+                                                             SolverScalars* sc) {
+  const bool sconv = sc->state == SOLVE_S_CONVERGED;
+  if (!(sconv || iter_active(sc))) return;
+  __shared__ double lds[2 * SOLVER_THREADS / WAVE];
+  double tot[2];
+  finalize_sum<2>(partials, nblk, tot, lds);
+  if (threadIdx.x == 0) {
     if (sconv) {
       sc->state = SOLVE_CONVERGED;
       sc->it += 1;
@@ -196,48 +255,61 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const T* __re
   }
 }
 
-// r = b - A S y with the FP64 matrix and FP64 y; optional r0 = r and zeroed
-// p/v (INIT). In replace mode it then sets rho/beta (or stops when the
-// true residual meets the tolerance).
-template <typename T, int G, bool INIT>
+// r = b - A S y with the FP64 matrix and FP64 y. INIT: also r0 = r, p = v
+// = 0 and the scalars of a fresh solve. Otherwise runs only when a
+// replacement is pending: sets rho/beta, stops when the true residual meets
+// the tolerance, and with SOLVE_RESTART restarts the Krylov space.
+template <typename T, bool INIT>
 __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_true_residual(
-    const int* __restrict__ rowp, const int* __restrict__ col, const double* __restrict__ val, int m,
+    const int* __restrict__ sptr, const int* __restrict__ col, const double* __restrict__ val, int m,
     const double* __restrict__ b, const double* __restrict__ scale, const double* __restrict__ y, T* __restrict__ r,
-    T* __restrict__ r0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ v0, T* __restrict__ v1,
-    SolverScalars* sc, double* __restrict__ partials) {
+    T* __restrict__ r0, T* __restrict__ p, T* __restrict__ v, SolverScalars* sc, double* __restrict__ partials) {
   if (!INIT && sc->replace == 0) return;
+  const bool restart = !INIT && (sc->flags & SOLVE_RESTART);
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
-  const int lane = threadIdx.x % G, grp = threadIdx.x / G, gpb = SOLVER_THREADS / G;
   double rr = 0.0, r0r = 0.0, bb = 0.0;
-  for (int row = blockIdx.x * gpb + grp; row < m; row += gridDim.x * gpb) {
+  for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
     double acc = 0.0;
     if (y) {
-      const int e = rowp[row + 1];
-      for (int k = rowp[row] + lane; k < e; k += G) {
-        const int c = col[k];
-        const double yc = scale ? scale[c] * y[c] : y[c];
-        acc = fma(val[k], yc, acc);
+      const SellRow R = sell_row(sptr, row);
+      for (int k = 0; k < R.width; ++k) {
+        const int e = R.base + k * SLICE;
+        const int c = col[e];
+        acc = fma(val[e], scale ? scale[c] * y[c] : y[c], acc);
       }
-      acc = group_sum<G>(acc);
     }
-    if (lane == 0) {
-      const double bi = b[row];
-      const double ri = bi - acc;
-      const T rt = (T)ri;
-      r[row] = rt;
-      if (INIT) {
-        r0[row] = rt;
-        p0[row] = p1[row] = v0[row] = v1[row] = (T)0;
-        bb += bi * bi;
-        rr += (double)rt * (double)rt;
-      } else {
-        rr += ri * ri;
-        r0r += (double)r0[row] * (double)rt;
-      }
+    const double bi = b[row];
+    const double ri = bi - acc;
+    const T rt = (T)ri;
+    r[row] = rt;
+    if (INIT) {
+      r0[row] = rt;
+      p[row] = v[row] = (T)0;
+      bb += bi * bi;
+      rr += (double)rt * (double)rt;
+    } else if (restart) {
+      r0[row] = rt;
+      p[row] = v[row] = (T)0;
+      rr += ri * ri;
+      r0r += (double)rt * (double)rt;
+    } else {
+      rr += ri * ri;
+      r0r += (double)r0[row] * (double)rt;
     }
   }
-  double vv[3] = {rr, r0r, bb}, tot[3];
-  if (grid_sum_last_block<3>(vv, partials, &sc->ticket[3], tot, lds) && threadIdx.x == 0) {
+  double vv[3] = {rr, r0r, bb};
+  block_partials<3>(vv, partials, lds);
+}
+
+template <bool INIT>
+__global__ __launch_bounds__(SOLVER_THREADS) void k_fin_residual(const double* __restrict__ partials, int nblk,
+                                                                 SolverScalars* sc) {
+  if (!INIT && sc->replace == 0) return;
+  const bool restart = !INIT && (sc->flags & SOLVE_RESTART);
+  __shared__ double lds[3 * SOLVER_THREADS / WAVE];
+  double tot[3];
+  finalize_sum<3>(partials, nblk, tot, lds);
+  if (threadIdx.x == 0) {
     if (INIT) {
       sc->bnorm = sqrt(tot[2]);
       sc->thresh = sc->tol * (sc->bnorm > 1.0 ? sc->bnorm : 1.0);
@@ -269,6 +341,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_true_residual(
         return;
       }
       sc->rho = tot[1];
+      if (restart) sc->rho_prev = sc->alpha = sc->omega = 1.0;
       if (fabs(sc->rho) < 1e-30 || fabs(sc->omega) < 1e-30) {
         sc->state = SOLVE_BREAKDOWN;
         return;
@@ -280,8 +353,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_true_residual(
 }
 
 // x = S y; residual ||A x - b||, error against x*, max |x - x*|.
-template <int G>
-__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_final(const int* __restrict__ rowp, const int* __restrict__ col,
+__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_final(const int* __restrict__ sptr, const int* __restrict__ col,
                                                                const double* __restrict__ val, int m,
                                                                const double* __restrict__ b,
                                                                const double* __restrict__ scale,
@@ -289,44 +361,99 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_final(const int* __rest
                                                                const double* __restrict__ xe, double* __restrict__ x,
                                                                SolverScalars* sc, double* __restrict__ partials) {
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
-  const int lane = threadIdx.x % G, grp = threadIdx.x / G, gpb = SOLVER_THREADS / G;
   double res = 0.0, se = 0.0, xn = 0.0, mx = 0.0;
-  for (int row = blockIdx.x * gpb + grp; row < m; row += gridDim.x * gpb) {
+  for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
+    const SellRow R = sell_row(sptr, row);
     double acc = 0.0;
-    const int e = rowp[row + 1];
-    for (int k = rowp[row] + lane; k < e; k += G) {
-      const int c = col[k];
-      acc = fma(val[k], scale ? scale[c] * y[c] : y[c], acc);
+    for (int k = 0; k < R.width; ++k) {
+      const int e = R.base + k * SLICE;
+      const int c = col[e];
+      acc = fma(val[e], scale ? scale[c] * y[c] : y[c], acc);
     }
-    acc = group_sum<G>(acc);
-    if (lane == 0) {
-      const double d = acc - b[row];
-      res += d * d;
-      const double xi = scale ? scale[row] * y[row] : y[row];
-      x[row] = xi;
-      if (xe) {
-        const double er = xi - xe[row];
-        se += er * er;
-        xn += xe[row] * xe[row];
-        mx = fabs(er) > mx ? fabs(er) : mx;
-      }
+    const double d = acc - b[row];
+    res += d * d;
+    const double xi = scale ? scale[row] * y[row] : y[row];
+    x[row] = xi;
+    if (xe) {
+      const double er = xi - xe[row];
+      se += er * er;
+      xn += xe[row] * xe[row];
+      mx = fabs(er) > mx ? fabs(er) : mx;
     }
   }
   mx = wave_max(mx);
   if ((threadIdx.x & (WAVE - 1)) == 0 && mx > 0.0)
     atomicMax(reinterpret_cast<unsigned long long*>(&sc->max_err), (unsigned long long)__double_as_longlong(mx));
-  double vv[3] = {res, se, xn}, tot[3];
-  if (grid_sum_last_block<3>(vv, partials, &sc->ticket[4], tot, lds) && threadIdx.x == 0) {
+  double vv[3] = {res, se, xn};
+  block_partials<3>(vv, partials, lds);
+}
+
+__global__ __launch_bounds__(SOLVER_THREADS) void k_fin_final(const double* __restrict__ partials, int nblk,
+                                                              SolverScalars* sc) {
+  __shared__ double lds[3 * SOLVER_THREADS / WAVE];
+  double tot[3];
+  finalize_sum<3>(partials, nblk, tot, lds);
+  if (threadIdx.x == 0) {
     sc->final_res = sqrt(tot[0]);
     sc->rel_l2 = sqrt(tot[1] / (tot[2] > 1e-30 ? tot[2] : 1e-30));
   }
 }
 
+// ---- sliced-ELL construction from the assembled CSR -------------------
+__global__ __launch_bounds__(256) void k_sell_widths(const int* __restrict__ prow, int m, int nslices,
+                                                     int* __restrict__ wsz) {
+  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslices) return;
+  int w = 0;
+  for (int l = 0; l < SLICE; ++l) {
+    const int row = sl * SLICE + l;
+    if (row < m) w = max(w, prow[row + 1] - prow[row]);
+  }
+  wsz[sl] = w * SLICE;
+}
+
+// Copies the CSR rows into their slices in ascending column order, scaled
+// by S at the columns (values in T), with the unscaled FP64 values when
+// val64 is given, and pads with (row, 0). One wavefront per row: the row's
+// columns are staged in LDS and each entry's slot is its rank (columns in a
+// row are distinct).
+constexpr int FILL_WAVES = 4;
 template <typename T>
-__global__ __launch_bounds__(256) void k_scale_matrix(const int* __restrict__ col, const double* __restrict__ val,
-                                                      const double* __restrict__ scale, int64_t nnz, T* __restrict__ out) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nnz) out[k] = (T)(scale ? val[k] * scale[col[k]] : val[k]);
+__global__ __launch_bounds__(FILL_WAVES * WAVE) void k_sell_fill(const int* __restrict__ prow,
+                                                                  const int* __restrict__ pcol,
+                                                                  const double* __restrict__ pval,
+                                                                  const double* __restrict__ scale, int m, int maxlen,
+                                                                  const int* __restrict__ sptr, int* __restrict__ col,
+                                                                  T* __restrict__ val, double* __restrict__ val64) {
+  extern __shared__ int sh[];  // FILL_WAVES * maxlen columns
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  int* rc = sh + w * maxlen;
+  // Block-uniform trip count, so the barriers below are reached by all.
+  for (int base = blockIdx.x * FILL_WAVES; base < m; base += gridDim.x * FILL_WAVES) {
+    const int row = base + w;
+    const bool live = row < m;
+    const SellRow R = live ? sell_row(sptr, row) : SellRow{0, 0};
+    const int b = live ? prow[row] : 0, len = live ? prow[row + 1] - b : 0;
+    for (int q = lane; q < len; q += WAVE) rc[q] = pcol[b + q];
+    __syncthreads();
+    for (int q = lane; q < len; q += WAVE) {
+      const int c = rc[q];
+      int rank = 0;
+      for (int u = 0; u < len; ++u) rank += rc[u] < c ? 1 : 0;
+      const int e = R.base + rank * SLICE;
+      const double a = pval[b + q];
+      col[e] = c;
+      val[e] = (T)(scale ? a * scale[c] : a);
+      if (val64) val64[e] = a;
+    }
+    for (int k = len + lane; k < R.width; k += WAVE) {
+      const int e = R.base + k * SLICE;
+      col[e] = row;
+      val[e] = (T)0;
+      if (val64) val64[e] = 0.0;
+    }
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(256) void k_inv_diag(const double* __restrict__ diag, int m, double* __restrict__ s) {
@@ -349,29 +476,41 @@ __global__ void k_solver_setup(SolverScalars* sc, int max_iter, double tol, int 
     sc->delta = delta;
     sc->max_err = 0.0;
     sc->state = SOLVE_RUNNING;
-    for (int k = 0; k < 8; ++k) sc->ticket[k] = 0u;
   }
 }
 
 static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
-constexpr int SPMV_G = 8;
 
 unsigned solver_grid(int m) {
-  const unsigned rows_per_block = SOLVER_THREADS / SPMV_G;
-  unsigned g = blocks(m, rows_per_block);
+  unsigned g = blocks(m, SOLVER_THREADS);
   return g < 1 ? 1 : (g > SOLVER_MAX_GRID ? SOLVER_MAX_GRID : g);
 }
+
+int sell_slices(int m) { return (m + SLICE - 1) / SLICE; }
+
+void launch_sell_widths(const int* prow, int m, int* wsz, hipStream_t st) {
+  const int ns = sell_slices(m);
+  if (ns > 0) hipLaunchKernelGGL(k_sell_widths, dim3(blocks(ns, 256)), dim3(256), 0, st, prow, m, ns, wsz);
+}
+
+template <typename T>
+void launch_sell_fill(const int* prow, const int* pcol, const double* pval, const double* scale, int m, int maxlen,
+                      const int* sptr, int* col, T* val, double* val64, hipStream_t st) {
+  if (m <= 0) return;
+  unsigned g = blocks(m, FILL_WAVES);
+  g = g > 8192u ? 8192u : g;
+  const size_t lds = (size_t)FILL_WAVES * (size_t)(maxlen > 0 ? maxlen : 1) * sizeof(int);
+  hipLaunchKernelGGL(k_sell_fill<T>, dim3(g), dim3(FILL_WAVES * WAVE), lds, st, prow, pcol, pval, scale, m, maxlen,
+                     sptr, col, val, val64);
+}
+template void launch_sell_fill<float>(const int*, const int*, const double*, const double*, int, int, const int*,
+                                      int*, float*, double*, hipStream_t);
+template void launch_sell_fill<double>(const int*, const int*, const double*, const double*, int, int, const int*,
+                                       int*, double*, double*, hipStream_t);
 
 void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st) {
   if (m > 0) hipLaunchKernelGGL(k_inv_diag, dim3(blocks(m, 256)), dim3(256), 0, st, diag, m, s);
 }
-
-template <typename T>
-void launch_scale_matrix(const int* col, const double* val, const double* scale, int64_t nnz, T* out, hipStream_t st) {
-  if (nnz > 0) hipLaunchKernelGGL(k_scale_matrix<T>, dim3(blocks(nnz, 256)), dim3(256), 0, st, col, val, scale, nnz, out);
-}
-template void launch_scale_matrix<float>(const int*, const double*, const double*, int64_t, float*, hipStream_t);
-template void launch_scale_matrix<double>(const int*, const double*, const double*, int64_t, double*, hipStream_t);
 
 template <typename T>
 void launch_solver_init(const SolverBufs<T>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,
@@ -379,40 +518,47 @@ void launch_solver_init(const SolverBufs<T>& B, const double* x0, SolverScalars*
   const int m = B.m;
   hipLaunchKernelGGL(k_solver_setup, dim3(1), dim3(64), 0, st, sc, max_iter, tol, flags, delta);
   if (m > 0) hipLaunchKernelGGL(k_y_from_x, dim3(blocks(m, 256)), dim3(256), 0, st, x0, B.scale, m, B.y);
-  hipLaunchKernelGGL((k_bicg_true_residual<T, SPMV_G, true>), dim3(solver_grid(m)), dim3(SOLVER_THREADS), 0, st,
-                     B.rowp, B.col, B.val64, m, B.b, B.scale, x0 ? (const double*)B.y : nullptr, B.r, B.r0, B.p[0],
-                     B.p[1], B.v[0], B.v[1], sc, B.partials);
+  const unsigned g = solver_grid(m);
+  hipLaunchKernelGGL((k_bicg_true_residual<T, true>), dim3(g), dim3(SOLVER_THREADS), 0, st, B.sptr, B.col, B.val64, m,
+                     B.b, B.scale, x0 ? (const double*)B.y : nullptr, B.r, B.r0, B.p, B.v, sc, B.partials);
+  hipLaunchKernelGGL(k_fin_residual<true>, dim3(1), dim3(SOLVER_THREADS), 0, st, B.partials, (int)g, sc);
 }
 
 template <typename T>
-void launch_solver_iteration(const SolverBufs<T>& B, int parity, bool with_replace, SolverScalars* sc, hipStream_t st) {
+void launch_solver_iteration(const SolverBufs<T>& B, bool with_replace, SolverScalars* sc, hipStream_t st) {
   const int m = B.m;
   const unsigned g = solver_grid(m);
-  hipLaunchKernelGGL((k_bicg_pv<T, SPMV_G>), dim3(g), dim3(SOLVER_THREADS), 0, st, B.rowp, B.col, B.val, m, B.r, B.r0,
-                     B.p[parity], B.v[parity], B.p[parity ^ 1], B.v[parity ^ 1], sc, B.partials);
-  hipLaunchKernelGGL((k_bicg_st<T, SPMV_G>), dim3(g), dim3(SOLVER_THREADS), 0, st, B.rowp, B.col, B.val, m, B.r,
-                     B.v[parity ^ 1], B.s, B.t, sc, B.partials);
-  hipLaunchKernelGGL((k_bicg_xr<T>), dim3(g), dim3(SOLVER_THREADS), 0, st, m, B.p[parity ^ 1], B.s, B.t, B.r0, B.r, B.y,
-                     sc, B.partials);
-  if (with_replace)
-    hipLaunchKernelGGL((k_bicg_true_residual<T, SPMV_G, false>), dim3(g), dim3(SOLVER_THREADS), 0, st, B.rowp, B.col,
-                       B.val64, m, B.b, B.scale, (const double*)B.y, B.r, B.r0, (T*)nullptr, (T*)nullptr,
-                       (T*)nullptr, (T*)nullptr, sc, B.partials);
+  const dim3 one(1), blk(SOLVER_THREADS), grid(g);
+  hipLaunchKernelGGL((k_bicg_p<T>), grid, blk, 0, st, m, B.r, B.v, B.p, sc);
+  hipLaunchKernelGGL((k_bicg_v<T>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.p, B.r0, B.v, sc, B.partials);
+  hipLaunchKernelGGL(k_fin_alpha, one, blk, 0, st, B.partials, (int)g, sc);
+  hipLaunchKernelGGL((k_bicg_s<T>), grid, blk, 0, st, m, B.r, B.v, B.s, sc);
+  hipLaunchKernelGGL((k_bicg_t<T>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.s, B.t, sc, B.partials);
+  hipLaunchKernelGGL(k_fin_omega, one, blk, 0, st, B.partials, (int)g, sc);
+  hipLaunchKernelGGL((k_bicg_xr<T>), grid, blk, 0, st, m, B.p, B.s, B.t, B.r0, B.r, B.y, sc, B.partials);
+  hipLaunchKernelGGL(k_fin_iter, one, blk, 0, st, B.partials, (int)g, sc);
+  if (with_replace) {
+    hipLaunchKernelGGL((k_bicg_true_residual<T, false>), grid, blk, 0, st, B.sptr, B.col, B.val64, m, B.b, B.scale,
+                       (const double*)B.y, B.r, B.r0, B.p, B.v, sc, B.partials);
+    hipLaunchKernelGGL(k_fin_residual<false>, one, blk, 0, st, B.partials, (int)g, sc);
+  }
 }
 
 template <typename T>
 void launch_solver_final(const SolverBufs<T>& B, const double* xe, double* x, SolverScalars* sc, hipStream_t st) {
   const int m = B.m;
-  hipLaunchKernelGGL((k_bicg_final<SPMV_G>), dim3(solver_grid(m)), dim3(SOLVER_THREADS), 0, st, B.rowp, B.col, B.val64,
-                     m, B.b, B.scale, (const double*)B.y, xe, x, sc, B.partials);
+  const unsigned g = solver_grid(m);
+  hipLaunchKernelGGL(k_bicg_final, dim3(g), dim3(SOLVER_THREADS), 0, st, B.sptr, B.col, B.val64, m, B.b, B.scale,
+                     (const double*)B.y, xe, x, sc, B.partials);
+  hipLaunchKernelGGL(k_fin_final, dim3(1), dim3(SOLVER_THREADS), 0, st, B.partials, (int)g, sc);
 }
 
 template void launch_solver_init<float>(const SolverBufs<float>&, const double*, SolverScalars*, int, double, int, double,
                                         hipStream_t);
 template void launch_solver_init<double>(const SolverBufs<double>&, const double*, SolverScalars*, int, double, int,
                                          double, hipStream_t);
-template void launch_solver_iteration<float>(const SolverBufs<float>&, int, bool, SolverScalars*, hipStream_t);
-template void launch_solver_iteration<double>(const SolverBufs<double>&, int, bool, SolverScalars*, hipStream_t);
+template void launch_solver_iteration<float>(const SolverBufs<float>&, bool, SolverScalars*, hipStream_t);
+template void launch_solver_iteration<double>(const SolverBufs<double>&, bool, SolverScalars*, hipStream_t);
 template void launch_solver_final<float>(const SolverBufs<float>&, const double*, double*, SolverScalars*, hipStream_t);
 template void launch_solver_final<double>(const SolverBufs<double>&, const double*, double*, SolverScalars*,
                                           hipStream_t);

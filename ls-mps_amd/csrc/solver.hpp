@@ -17,10 +17,11 @@ enum : int {
   SOLVE_S_EXIT = 1,      // stop on ||s|| as the reference does
   SOLVE_RELIABLE = 2,    // replace the recurrence residual by b - A x (FP64) on 10x drops
   SOLVE_TRUE_CHECK = 4,  // confirm convergence with the true FP64 residual
+  SOLVE_RESTART = 8,     // restart the Krylov space (r0 = r, p = v = 0) at each replacement
 };
 
-// Read by every block at kernel start, written only by the last block of
-// a kernel (after all others have finished), so no intra-kernel races.
+// Read by every block of the streaming kernels, written only by the
+// one-block finalize kernels between them, so no intra-kernel races.
 struct SolverScalars {
   double rho, rho_prev, alpha, omega, beta;
   double tol, bnorm, thresh, init_res;
@@ -28,40 +29,45 @@ struct SolverScalars {
   double final_res, rel_l2, max_err;
   int it, max_iter, state, iterations;
   int flags, replace, n_replace, pad;
-  unsigned ticket[8];
 };
 
-constexpr unsigned SOLVER_MAX_GRID = 2048;  // 8 blocks of 256 per CU on 256 CUs
+constexpr unsigned SOLVER_MAX_GRID = 1024;  // 4 blocks of 256 per CU on 256 CUs, grid-stride
 
+// Iteration matrix in sliced-ELL form (slices of 64 rows, column-major,
+// padded with (row, 0)), plus the vectors of one solve.
 template <typename T>
 struct SolverBufs {
   int m = 0;
-  const int* rowp = nullptr;
-  const int* col = nullptr;
+  const int* sptr = nullptr;       // slice offsets, ceil(m/64) + 1 entries
+  const int* col = nullptr;        // padded columns
   const T* val = nullptr;          // A S in precision T
-  const double* val64 = nullptr;   // A in FP64 (true residuals)
+  const double* val64 = nullptr;   // A in FP64 (true residuals), same layout
   const double* b = nullptr;
   const double* scale = nullptr;   // S (nullptr = identity)
   double* y = nullptr;             // FP64 iterate of A S y = b
   T* r = nullptr;
   T* r0 = nullptr;
-  T* p[2] = {nullptr, nullptr};
-  T* v[2] = {nullptr, nullptr};
+  T* p = nullptr;
+  T* v = nullptr;
   T* s = nullptr;
   T* t = nullptr;
   double* partials = nullptr;      // >= 3 * SOLVER_MAX_GRID doubles
 };
 
 unsigned solver_grid(int m);
-void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st);
+int sell_slices(int m);
+// wsz[s] = 64 * (longest row of slice s); scan it into slice offsets.
+void launch_sell_widths(const int* prow, int m, int* wsz, hipStream_t st);
+// maxlen: an upper bound of the row lengths (sizes the LDS staging).
 template <typename T>
-void launch_scale_matrix(const int* col, const double* val, const double* scale, int64_t nnz, T* out, hipStream_t st);
+void launch_sell_fill(const int* prow, const int* pcol, const double* pval, const double* scale, int m, int maxlen,
+                      const int* sptr, int* col, T* val, double* val64, hipStream_t st);
+void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st);
 template <typename T>
 void launch_solver_init(const SolverBufs<T>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,
                         int flags, double delta, hipStream_t st);
 template <typename T>
-void launch_solver_iteration(const SolverBufs<T>& B, int parity, bool with_replace, SolverScalars* sc,
-                             hipStream_t st);
+void launch_solver_iteration(const SolverBufs<T>& B, bool with_replace, SolverScalars* sc, hipStream_t st);
 template <typename T>
 void launch_solver_final(const SolverBufs<T>& B, const double* xe, double* x, SolverScalars* sc, hipStream_t st);
 

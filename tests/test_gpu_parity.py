@@ -2,21 +2,24 @@
 the CPU oracle. Every call goes through the C ABI (lsmps_amd -> liblsmps_hip.so).
 
 Bar: bit-exact for neighbor lists, types, PPE structure/values/b and the FP64
-WLS operator; BiCGStab by tolerance. The GPU's parallel dot products round
-differently from the reference's sequential ones, and BiCGStab's iteration
-count on these pinned graph Laplacians is sensitive to that: the reference's
-own algorithm, rerun with numpy's pairwise dot or with 256-element blocked
-dots, moves from 704 to 558 and 592 iterations (bulk_trunc_1500) and from
-110 to 130 (film_1200); the GPU, which also fuses multiply-adds, tends to
-need fewer. So: same convergence flag, iterations between half and 1.3x the
-reference's, relative L2 error against x* no worse than 3x the reference's.
+WLS operator; BiCGStab by tolerance. BiCGStab's iteration count and final
+error on these pinned graph Laplacians are chaotic in the last bits of the
+dot products, and the GPU sums them in parallel (with fused multiply-adds).
+tests/golden/solver_bands.json holds the spread of the reference's own
+algorithm when every dot product is perturbed by one rounding (40 seeded
+trials, make_solver_bands.py): e.g. 306..526 iterations for rain_2000 whose
+unperturbed count is 356. The GPU must converge, land in that iteration
+range (15% margin) and have a relative L2 error against x* no larger than
+1.5x the band's largest.
 """
 import hashlib
+import json
+import os
 import sys
 
 import numpy as np
 import pytest
-from conftest import SMALL_FIXTURES, load_fixture
+from conftest import GOLDEN, SMALL_FIXTURES, load_fixture
 
 import lsmps_amd as L
 import oracle_py as O
@@ -40,12 +43,16 @@ def pipe():
     p.close()
 
 
+BANDS = json.load(open(os.path.join(GOLDEN, "solver_bands.json")))
+
+
 def check_solve(st, ref, what):
     assert st["converged"] == ref["converged"], (what, st, ref)
     if ref["converged"]:
-        it, rit = st["iterations"], ref["iterations"]
-        assert 0.5 * rit - 3 <= it <= 1.3 * rit + 3, (what, it, rit)
-        assert st["relative_l2_error"] <= 3.0 * ref["relative_l2_error"], (what, st, ref)
+        b = BANDS[what]
+        it = st["iterations"]
+        assert 0.85 * b["iter_min"] - 3 <= it <= 1.15 * b["iter_max"] + 3, (what, it, b)
+        assert st["relative_l2_error"] <= 1.5 * b["rel_l2_max"], (what, st["relative_l2_error"], b)
 
 
 @pytest.mark.parametrize("name", SMALL_FIXTURES)
@@ -121,7 +128,7 @@ def test_reference_configs(pipe, ref_configs):
         for k in ("row", "col", "val", "b", "x_exact"):
             assert sha(A[k]) == rec["sha"]["ppe_" + k], (name, k)
         st = pipe.solve(L.SOLVE_PLAIN, int(c["ppe_max_iter"]), float(c["ppe_tol"]))
-        check_solve(st, rec["solve"], name)
+        check_solve(st, rec["solve"], "config:" + name)
 
 
 def test_neighbors_against_oracle_random_sets(pipe):
@@ -216,7 +223,11 @@ def test_solvers_reach_tolerance_and_agree(pipe):
         np.add.at(ax, rows, A["val"] * x[A["col"]])
         res = np.linalg.norm(ax - A["b"])
         assert res <= 1e-10 * max(1.0, bnorm) * (1.0 if method != L.SOLVE_PLAIN else 10.0)
-        assert np.linalg.norm(x - ox) <= 1e-6 * np.linalg.norm(ox)
+        # |x - x_oracle| <= cond(A) * (residual tolerance) with cond ~1e5 for this
+        # pinned Laplacian; FP64 paths land far inside, mixed (FP32 Krylov
+        # vectors refined to the same FP64 residual) near 1e-6.
+        bound = 1e-5 if method == L.SOLVE_MIXED else 1e-6
+        assert np.linalg.norm(x - ox) <= bound * np.linalg.norm(ox)
         if method == L.SOLVE_JACOBI:
             assert abs(st["iterations"] - ost["iterations"]) <= max(5, 0.15 * ost["iterations"])
 
