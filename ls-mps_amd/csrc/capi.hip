@@ -83,7 +83,7 @@ struct lsmps_ctx {
   int maxn = 0, gx = 0, gy = 0, gz = 0;
   uint64_t pairs = 0;
   DBuf<int> cx, cy, cz, box, cell, slot, ccount, cstart, perm, iperm, scell, nbr, cnt, cnt_orig, row_ptr, col, hist;
-  DBuf<double> sx, sy, sz;
+  DBuf<double4> spos;
   DBuf<uint8_t> type_s;
   DBuf<int64_t> scan_scr;
   DBuf<unsigned long long> counters;  // 16 general-purpose counters
@@ -91,7 +91,7 @@ struct lsmps_ctx {
   // field and WLS (results in sorted order; original order on download)
   bool have_field = false, field_quadratic = false, have_wls = false;
   QuadField qf{};
-  DBuf<double> f, fs, grad_s, lap_s, grad, lap, ge, le;
+  DBuf<double> f, grad_s, lap_s, grad, lap, ge, le;
   DBuf<uint8_t> wstatus_s, wstatus, st_s, ty_s;
   DBuf<int> redo, nredo;
 
@@ -203,7 +203,8 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->st) (void)hipStreamSynchronize(ctx->st);
   ctx->graph.reset();
-  DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x, &ctx->y, &ctx->z, &ctx->sx, &ctx->sy, &ctx->sz, &ctx->f, &ctx->fs,
+  ctx->spos.release();
+  DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x, &ctx->y, &ctx->z, &ctx->f,
                        &ctx->grad_s, &ctx->lap_s, &ctx->grad, &ctx->lap, &ctx->ge, &ctx->le, &ctx->pval, &ctx->b,
                        &ctx->xe, &ctx->diag, &ctx->scale, &ctx->yv, &ctx->xsol, &ctx->x0, &ctx->vec64, &ctx->partials,
                        &ctx->sell_v64, &ctx->sell_vs64};
@@ -323,9 +324,7 @@ CnlBufs cnl_bufs(lsmps_ctx* ctx) {
   B.perm = ctx->perm.p;
   B.iperm = ctx->iperm.p;
   B.scell = ctx->scell.p;
-  B.sx = ctx->sx.p;
-  B.sy = ctx->sy.p;
-  B.sz = ctx->sz.p;
+  B.spos = ctx->spos.p;
   B.nbr = ctx->nbr.p;
   B.cnt = ctx->cnt.p;
   B.cnt_orig = ctx->cnt_orig.p;
@@ -345,9 +344,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, int32_t maxn, bool classify, doub
   DBuf<int>* per_particle[] = {&ctx->cx,   &ctx->cy,    &ctx->cz,    &ctx->cell, &ctx->slot,
                                &ctx->perm, &ctx->iperm, &ctx->scell, &ctx->cnt,  &ctx->cnt_orig};
   for (auto* b : per_particle) HIPCHK(b->ensure(n));
-  HIPCHK(ctx->sx.ensure(n));
-  HIPCHK(ctx->sy.ensure(n));
-  HIPCHK(ctx->sz.ensure(n));
+  HIPCHK(ctx->spos.ensure(n));
   HIPCHK(ctx->type_s.ensure(n));
   HIPCHK(ctx->nbr.ensure((size_t)n * (size_t)std::max(1, maxn)));
   HIPCHK(ctx->hist.ensure((size_t)maxn + 1));
@@ -489,25 +486,24 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   HIPCHK(ctx->grad_s.ensure(3 * (size_t)n));
   HIPCHK(ctx->lap_s.ensure(n));
   HIPCHK(ctx->wstatus_s.ensure(n));
-  HIPCHK(ctx->fs.ensure(n));
   if (p->precision == LSMPS_WLS_MIXED) HIPCHK(ctx->redo.ensure(n));
   HIPCHK(hipEventRecord(ctx->e0, st));
-  // Field at the sorted slots: evaluated there for a quadratic, gathered for values.
+  // Field into the sorted records: evaluated there for a quadratic,
+  // gathered for values.
   if (ctx->field_quadratic)
-    launch_field_quadratic(ctx->sx.p, ctx->sy.p, ctx->sz.p, n, ctx->qf, ctx->fs.p, st);
+    launch_field_quadratic(ctx->spos.p, n, ctx->qf, st);
   else
-    launch_gather_double(ctx->perm.p, ctx->f.p, n, ctx->fs.p, st);
+    launch_gather_field(ctx->perm.p, ctx->f.p, n, ctx->spos.p, st);
   zero_counters(ctx, 16);
   const int* nb = ctx->nbr.p;
   const int* nc = ctx->cnt.p;
   if (p->precision == LSMPS_WLS_FP64) {
-    launch_wls_exact(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->fs.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0,
-                     p->reg, ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, ctx->counters.p, st);
+    launch_wls_exact(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
+                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->counters.p, st);
   } else {
     launch_fill_int(ctx->nredo.p, 1, 0, st);
-    launch_wls_mixed(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->fs.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0,
-                     p->reg, ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p,
-                     ctx->counters.p, st);
+    launch_wls_mixed(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
+                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, st);
   }
   if (int rc = check_launch(ctx, "wls")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
@@ -541,7 +537,7 @@ int lsmps_wls_sampled_stats(lsmps_ctx* ctx, int32_t samples, lsmps_wls_stats* ou
   HIPCHK(ctx->le.ensure(ns));
   HIPCHK(ctx->st_s.ensure(ns));
   HIPCHK(ctx->ty_s.ensure(ns));
-  launch_wls_sample_errors(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->type.p, ctx->grad_s.p, ctx->lap_s.p,
+  launch_wls_sample_errors(ctx->spos.p, ctx->type.p, ctx->grad_s.p, ctx->lap_s.p,
                            ctx->wstatus_s.p, ctx->iperm.p, stride, ns, ctx->qf, ctx->ge.p, ctx->le.p, ctx->st_s.p,
                            ctx->ty_s.p, ctx->st);
   if (int rc = check_launch(ctx, "wls_sample_errors")) return rc;
@@ -618,10 +614,10 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)m + 1)));
   launch_unknown_map(ctx->flag.p, ctx->pos.p, n, cap, ctx->iperm.p, ctx->smap.p, ctx->rev.p, st);
   zero_counters(ctx, 16);
-  launch_ppe_row_count(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, ctx->smap.p, ctx->rev.p,
+  launch_ppe_row_count(ctx->spos.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, ctx->smap.p, ctx->rev.p,
                        ctx->iperm.p, m, p->re, ctx->rcnt.p, st);
   launch_exclusive_scan(ctx->rcnt.p, m, ctx->prow.p, ctx->scan_scr.p, st);
-  launch_ppe_row_fill(ctx->sx.p, ctx->sy.p, ctx->sz.p, ctx->type.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, ctx->smap.p,
+  launch_ppe_row_fill(ctx->spos.p, ctx->type.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, ctx->smap.p,
                       ctx->rev.p, ctx->iperm.p, m, p->re, p->shift, p->xmode, p->xseed, ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->xe.p, ctx->diag.p,
                       ctx->counters.p, st);
   launch_ppe_rhs(ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->xe.p, m, ctx->b.p, st);

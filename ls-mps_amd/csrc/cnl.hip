@@ -112,18 +112,16 @@ __global__ __launch_bounds__(256) void k_cell_sort(const int* __restrict__ start
   }
 }
 
-// Sorted-slot copies of positions and cell ids, and the inverse permutation.
+// Sorted-slot copies of positions ({x, y, z, field} records, the field
+// filled in later) and cell ids, and the inverse permutation.
 __global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ perm, const int* __restrict__ cell,
                                                        const double* __restrict__ x, const double* __restrict__ y,
-                                                       const double* __restrict__ z, int n, double* __restrict__ sx,
-                                                       double* __restrict__ sy, double* __restrict__ sz,
+                                                       const double* __restrict__ z, int n, double4* __restrict__ spos,
                                                        int* __restrict__ scell, int* __restrict__ iperm) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
-  sx[s] = x[i];
-  sy[s] = y[i];
-  sz[s] = z[i];
+  spos[s] = make_double4(x[i], y[i], z[i], 0.0);
   scell[s] = cell[i];
   iperm[i] = s;
 }
@@ -148,34 +146,36 @@ __device__ __forceinline__ uint8_t classify_rule(uint8_t t0, int kept, double z,
   return T_INTERIOR;
 }
 
-// Block-aggregated small histograms (types, neighbor counts), flushed once.
-struct PassCounters {
-  unsigned long long* trunc_total;  // dropped pairs
-  int* hist;                        // kept-count histogram, maxn + 1 bins
-  unsigned long long* tcount;       // 5 type counts (classify only)
-};
-
-// One thread per sorted slot; a wavefront walks the same few candidate runs.
-// Writes the ELL row, its length (sorted and original order) and, when
-// CLASSIFY, the new type of the particle.
+// One wavefront per 8 consecutive sorted particles (nearly always from the
+// same few cells), 8 lanes per particle. The lanes step through each
+// candidate run 8 at a time, so position loads are coalesced; a wavefront
+// ballot gives every hit its rank among the group's hits, which keeps the
+// reference's visiting order (and the cut at max_neighbors). Hits are
+// staged in LDS; afterwards the group writes the row out with coalesced
+// stores and, when CLASSIFY, takes the direction sectors of the kept
+// neighbors only (~30) rather than of every candidate (~250).
+constexpr int CNL_GROUP = 8;
 template <bool CLASSIFY>
-__global__ __launch_bounds__(256) void k_cnl_pass(const double* __restrict__ sx, const double* __restrict__ sy,
-                                                  const double* __restrict__ sz, const int* __restrict__ perm,
-                                                  const int* __restrict__ start, const int* __restrict__ scell,
-                                                  int n, int gx, int gy, int gz, double re2, int maxn,
-                                                  int* __restrict__ nbr, int* __restrict__ cnt,
-                                                  int* __restrict__ cnt_orig, uint8_t* __restrict__ type,
-                                                  uint8_t* __restrict__ type_s, double zlim, PassCounters pc) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  int found = 0, kept = 0, newt = -1;
-  if (s < n) {
-    const int i = perm[s];
-    const double xi = sx[s], yi = sy[s], zi = sz[s];
+__global__ __launch_bounds__(WAVE) void k_cnl_pass(const double4* __restrict__ spos, const int* __restrict__ perm,
+                                                   const int* __restrict__ start, const int* __restrict__ scell, int n,
+                                                   int gx, int gy, int gz, double re2, int maxn, int* __restrict__ nbr,
+                                                   int* __restrict__ cnt, int* __restrict__ cnt_orig,
+                                                   uint8_t* __restrict__ type, uint8_t* __restrict__ type_s,
+                                                   double zlim, unsigned long long* __restrict__ trunc_total) {
+  extern __shared__ int rowbuf[];  // (WAVE / CNL_GROUP) rows of maxn
+  const int lane = threadIdx.x, g = lane & (CNL_GROUP - 1), sub = lane / CNL_GROUP;
+  const int s = blockIdx.x * (WAVE / CNL_GROUP) + sub;
+  const bool live = s < n;
+  int* buf = rowbuf + sub * maxn;
+  int found = 0;
+  double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (live) {
+    pi = spos[s];
     const int c = scell[s];
     const int cxl = c % gx, t = c / gx, cyl = t % gy, czl = t / gy;
     const int x0 = cxl > 0 ? cxl - 1 : 0, x1 = cxl + 1 < gx ? cxl + 1 : gx - 1;
-    int* out = nbr + (size_t)s * maxn;
-    unsigned mask = 0;
+    const unsigned below = (1u << g) - 1u;
+    const int gshift = lane & ~(CNL_GROUP - 1);
     for (int dz = -1; dz <= 1; ++dz) {
       const int zz = czl + dz;
       if (zz < 0 || zz >= gz) continue;
@@ -184,50 +184,76 @@ __global__ __launch_bounds__(256) void k_cnl_pass(const double* __restrict__ sx,
         if (yy < 0 || yy >= gy) continue;
         const int rowc = (zz * gy + yy) * gx;
         const int b = start[rowc + x0], e = start[rowc + x1 + 1];
-        for (int s2 = b; s2 < e; ++s2) {
-          if (s2 == s) continue;
-          const double ddx = sx[s2] - xi, ddy = sy[s2] - yi, ddz = sz[s2] - zi;
-          if (ddx * ddx + ddy * ddy + ddz * ddz < re2) {
-            if (found < maxn) {
-              out[found] = s2;
-              if (CLASSIFY) mask |= sector_bits(ddx, ddy, ddz);
-            }
-            ++found;
+        for (int base = b; base < e; base += CNL_GROUP) {
+          const int s2 = base + g;
+          bool hit = false;
+          if (s2 < e && s2 != s) {
+            const double4 q = spos[s2];
+            const double ddx = q.x - pi.x, ddy = q.y - pi.y, ddz = q.z - pi.z;
+            hit = ddx * ddx + ddy * ddy + ddz * ddz < re2;
           }
+          const unsigned gm = (unsigned)(__ballot(hit) >> gshift) & 0xFFu;
+          if (hit) {
+            const int pos = found + __popc(gm & below);
+            if (pos < maxn) buf[pos] = s2;
+          }
+          found += __popc(gm);
         }
       }
     }
-    kept = found < maxn ? found : maxn;
+  }
+  __syncthreads();
+  const int kept = found < maxn ? found : maxn;
+  unsigned mask = 0;
+  if (live) {
+    int* out = nbr + (size_t)s * maxn;
+    for (int k = g; k < kept; k += CNL_GROUP) {
+      const int s2 = buf[k];
+      out[k] = s2;
+      if (CLASSIFY) {
+        const double4 q = spos[s2];
+        mask |= sector_bits(q.x - pi.x, q.y - pi.y, q.z - pi.z);
+      }
+    }
+  }
+  if (CLASSIFY) mask = group8_or(mask);
+  const bool leader = live && g == 0;
+  if (leader) {
+    const int i = perm[s];
     cnt[s] = kept;
     cnt_orig[i] = kept;
     if (CLASSIFY) {
-      const uint8_t nt = classify_rule(type[i], kept, zi, zlim, mask);
+      const uint8_t nt = classify_rule(type[i], kept, pi.z, zlim, mask);
       type[i] = nt;
       type_s[s] = nt;
-      newt = nt;
     }
   }
-  // Block-local histograms in LDS, flushed once per block.
-  extern __shared__ int lhist[];  // maxn + 1 bins, then 5 type bins
-  for (int k = threadIdx.x; k <= maxn + 5; k += blockDim.x) lhist[k] = 0;
-  __syncthreads();
-  if (s < n) {
-    atomicAdd(&lhist[kept], 1);
-    if (CLASSIFY) atomicAdd(&lhist[maxn + 1 + newt], 1);
-  }
-  unsigned long long tr = (unsigned long long)(found - kept);
+  unsigned long long tr = leader ? (unsigned long long)(found - kept) : 0ull;
   tr = wave_sum(tr);
-  if ((threadIdx.x & (WAVE - 1)) == 0 && tr) atomicAdd(pc.trunc_total, tr);
+  if (lane == 0 && tr) atomicAdd(trunc_total, tr);
+}
+
+// Histogram of the kept counts and (optionally) the type counts, from the
+// sorted-slot arrays; LDS bins per block, one flush per block.
+__global__ __launch_bounds__(256) void k_cnl_stats(const int* __restrict__ cnt, const uint8_t* __restrict__ type_s,
+                                                   int n, int maxn, int* __restrict__ hist,
+                                                   unsigned long long* __restrict__ tcount) {
+  extern __shared__ int bins[];  // maxn + 1 count bins, then 5 type bins
+  for (int k = threadIdx.x; k <= maxn + 5; k += blockDim.x) bins[k] = 0;
+  __syncthreads();
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    atomicAdd(&bins[cnt[s]], 1);
+    if (tcount) atomicAdd(&bins[maxn + 1 + type_s[s]], 1);
+  }
   __syncthreads();
   for (int k = threadIdx.x; k <= maxn; k += blockDim.x)
-    if (lhist[k]) atomicAdd(&pc.hist[k], lhist[k]);
-  if (CLASSIFY && threadIdx.x < 5 && lhist[maxn + 1 + threadIdx.x])
-    atomicAdd(&pc.tcount[threadIdx.x], (unsigned long long)lhist[maxn + 1 + threadIdx.x]);
+    if (bins[k]) atomicAdd(&hist[k], bins[k]);
+  if (tcount && threadIdx.x < 5 && bins[maxn + 1 + threadIdx.x])
+    atomicAdd(&tcount[threadIdx.x], (unsigned long long)bins[maxn + 1 + threadIdx.x]);
 }
 
 // Stand-alone classification over the ELL table (src/lsmps.cpp:27,73).
-__global__ __launch_bounds__(256) void k_classify(const double* __restrict__ sx, const double* __restrict__ sy,
-                                                  const double* __restrict__ sz, const int* __restrict__ perm,
+__global__ __launch_bounds__(256) void k_classify(const double4* __restrict__ spos, const int* __restrict__ perm,
                                                   const int* __restrict__ nbr, const int* __restrict__ cnt, int n,
                                                   int maxn, double zlim, uint8_t* __restrict__ type,
                                                   uint8_t* __restrict__ type_s, unsigned long long* __restrict__ tcount) {
@@ -238,12 +264,12 @@ __global__ __launch_bounds__(256) void k_classify(const double* __restrict__ sx,
     const int k = cnt[s];
     const int* row = nbr + (size_t)s * maxn;
     unsigned mask = 0;
-    const double xi = sx[s], yi = sy[s], zi = sz[s];
+    const double4 pi = spos[s];
     for (int q = 0; q < k; ++q) {
-      const int s2 = row[q];
-      mask |= sector_bits(sx[s2] - xi, sy[s2] - yi, sz[s2] - zi);
+      const double4 o = spos[row[q]];
+      mask |= sector_bits(o.x - pi.x, o.y - pi.y, o.z - pi.z);
     }
-    out = classify_rule(type[i], k, zi, zlim, mask);
+    out = classify_rule(type[i], k, pi.z, zlim, mask);
     type[i] = (uint8_t)out;
     type_s[s] = (uint8_t)out;
   }
@@ -302,7 +328,7 @@ void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t 
     hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart, n, B.perm);
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks(ncells, 256)), dim3(256), 0, st, B.cstart, (int)ncells, B.perm);
     hipLaunchKernelGGL(k_gather_sorted, dim3(blocks(n, 256)), dim3(256), 0, st, B.perm, B.cell, B.x, B.y, B.z, n,
-                       B.sx, B.sy, B.sz, B.scell, B.iperm);
+                       B.spos, B.scell, B.iperm);
   }
 }
 
@@ -310,23 +336,24 @@ void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classif
                      uint8_t* type_s, unsigned long long* trunc_total, int* hist, unsigned long long* tcount,
                      hipStream_t st) {
   if (n <= 0) return;
-  PassCounters pc{trunc_total, hist, tcount};
-  const size_t lds = (size_t)(maxn + 6) * sizeof(int);
+  const size_t lds = (size_t)(WAVE / CNL_GROUP) * (size_t)(maxn > 0 ? maxn : 1) * sizeof(int);
+  const unsigned grid = blocks(n, WAVE / CNL_GROUP);
   if (classify)
-    hipLaunchKernelGGL(k_cnl_pass<true>, dim3(blocks(n, 256)), dim3(256), lds, st, B.sx, B.sy, B.sz, B.perm,
-                       B.cstart, B.scell, n, B.gx, B.gy, B.gz, re2, maxn, B.nbr, B.cnt, B.cnt_orig, type, type_s,
-                       zlim, pc);
+    hipLaunchKernelGGL(k_cnl_pass<true>, dim3(grid), dim3(WAVE), lds, st, B.spos, B.perm, B.cstart, B.scell, n, B.gx,
+                       B.gy, B.gz, re2, maxn, B.nbr, B.cnt, B.cnt_orig, type, type_s, zlim, trunc_total);
   else
-    hipLaunchKernelGGL(k_cnl_pass<false>, dim3(blocks(n, 256)), dim3(256), lds, st, B.sx, B.sy, B.sz, B.perm,
-                       B.cstart, B.scell, n, B.gx, B.gy, B.gz, re2, maxn, B.nbr, B.cnt, B.cnt_orig, type, type_s,
-                       zlim, pc);
+    hipLaunchKernelGGL(k_cnl_pass<false>, dim3(grid), dim3(WAVE), lds, st, B.spos, B.perm, B.cstart, B.scell, n, B.gx,
+                       B.gy, B.gz, re2, maxn, B.nbr, B.cnt, B.cnt_orig, type, type_s, zlim, trunc_total);
+  const unsigned gs = blocks(n, 256) < 1024u ? blocks(n, 256) : 1024u;
+  hipLaunchKernelGGL(k_cnl_stats, dim3(gs), dim3(256), (size_t)(maxn + 6) * sizeof(int), st, B.cnt, type_s, n, maxn,
+                     hist, classify ? tcount : nullptr);
 }
 
 void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* type, uint8_t* type_s,
                      unsigned long long* tcount, hipStream_t st) {
   if (n > 0)
-    hipLaunchKernelGGL(k_classify, dim3(blocks(n, 256)), dim3(256), 0, st, B.sx, B.sy, B.sz, B.perm, B.nbr, B.cnt, n,
-                       maxn, zlim, type, type_s, tcount);
+    hipLaunchKernelGGL(k_classify, dim3(blocks(n, 256)), dim3(256), 0, st, B.spos, B.perm, B.nbr, B.cnt, n, maxn, zlim,
+                       type, type_s, tcount);
 }
 
 void launch_ell_to_csr(const CnlBufs& B, int n, int maxn, const int* row_ptr, int* col, hipStream_t st) {

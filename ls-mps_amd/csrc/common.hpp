@@ -42,6 +42,30 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
+// Reductions over aligned 8-lane groups with DPP lane permutes (no LDS
+// traffic): swap neighbours, swap pairs, then mirror inside the half-row.
+// After the first two steps all four lanes of a quad hold the quad's value,
+// so the half-row mirror pairs each quad with the other quad of its group.
+constexpr int DPP_QUAD_SWAP1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_SWAP2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int DPP_HALF_MIRROR = 0x141;
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float group8_sum(float v) {
+  v += __int_as_float(dpp_i<DPP_QUAD_SWAP1>(__float_as_int(v)));
+  v += __int_as_float(dpp_i<DPP_QUAD_SWAP2>(__float_as_int(v)));
+  v += __int_as_float(dpp_i<DPP_HALF_MIRROR>(__float_as_int(v)));
+  return v;
+}
+__device__ __forceinline__ unsigned group8_or(unsigned v) {
+  v |= (unsigned)dpp_i<DPP_QUAD_SWAP1>((int)v);
+  v |= (unsigned)dpp_i<DPP_QUAD_SWAP2>((int)v);
+  v |= (unsigned)dpp_i<DPP_HALF_MIRROR>((int)v);
+  return v;
+}
+
 // Inclusive prefix sum over the wavefront.
 __device__ __forceinline__ int wave_incl_scan(int v) {
   const int lane = threadIdx.x & (WAVE - 1);

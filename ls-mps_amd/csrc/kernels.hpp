@@ -33,7 +33,7 @@ struct CnlBufs {
   int *ccount = nullptr, *cstart = nullptr;               // per cell
   int64_t* scan_scratch = nullptr;
   int *perm = nullptr, *iperm = nullptr, *scell = nullptr;  // sorted slot <-> particle
-  double *sx = nullptr, *sy = nullptr, *sz = nullptr;       // sorted positions
+  double4* spos = nullptr;                                  // sorted {x, y, z, field}
   int *nbr = nullptr, *cnt = nullptr, *cnt_orig = nullptr;  // ELL table
 };
 void launch_cell_coords(const double* x, const double* y, const double* z, int n, double re, int* cx, int* cy,
@@ -48,35 +48,32 @@ void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* ty
 void launch_ell_to_csr(const CnlBufs& B, int n, int maxn, const int* row_ptr, int* col, hipStream_t st);
 void launch_gather_types(const int* perm, const uint8_t* type, int n, uint8_t* type_s, hipStream_t st);
 
-// wls.hip (positions, field, types, neighbor rows and outputs in sorted order)
-void launch_field_quadratic(const double* x, const double* y, const double* z, int n, const QuadField& q, double* f,
-                            hipStream_t st);
-void launch_wls_exact(const double* x, const double* y, const double* z, const double* f, const uint8_t* type,
-                      const int* nbr, const int* cnt, int maxn, int n, double re, double l0, double reg, double* grad,
-                      double* lap, uint8_t* status, unsigned long long* counts, hipStream_t st);
-void launch_wls_mixed(const double* x, const double* y, const double* z, const double* f, const uint8_t* type,
-                      const int* nbr, const int* cnt, int maxn, int n, double re, double l0, double reg, double* grad,
-                      double* lap, uint8_t* status, int* redo, int* nredo, unsigned long long* counts,
-                      hipStream_t st);
-void launch_wls_sample_errors(const double* x, const double* y, const double* z, const uint8_t* type,
-                              const double* grad, const double* lap, const uint8_t* status, const int* iperm,
-                              int64_t stride, int ns, const QuadField& q, double* ge, double* le, uint8_t* st_out,
-                              uint8_t* ty_out, hipStream_t st);
+// wls.hip (sorted {x, y, z, field} records, types, neighbor rows and
+// outputs in sorted order)
+void launch_field_quadratic(double4* spos, int n, const QuadField& q, hipStream_t st);
+void launch_gather_field(const int* perm, const double* v, int n, double4* spos, hipStream_t st);
+void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
+                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status,
+                      unsigned long long* counts, hipStream_t st);
+void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
+                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
+                      int* nredo, unsigned long long* counts, hipStream_t st);
+void launch_wls_sample_errors(const double4* spos, const uint8_t* type, const double* grad, const double* lap,
+                              const uint8_t* status, const int* iperm, int64_t stride, int ns, const QuadField& q,
+                              double* ge, double* le, uint8_t* st_out, uint8_t* ty_out, hipStream_t st);
 void launch_wls_unsort(const int* perm, const double* gs, const double* ls, const uint8_t* ss, int n, double* g,
                        double* l, uint8_t* st, hipStream_t stream);
-void launch_gather_double(const int* perm, const double* v, int n, double* out, hipStream_t st);
 
 // ppe.hip (rows in unknown order; positions and neighbor rows in sorted order)
 void launch_unknown_flags(const uint8_t* type, int n, int* flag, hipStream_t st);
 void launch_unknown_map(const int* flag, const int* pos, int n, int cap, const int* iperm, int* smap, int* rev,
                         hipStream_t st);
-void launch_ppe_row_count(const double* x, const double* y, const double* z, const int* nbr, const int* ncnt, int maxn,
-                          const int* smap, const int* rev, const int* iperm, int m, double re, int* cnt,
-                          hipStream_t st);
-void launch_ppe_row_fill(const double* x, const double* y, const double* z, const uint8_t* type, const int* nbr,
-                         const int* ncnt, int maxn, const int* smap, const int* rev, const int* iperm, int m, double re,
-                         double shift, int xmode, uint32_t xseed, const int* prow, int* pcol, double* pval, double* xe,
-                         double* diag, unsigned long long* counters, hipStream_t st);
+void launch_ppe_row_count(const double4* spos, const int* nbr, const int* ncnt, int maxn, const int* smap,
+                          const int* rev, const int* iperm, int m, double re, int* cnt, hipStream_t st);
+void launch_ppe_row_fill(const double4* spos, const uint8_t* type, const int* nbr, const int* ncnt, int maxn,
+                         const int* smap, const int* rev, const int* iperm, int m, double re, double shift, int xmode,
+                         uint32_t xseed, const int* prow, int* pcol, double* pval, double* xe, double* diag,
+                         unsigned long long* counters, hipStream_t st);
 void launch_ppe_rhs(const int* prow, const int* pcol, const double* pval, const double* xe, int m, double* b,
                     hipStream_t st);
 

@@ -47,8 +47,7 @@ __device__ __forceinline__ bool pair_coef(double dx, double dy, double dz, doubl
 }
 
 // Positions (x, y, z), neighbor rows and smap are in sorted-slot order.
-__global__ __launch_bounds__(256) void k_ppe_row_count(const double* __restrict__ x, const double* __restrict__ y,
-                                                       const double* __restrict__ z, const int* __restrict__ nbr,
+__global__ __launch_bounds__(256) void k_ppe_row_count(const double4* __restrict__ spos, const int* __restrict__ nbr,
                                                        const int* __restrict__ ncnt, int maxn,
                                                        const int* __restrict__ smap, const int* __restrict__ rev,
                                                        const int* __restrict__ iperm, int m, double re,
@@ -60,15 +59,16 @@ __global__ __launch_bounds__(256) void k_ppe_row_count(const double* __restrict_
     return;
   }
   const int s = iperm[rev[rr]];
-  const double xi = x[s], yi = y[s], zi = z[s];
+  const double4 pi = spos[s];
   const int* row = nbr + (size_t)s * maxn;
   const int k = ncnt[s];
   int c = 1;
   for (int q = 0; q < k; ++q) {
     const int s2 = row[q];
     if (smap[s2] < 0) continue;
+    const double4 pj = spos[s2];
     double a;
-    if (pair_coef(x[s2] - xi, y[s2] - yi, z[s2] - zi, re, &a)) ++c;
+    if (pair_coef(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, re, &a)) ++c;
   }
   cnt[rr] = c;
 }
@@ -85,8 +85,7 @@ __device__ __forceinline__ double xstar_random(uint32_t seed, int k) {
 }
 
 // counters: [0] failed rows, [1] NearWall rows except row 0.
-__global__ __launch_bounds__(256) void k_ppe_row_fill(const double* __restrict__ x, const double* __restrict__ y,
-                                                      const double* __restrict__ z, const uint8_t* __restrict__ type,
+__global__ __launch_bounds__(256) void k_ppe_row_fill(const double4* __restrict__ spos, const uint8_t* __restrict__ type,
                                                       const int* __restrict__ nbr, const int* __restrict__ ncnt,
                                                       int maxn, const int* __restrict__ smap,
                                                       const int* __restrict__ rev, const int* __restrict__ iperm, int m,
@@ -100,8 +99,8 @@ __global__ __launch_bounds__(256) void k_ppe_row_fill(const double* __restrict__
   if (rr < m) {
     const int i = rev[rr];
     const int s = iperm[i];
-    const double xi = x[s], yi = y[s], zi = z[s];
-    xe[rr] = xmode == 0 ? xi * xi + yi * yi  // x* = x^2 + y^2 (src/lsmps.cpp:29)
+    const double4 pi = spos[s];
+    xe[rr] = xmode == 0 ? pi.x * pi.x + pi.y * pi.y  // x* = x^2 + y^2 (src/lsmps.cpp:29)
                         : xstar_random(xseed, rr);
     int at = prow[rr];
     if (rr == 0) {
@@ -116,8 +115,9 @@ __global__ __launch_bounds__(256) void k_ppe_row_fill(const double* __restrict__
         const int s2 = row[q];
         const int cj = smap[s2];
         if (cj < 0) continue;
+        const double4 pj = spos[s2];
         double a;
-        if (!pair_coef(x[s2] - xi, y[s2] - yi, z[s2] - zi, re, &a)) continue;
+        if (!pair_coef(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, re, &a)) continue;
         d += a;
         pcol[at] = cj;
         pval[at] = -a;
@@ -162,20 +162,19 @@ void launch_unknown_map(const int* flag, const int* pos, int n, int cap, const i
   if (n > 0)
     hipLaunchKernelGGL(k_unknown_map, dim3(blocks(n, 256)), dim3(256), 0, st, flag, pos, n, cap, iperm, smap, rev);
 }
-void launch_ppe_row_count(const double* x, const double* y, const double* z, const int* nbr, const int* ncnt, int maxn,
-                          const int* smap, const int* rev, const int* iperm, int m, double re, int* cnt,
-                          hipStream_t st) {
+void launch_ppe_row_count(const double4* spos, const int* nbr, const int* ncnt, int maxn, const int* smap,
+                          const int* rev, const int* iperm, int m, double re, int* cnt, hipStream_t st) {
   if (m > 0)
-    hipLaunchKernelGGL(k_ppe_row_count, dim3(blocks(m, 256)), dim3(256), 0, st, x, y, z, nbr, ncnt, maxn, smap, rev,
-                       iperm, m, re, cnt);
+    hipLaunchKernelGGL(k_ppe_row_count, dim3(blocks(m, 256)), dim3(256), 0, st, spos, nbr, ncnt, maxn, smap, rev, iperm,
+                       m, re, cnt);
 }
-void launch_ppe_row_fill(const double* x, const double* y, const double* z, const uint8_t* type, const int* nbr,
-                         const int* ncnt, int maxn, const int* smap, const int* rev, const int* iperm, int m, double re,
-                         double shift, int xmode, uint32_t xseed, const int* prow, int* pcol, double* pval, double* xe,
-                         double* diag, unsigned long long* counters, hipStream_t st) {
+void launch_ppe_row_fill(const double4* spos, const uint8_t* type, const int* nbr, const int* ncnt, int maxn,
+                         const int* smap, const int* rev, const int* iperm, int m, double re, double shift, int xmode,
+                         uint32_t xseed, const int* prow, int* pcol, double* pval, double* xe, double* diag,
+                         unsigned long long* counters, hipStream_t st) {
   if (m > 0)
-    hipLaunchKernelGGL(k_ppe_row_fill, dim3(blocks(m, 256)), dim3(256), 0, st, x, y, z, type, nbr, ncnt, maxn, smap,
-                       rev, iperm, m, re, shift, xmode, xseed, prow, pcol, pval, xe, diag, counters);
+    hipLaunchKernelGGL(k_ppe_row_fill, dim3(blocks(m, 256)), dim3(256), 0, st, spos, type, nbr, ncnt, maxn, smap, rev,
+                       iperm, m, re, shift, xmode, xseed, prow, pcol, pval, xe, diag, counters);
 }
 void launch_ppe_rhs(const int* prow, const int* pcol, const double* pval, const double* xe, int m, double* b,
                     hipStream_t st) {

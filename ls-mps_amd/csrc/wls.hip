@@ -72,11 +72,11 @@ __device__ __forceinline__ void chol9_solve(const T (&L)[45], const T (&b)[9], T
 
 // FP64 solve of particle i with the reference's operation order. Returns
 // the status; writes grad/lap.
-__device__ __noinline__ uint8_t wls_exact_one(int i, const double* __restrict__ x, const double* __restrict__ y,
-                                              const double* __restrict__ z, const double* __restrict__ f,
-                                              const int* __restrict__ nbr, const int* __restrict__ cnt, int maxn,
-                                              double re, double l0, double reg, double* g3, double* lap) {
-  const double xi = x[i], yi = y[i], zi = z[i], fi = f[i];
+__device__ __noinline__ uint8_t wls_exact_one(int i, const double4* __restrict__ spos, const int* __restrict__ nbr,
+                                              const int* __restrict__ cnt, int maxn, double re, double l0, double reg,
+                                              double* g3, double* lap) {
+  const double4 pi = spos[i];
+  const double xi = pi.x, yi = pi.y, zi = pi.z, fi = pi.w;
   const int* col = nbr + (size_t)i * maxn;
   const int rb = 0, rend = cnt[i];
   const double h2 = 2 * l0;
@@ -87,8 +87,8 @@ __device__ __noinline__ uint8_t wls_exact_one(int i, const double* __restrict__ 
 #pragma unroll
     for (int a = 0; a < 9; ++a) B[a] = 0.0;
     for (int k = rb; k < rend; ++k) {
-      const int j = col[k];
-      const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+      const double4 pj = spos[col[k]];
+      const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
       const double r = sqrt(dx * dx + dy * dy + dz * dz);
       if (r <= 1e-12 || r >= re) continue;
       const double tq = re / r - 1.0;
@@ -105,7 +105,7 @@ __device__ __noinline__ uint8_t wls_exact_one(int i, const double* __restrict__ 
       q[6] = dx * dy / g;
       q[7] = dx * dz / g;
       q[8] = dy * dz / g;
-      const double df = f[j] - fi;
+      const double df = pj.w - fi;
       ws += w;
 #pragma unroll
       for (int a = 0; a < 9; ++a) {
@@ -160,8 +160,7 @@ __device__ __forceinline__ void store_result(int i, uint8_t st, const double* g3
   status[i] = st;
 }
 
-__global__ __launch_bounds__(128) void k_wls_exact(const double* __restrict__ x, const double* __restrict__ y,
-                                                   const double* __restrict__ z, const double* __restrict__ f,
+__global__ __launch_bounds__(128) void k_wls_exact(const double4* __restrict__ spos,
                                                    const uint8_t* __restrict__ type, const int* __restrict__ nbr,
                                                    const int* __restrict__ cnt, int maxn, int n, double re, double l0,
                                                    double reg, double* __restrict__ grad, double* __restrict__ lap,
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(128) void k_wls_exact(const double* __restrict__ x,
     const uint8_t t = type[i];
     double g3[3] = {0.0, 0.0, 0.0}, L = 0.0;
     uint8_t st = WLS_SKIPPED;
-    if (t != T_WALL && t != T_SPLASH) st = wls_exact_one(i, x, y, z, f, nbr, cnt, maxn, re, l0, reg, g3, &L);
+    if (t != T_WALL && t != T_SPLASH) st = wls_exact_one(i, spos, nbr, cnt, maxn, re, l0, reg, g3, &L);
     store_result(i, st, g3, L, grad, lap, status);
     slot = st;
   }
@@ -181,8 +180,7 @@ __global__ __launch_bounds__(128) void k_wls_exact(const double* __restrict__ x,
 }
 
 // Redo in FP64 the particles the FP32 kernel could not trust.
-__global__ __launch_bounds__(128) void k_wls_exact_list(const double* __restrict__ x, const double* __restrict__ y,
-                                                        const double* __restrict__ z, const double* __restrict__ f,
+__global__ __launch_bounds__(128) void k_wls_exact_list(const double4* __restrict__ spos,
                                                         const int* __restrict__ nbr, const int* __restrict__ cnt,
                                                         int maxn, const int* __restrict__ list,
                                                         const int* __restrict__ nlist,
@@ -193,129 +191,201 @@ __global__ __launch_bounds__(128) void k_wls_exact_list(const double* __restrict
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x) {
     const int i = list[k];
     double g3[3] = {0.0, 0.0, 0.0}, L = 0.0;
-    const uint8_t st = wls_exact_one(i, x, y, z, f, nbr, cnt, maxn, re, l0, reg, g3, &L);
+    const uint8_t st = wls_exact_one(i, spos, nbr, cnt, maxn, re, l0, reg, g3, &L);
     store_result(i, st, g3, L, grad, lap, status);
     atomicAdd(&counts[st], 1ull);
     atomicAdd(&counts[WLS_REDO_SLOT], 1ull);
   }
 }
 
-// FP32 moments on offsets scaled by 1/re (so re -> 1, l0 -> l0/re);
-// the solution vector is unchanged by the scaling because the monomials are
+// FP32 moments on offsets scaled by 1/re (so re -> 1, l0 -> l0/re); the
+// solution vector is unchanged by the scaling because the monomials are
 // dimensionless and df/r = (df/re)/(r/re).
-// Returns the status slot to count (-1 when the row went to the FP64 list).
-__device__ __forceinline__ int wls_mixed_one(int i, const double* __restrict__ x, const double* __restrict__ y,
-                                             const double* __restrict__ z, const double* __restrict__ f,
-                                             const uint8_t* __restrict__ type, const int* __restrict__ nbr,
-                                             const int* __restrict__ cnt, int maxn, double re, double l0,
-                                             double* __restrict__ grad, double* __restrict__ lap,
-                                             uint8_t* __restrict__ status, int* __restrict__ redo,
-                                             int* __restrict__ nredo) {
-  const uint8_t t = type[i];
-  if (t == T_WALL || t == T_SPLASH) {
-    const double g0[3] = {0.0, 0.0, 0.0};
-    store_result(i, WLS_SKIPPED, g0, 0.0, grad, lap, status);
-    return WLS_SKIPPED;
+//
+// One wavefront per 64 consecutive sorted particles, in two halves of 32:
+//  phase 1: 8 lanes per particle (8 particles at a time) stride through the
+//           particle's neighbor row, so row reads are coalesced and the 8
+//           lanes gather neighboring particles; each lane accumulates the 55
+//           sums (45 lower-triangle moments, 9 right-hand sides, weight
+//           sum), the 8 lanes combine them with xor shuffles and write them
+//           to LDS;
+//  phase 2: one lane per particle factorises its 9x9 from LDS (FP32
+//           Cholesky with reciprocal square roots) and solves. Rows whose
+//           pivots lose more than three digits go to the FP64 work list.
+constexpr int WLS_GROUP = 8;   // lanes per particle in phase 1
+constexpr int WLS_NACC = 55;   // 45 moments + 9 RHS + weight sum
+constexpr int WLS_STRIDE = 57; // LDS floats per particle (odd: no bank conflicts)
+constexpr int WLS_HALF = 32;   // particles per phase-2 batch
+
+__device__ __forceinline__ void wls_accumulate(const double4 pj, const double4 pi, double inv_re, float inv_h2,
+                                               float inv_g, float rmin2, float (&a)[WLS_NACC]) {
+  const float dx = (float)((pj.x - pi.x) * inv_re), dy = (float)((pj.y - pi.y) * inv_re),
+              dz = (float)((pj.z - pi.z) * inv_re);
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  if (r2 <= rmin2 || r2 >= 1.0f) return;
+  const float inv = __builtin_amdgcn_rsqf(r2);
+  const float tq = inv - 1.0f;
+  const float w = tq * tq;
+  float q[9];
+  q[0] = dx * inv;
+  q[1] = dy * inv;
+  q[2] = dz * inv;
+  const float ih = inv_h2 * inv, ig = inv_g * inv;
+  q[3] = dx * dx * ih;
+  q[4] = dy * dy * ih;
+  q[5] = dz * dz * ih;
+  q[6] = dx * dy * ig;
+  q[7] = dx * dz * ig;
+  q[8] = dy * dz * ig;
+  const float dfr = (float)((pj.w - pi.w) * inv_re) * inv;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const float wq = w * q[i];
+    a[45 + i] = fmaf(wq, dfr, a[45 + i]);
+#pragma unroll
+    for (int c = 0; c <= i; ++c) a[tri(i, c)] = fmaf(wq, q[c], a[tri(i, c)]);
   }
+  a[54] += w;
+}
+
+// FP32 Cholesky + solve from the 55 sums; false when a pivot is not
+// > 1e-3 of its diagonal (or not finite).
+__device__ __forceinline__ bool wls_solve_f32(const float* __restrict__ a, float (&sol)[9]) {
+  const float ws = a[54];
+  if (!(ws > 0.0f)) return false;
+  const float iw = 1.0f / ws;
+  float m[45], bb[9], il[9];
+#pragma unroll
+  for (int k = 0; k < 45; ++k) m[k] = a[k] * iw;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) bb[i] = a[45 + i] * iw;
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const float d0 = m[tri(i, i)];
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      float sm = m[tri(i, j)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) sm = fmaf(-m[tri(i, k)], m[tri(j, k)], sm);
+      if (i == j) {
+        if (!(sm > 1e-3f * d0) || !isfinite(sm)) ok = false;
+        const float r = __builtin_amdgcn_rsqf(sm);
+        il[i] = r;
+        m[tri(i, i)] = sm * r;
+      } else {
+        m[tri(i, j)] = sm * il[j];
+      }
+    }
+  }
+  if (!ok) return false;
+  float yv[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    float sm = bb[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) sm = fmaf(-m[tri(i, k)], yv[k], sm);
+    yv[i] = sm * il[i];
+  }
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+    float sm = yv[i];
+#pragma unroll
+    for (int k = i + 1; k < 9; ++k) sm = fmaf(-m[tri(k, i)], sol[k], sm);
+    sol[i] = sm * il[i];
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ spos,
+                                                    const uint8_t* __restrict__ type, const int* __restrict__ nbr,
+                                                    const int* __restrict__ cnt, int maxn, int n, double re, double l0,
+                                                    double* __restrict__ grad, double* __restrict__ lap,
+                                                    uint8_t* __restrict__ status, int* __restrict__ redo,
+                                                    int* __restrict__ nredo, unsigned long long* __restrict__ counts) {
+  __shared__ float sums[WLS_HALF * WLS_STRIDE];
+  const int lane = threadIdx.x;
   const double inv_re = 1.0 / re;
   const float l0s = (float)(l0 * inv_re);
   const float inv_h2 = 1.0f / (2.0f * l0s), inv_g = 1.0f / l0s;
-  const float rmin = (float)(1e-12 * inv_re);
-  const double xi = x[i], yi = y[i], zi = z[i], fi = f[i];
-  const int* col = nbr + (size_t)i * maxn;
-  const int rb = 0, rend = cnt[i];
-  float M[45], B[9], ws = 0.0f;
+  const float rmin = (float)(1e-12 * inv_re), rmin2 = rmin * rmin;
+  const int s0 = blockIdx.x * WAVE;
+  int n_ok = 0, n_skip = 0;
+  for (int half = 0; half < WAVE / WLS_HALF; ++half) {
+    // phase 1: 8 lanes per particle, 8 particles per pass, 4 passes.
+    const int g = lane % WLS_GROUP, sub = lane / WLS_GROUP;
+    for (int pass = 0; pass < WLS_HALF / (WAVE / WLS_GROUP); ++pass) {
+      const int local = pass * (WAVE / WLS_GROUP) + sub;  // 0..31 within the half
+      const int s = s0 + half * WLS_HALF + local;
+      float a[WLS_NACC];
 #pragma unroll
-  for (int k = 0; k < 45; ++k) M[k] = 0.0f;
+      for (int k = 0; k < WLS_NACC; ++k) a[k] = 0.0f;
+      if (s < n) {
+        const uint8_t t = type[s];
+        if (t != T_WALL && t != T_SPLASH) {
+          const double4 pi = spos[s];
+          const int* row = nbr + (size_t)s * maxn;
+          const int kn = cnt[s];
+          for (int k = g; k < kn; k += WLS_GROUP) wls_accumulate(spos[row[k]], pi, inv_re, inv_h2, inv_g, rmin2, a);
+        }
+      }
 #pragma unroll
-  for (int a = 0; a < 9; ++a) B[a] = 0.0f;
-  for (int k = rb; k < rend; ++k) {
-    const int j = col[k];
-    const float dx = (float)((x[j] - xi) * inv_re), dy = (float)((y[j] - yi) * inv_re),
-                dz = (float)((z[j] - zi) * inv_re);
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    const float r = sqrtf(r2);
-    if (r <= rmin || r >= 1.0f) continue;
-    const float inv = 1.0f / r;
-    const float tq = inv - 1.0f;
-    const float w = tq * tq;
-    float q[9];
-    q[0] = dx * inv;
-    q[1] = dy * inv;
-    q[2] = dz * inv;
-    const float ih = inv_h2 * inv, ig = inv_g * inv;
-    q[3] = dx * dx * ih;
-    q[4] = dy * dy * ih;
-    q[5] = dz * dz * ih;
-    q[6] = dx * dy * ig;
-    q[7] = dx * dz * ig;
-    q[8] = dy * dz * ig;
-    const float dfr = (float)((f[j] - fi) * inv_re) * inv;
-    ws += w;
+      for (int k = 0; k < WLS_NACC; ++k) a[k] = group8_sum(a[k]);
+      // Lane g of the group stores sums g*7 .. g*7+6.
 #pragma unroll
-    for (int a = 0; a < 9; ++a) {
-      const float wq = w * q[a];
-      B[a] = fmaf(wq, dfr, B[a]);
-#pragma unroll
-      for (int c = 0; c <= a; ++c) M[tri(a, c)] = fmaf(wq, q[c], M[tri(a, c)]);
+      for (int k = 0; k < WLS_NACC; ++k)
+        if (k / 7 == g) sums[local * WLS_STRIDE + k] = a[k];
     }
+    __syncthreads();
+    // phase 2: one lane per particle of this half.
+    if (lane < WLS_HALF) {
+      const int s = s0 + half * WLS_HALF + lane;
+      if (s < n) {
+        const uint8_t t = type[s];
+        if (t == T_WALL || t == T_SPLASH) {
+          const double g0[3] = {0.0, 0.0, 0.0};
+          store_result(s, WLS_SKIPPED, g0, 0.0, grad, lap, status);
+          ++n_skip;
+        } else {
+          float sol[9];
+          if (wls_solve_f32(sums + lane * WLS_STRIDE, sol)) {
+            const double g3[3] = {(double)sol[0], (double)sol[1], (double)sol[2]};
+            store_result(s, WLS_OK, g3, (double)(sol[3] + sol[4] + sol[5]) / l0, grad, lap, status);
+            ++n_ok;
+          } else {
+            const int at = atomicAdd(nredo, 1);
+            redo[at] = s;
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
-  bool ok = ws > 0.0f;
-  float sol[9];
-  if (ok) {
-    const float iw = 1.0f / ws;
-#pragma unroll
-    for (int k = 0; k < 45; ++k) M[k] *= iw;
-#pragma unroll
-    for (int a = 0; a < 9; ++a) B[a] *= iw;
-    float d0[9];
-#pragma unroll
-    for (int a = 0; a < 9; ++a) d0[a] = M[tri(a, a)];
-    ok = chol9<float>(M, 1e-14f, 1e-3f, d0);
-    if (ok) chol9_solve<float>(M, B, sol);
+  // Per-wave status counts (rows sent to the FP64 list are counted there).
+  unsigned long long c_ok = wave_sum((unsigned long long)n_ok), c_skip = wave_sum((unsigned long long)n_skip);
+  if (lane == 0) {
+    if (c_ok) atomicAdd(&counts[WLS_OK], c_ok);
+    if (c_skip) atomicAdd(&counts[WLS_SKIPPED], c_skip);
   }
-  if (!ok) {
-    const int at = atomicAdd(nredo, 1);
-    redo[at] = i;
-    return -1;
-  }
-  const double g3[3] = {(double)sol[0], (double)sol[1], (double)sol[2]};
-  store_result(i, WLS_OK, g3, (double)(sol[3] + sol[4] + sol[5]) / l0, grad, lap, status);
-  return WLS_OK;
-}
-
-__global__ __launch_bounds__(128) void k_wls_mixed(const double* __restrict__ x, const double* __restrict__ y,
-                                                   const double* __restrict__ z, const double* __restrict__ f,
-                                                   const uint8_t* __restrict__ type, const int* __restrict__ nbr,
-                                                   const int* __restrict__ cnt, int maxn, int n, double re, double l0,
-                                                   double* __restrict__ grad, double* __restrict__ lap,
-                                                   uint8_t* __restrict__ status, int* __restrict__ redo,
-                                                   int* __restrict__ nredo, unsigned long long* __restrict__ counts) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int slot = -1;
-  if (i < n) slot = wls_mixed_one(i, x, y, z, f, type, nbr, cnt, maxn, re, l0, grad, lap, status, redo, nredo);
-  count_status(slot, counts);
 }
 
 // Quadratic field f = c0 + cx x + cy y + cz z + cxx x^2 + cyy y^2 + czz z^2
 // + cxy x y + cxz x z + cyz y z, summed left to right with (c*u)*v
 // products, which reproduces the reference's test-field expression bit for
 // bit (src/lsmps.cpp:28).
-__global__ __launch_bounds__(256) void k_field_quadratic(const double* __restrict__ x, const double* __restrict__ y,
-                                                         const double* __restrict__ z, int n, QuadField q,
-                                                         double* __restrict__ f) {
+__global__ __launch_bounds__(256) void k_field_quadratic(double4* __restrict__ spos, int n, QuadField q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double a = x[i], b = y[i], c = z[i];
-  f[i] = q.c[0] + q.c[1] * a + q.c[2] * b + q.c[3] * c + q.c[4] * a * a + q.c[5] * b * b + q.c[6] * c * c +
-         q.c[7] * a * b + q.c[8] * a * c + q.c[9] * b * c;
+  const double a = spos[i].x, b = spos[i].y, c = spos[i].z;
+  spos[i].w = q.c[0] + q.c[1] * a + q.c[2] * b + q.c[3] * c + q.c[4] * a * a + q.c[5] * b * b + q.c[6] * c * c +
+              q.c[7] * a * b + q.c[8] * a * c + q.c[9] * b * c;
 }
 
 // Errors of the sampled particles i = k*stride (original index) against the
 // closed-form gradient and Laplacian of the quadratic field
 // (src/lsmps.cpp:28); positions and results are read at the sorted slot.
-__global__ __launch_bounds__(256) void k_wls_sample_errors(const double* __restrict__ x, const double* __restrict__ y,
-                                                           const double* __restrict__ z, const uint8_t* __restrict__ type,
+__global__ __launch_bounds__(256) void k_wls_sample_errors(const double4* __restrict__ spos,
+                                                           const uint8_t* __restrict__ type,
                                                            const double* __restrict__ grad,
                                                            const double* __restrict__ lap,
                                                            const uint8_t* __restrict__ status,
@@ -327,7 +397,7 @@ __global__ __launch_bounds__(256) void k_wls_sample_errors(const double* __restr
   if (k >= ns) return;
   const int64_t io = (int64_t)k * stride;  // original index; arrays below are in sorted order
   const int64_t i = iperm[io];
-  const double a = x[i], b = y[i], c = z[i];
+  const double a = spos[i].x, b = spos[i].y, c = spos[i].z;
   const double ex = q.c[1] + 2 * q.c[4] * a + q.c[7] * b + q.c[8] * c;
   const double ey = q.c[2] + 2 * q.c[5] * b + q.c[7] * a + q.c[9] * c;
   const double ez = q.c[3] + 2 * q.c[6] * c + q.c[8] * a + q.c[9] * b;
@@ -353,10 +423,11 @@ __global__ __launch_bounds__(256) void k_wls_unsort(const int* __restrict__ perm
   st[i] = ss[s];
 }
 
-__global__ __launch_bounds__(256) void k_gather_double(const int* __restrict__ perm, const double* __restrict__ v, int n,
-                                                       double* __restrict__ out) {
+// Field values given in particle order into the sorted records.
+__global__ __launch_bounds__(256) void k_gather_field(const int* __restrict__ perm, const double* __restrict__ v, int n,
+                                                      double4* __restrict__ spos) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n) out[s] = v[perm[s]];
+  if (s < n) spos[s].w = v[perm[s]];
 }
 
 static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -366,43 +437,40 @@ void launch_wls_unsort(const int* perm, const double* gs, const double* ls, cons
   if (n > 0) hipLaunchKernelGGL(k_wls_unsort, dim3(blocks(n, 256)), dim3(256), 0, stream, perm, gs, ls, ss, n, g, l, st);
 }
 
-void launch_gather_double(const int* perm, const double* v, int n, double* out, hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_gather_double, dim3(blocks(n, 256)), dim3(256), 0, st, perm, v, n, out);
+void launch_gather_field(const int* perm, const double* v, int n, double4* spos, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_field, dim3(blocks(n, 256)), dim3(256), 0, st, perm, v, n, spos);
 }
 
-void launch_field_quadratic(const double* x, const double* y, const double* z, int n, const QuadField& q, double* f,
-                            hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_field_quadratic, dim3(blocks(n, 256)), dim3(256), 0, st, x, y, z, n, q, f);
+void launch_field_quadratic(double4* spos, int n, const QuadField& q, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_field_quadratic, dim3(blocks(n, 256)), dim3(256), 0, st, spos, n, q);
 }
 
-void launch_wls_exact(const double* x, const double* y, const double* z, const double* f, const uint8_t* type,
-                      const int* nbr, const int* cnt, int maxn, int n, double re, double l0, double reg, double* grad,
-                      double* lap, uint8_t* status, unsigned long long* counts, hipStream_t st) {
+void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
+                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status,
+                      unsigned long long* counts, hipStream_t st) {
   if (n > 0)
-    hipLaunchKernelGGL(k_wls_exact, dim3(blocks(n, 128)), dim3(128), 0, st, x, y, z, f, type, nbr, cnt, maxn, n, re,
-                       l0, reg, grad, lap, status, counts);
+    hipLaunchKernelGGL(k_wls_exact, dim3(blocks(n, 128)), dim3(128), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, reg,
+                       grad, lap, status, counts);
 }
 
-void launch_wls_mixed(const double* x, const double* y, const double* z, const double* f, const uint8_t* type,
-                      const int* nbr, const int* cnt, int maxn, int n, double re, double l0, double reg, double* grad,
-                      double* lap, uint8_t* status, int* redo, int* nredo, unsigned long long* counts,
-                      hipStream_t st) {
+void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
+                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
+                      int* nredo, unsigned long long* counts, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_wls_mixed, dim3(blocks(n, 128)), dim3(128), 0, st, x, y, z, f, type, nbr, cnt, maxn, n, re, l0,
-                     grad, lap, status, redo, nredo, counts);
+  hipLaunchKernelGGL(k_wls_mixed, dim3(blocks(n, WAVE)), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, grad,
+                     lap, status, redo, nredo, counts);
   // The redo list length is only known on the device: a bounded grid
-  // strides over it (typical lists are empty or a few hundred entries).
+  // strides over it (typical lists are empty or a few thousand entries).
   const unsigned g = blocks(n, 128) < 512u ? blocks(n, 128) : 512u;
-  hipLaunchKernelGGL(k_wls_exact_list, dim3(g), dim3(128), 0, st, x, y, z, f, nbr, cnt, maxn, redo, nredo,
-                     re, l0, reg, grad, lap, status, counts);
+  hipLaunchKernelGGL(k_wls_exact_list, dim3(g), dim3(128), 0, st, spos, nbr, cnt, maxn, redo, nredo, re, l0, reg, grad,
+                     lap, status, counts);
 }
 
-void launch_wls_sample_errors(const double* x, const double* y, const double* z, const uint8_t* type,
-                              const double* grad, const double* lap, const uint8_t* status, const int* iperm,
-                              int64_t stride, int ns, const QuadField& q, double* ge, double* le, uint8_t* st_out,
-                              uint8_t* ty_out, hipStream_t st) {
+void launch_wls_sample_errors(const double4* spos, const uint8_t* type, const double* grad, const double* lap,
+                              const uint8_t* status, const int* iperm, int64_t stride, int ns, const QuadField& q,
+                              double* ge, double* le, uint8_t* st_out, uint8_t* ty_out, hipStream_t st) {
   if (ns > 0)
-    hipLaunchKernelGGL(k_wls_sample_errors, dim3(blocks(ns, 256)), dim3(256), 0, st, x, y, z, type, grad, lap, status,
+    hipLaunchKernelGGL(k_wls_sample_errors, dim3(blocks(ns, 256)), dim3(256), 0, st, spos, type, grad, lap, status,
                        iperm, stride, ns, q, ge, le, st_out, ty_out);
 }
 
