@@ -16,9 +16,15 @@ FP64 right-Jacobi BiCGStab and with mixed-precision BiCGStab (FP32 Krylov
 vectors and matrix, FP64 reliable updates), both on the same matrix and
 right-hand side (seeded x* ~ U(-1, 1)).
 
-The CPU baseline is the repository's single-threaded CPU oracle (a port of
-the reference's algorithm) running neighbors + classification + WLS on a
-bounded sample of the same scene, on this host.
+The CPU baseline is the reference's own src/lsmps.cpp (built into
+oracle/_ref where the reference tree was present) running neighbors +
+classification + WLS over the same particles on one host core; without that
+build, the repository's CPU oracle on a tenth-size scene.
+
+`roofline` is for the step's dominant kernel (the neighbor pass or the WLS
+kernel, whichever takes longer): algorithmic bytes of one launch over its
+average duration from HIP events around it. `traffic` comes from the
+committed counter summary under profiles/ when its particle count matches.
 """
 from __future__ import annotations
 
@@ -43,6 +49,8 @@ def parse():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-max-iter", type=int, default=40000)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run 3 device copies of MIB MiB (for counter calibration)")
     return ap.parse_args()
 
 
@@ -51,34 +59,108 @@ def log(*a):
 
 
 def wls_bytes_per_particle(k_mean):
-    # positions 24, row_ptr 4, neighbor indices 4k, field 8, type 1,
-    # gradient 24, Laplacian 8, status 1 (each read or written once).
-    return 24 + 4 + 4 * k_mean + 8 + 1 + 24 + 8 + 1
+    # k_wls_mixed: position 24 + field 8, list length 4, neighbor indices 4k,
+    # type 1 read; gradient 24, Laplacian 8, status 1 written.
+    return 24 + 8 + 4 + 4 * k_mean + 1 + 24 + 8 + 1
 
 
 def cnl_bytes_per_particle(k_mean):
-    # positions 24, row pointer 4, neighbor indices 4k, type read+write 2.
-    return 24 + 4 + 4 * k_mean + 2
+    # k_cnl_pass (neighbor search + classification): position 24 and type 1
+    # read; neighbor indices 4k, list length 4, new type 1 written.
+    return 24 + 1 + 4 * k_mean + 4 + 1
 
 
-def cpu_baseline(n_sample, l0, maxn):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_cnl_pass_k_wls_mixed_industrial_mix_3M.txt")
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter
+    reads for a device copy of known size), or None when the summary is
+    absent or was taken on another particle count."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            lines = [ln.split() for ln in f if ln.startswith("traffic ")]
+    except OSError:
+        return None
+    for fields in lines:
+        # traffic <kernel> <particles> <fetch_bytes> <write_bytes>
+        if len(fields) == 5 and fields[1] == kernel and int(fields[2]) == n:
+            return int(float(fields[3]) + float(fields[4]))
+    return None
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB through the HIP runtime (its
+    blit kernel), so counter runs can scale FETCH_SIZE / WRITE_SIZE by what
+    they read for a transfer of known size."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy calibration: 3 device copies of {nbytes} bytes")
+
+
+def cpu_baseline(xyz, types, l0, maxn):
+    """Neighbor search + classification + WLS over every particle on the
+    host, single thread. Prefers the reference's own src/lsmps.cpp built into
+    oracle/_ref (kind "reference"); falls back to the repository's oracle
+    (kind "port") on a bounded sample when that build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+
+    import numpy as np
     import oracle_py as O
 
+    n = len(types)
+    try:
+        ref = O.RefHarness()
+    except (FileNotFoundError, OSError):
+        ref = None
+    if ref is not None:
+        # wls_samples = n makes the reference's wls_test visit every particle
+        # (stride 1), so its WLS stage covers the whole workload.
+        ref.setup_particles(xyz, types, l0, 2.1, maxn=maxn, wls_samples=n, max_unknowns=n)
+        out = np.zeros(4)
+        ref.L.ref_time_stages(out.ctypes.data_as(C.c_void_p))
+        cpu_ms = out[0] + out[1] + out[2]
+        return {
+            "value": round(n / cpu_ms / 1e3, 4),
+            "unit": "Mparticles/s",
+            "cores": 1,
+            "kind": "reference",
+            "sample": f"the full {n}-particle workload through the reference's src/lsmps.cpp (g++ -O3, its own "
+                      f"flags), one thread: neighbors {out[0]:.0f} ms + classify {out[1]:.0f} ms + wls_test over "
+                      f"all particles {out[2]:.0f} ms; its assemble_ppe over all unknowns took {out[3]:.0f} ms "
+                      f"(not in value)",
+        }
     import lsmps_amd as L
 
-    xyz, t = L.generate_case("industrial_mix", n_sample, 23, l0)
-    o = O.Oracle(xyz, t)
-    rs, ws, ps = o.run(l0, 2.1, maxn, 8192, 0, 0, 0.0, do_solve=False)
+    n_sample = max(10_000, n // 10)
+    xyz_s, t_s = L.generate_case("industrial_mix", n_sample, 23, l0)
+    o = O.Oracle(xyz_s, t_s)
+    rs, _, _ = o.run(l0, 2.1, maxn, 8192, 0, 0, 0.0, do_solve=False)
     cpu_ms = rs["neighbor_ms"] + rs["classify_ms"] + rs["wls_all_ms"]
     return {
         "value": round(n_sample / cpu_ms / 1e3, 4),
         "unit": "Mparticles/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"industrial_mix scene of {n_sample} particles (same generator, 1/{3_000_000 // n_sample} size): "
-                  f"neighbors {rs['neighbor_ms']:.0f} ms + classify {rs['classify_ms']:.0f} ms + "
-                  f"WLS over all particles {rs['wls_all_ms']:.0f} ms, single thread",
+        "sample": f"industrial_mix scene of {n_sample} particles (same generator): neighbors "
+                  f"{rs['neighbor_ms']:.0f} ms + classify {rs['classify_ms']:.0f} ms + WLS over all particles "
+                  f"{rs['wls_all_ms']:.0f} ms, single thread",
     }
 
 
@@ -97,6 +179,8 @@ def main():
     n = args.particles
     log(f"generating industrial_mix n={n}")
     xyz, types = L.generate_case("industrial_mix", n, 23, l0)
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
     p = L.Pipeline(0)
     p.upload(xyz, types)
     p.set_field(L.C1_FIELD)
@@ -111,32 +195,34 @@ def main():
     log(f"warmup done ({args.warmup}); timing {args.steps} steps")
     p.L.lsmps_synchronize(p.h)
     t0 = time.perf_counter()
-    stage = {"cnl": [], "classify": [], "wls": []}
+    stage = {"cnl": [], "wls": [], "k_cnl_pass": [], "k_wls_mixed": []}
     for _ in range(args.steps):
         cs, ts, wr = step()
         stage["cnl"].append(cs["ms"])
-        stage["classify"].append(ts["ms"])
         stage["wls"].append(wr["ms"])
+        stage["k_cnl_pass"].append(cs["kernel_ms"])
+        stage["k_wls_mixed"].append(wr["kernel_ms"])
     p.L.lsmps_synchronize(p.h)
     wall_ms = (time.perf_counter() - t0) * 1e3 / max(1, args.steps)
-    med = {k: float(np.median(v)) if v else 0.0 for k, v in stage.items()}
-    gpu_step_ms = med["cnl"] + med["classify"] + med["wls"]
+    avg = {k: float(np.mean(v)) if v else 0.0 for k, v in stage.items()}
     kmean = cs["mean"]
-    log(f"step wall {wall_ms:.3f} ms, gpu stages {med}, k_mean {kmean:.2f}, wls {wr}")
+    log(f"step wall {wall_ms:.3f} ms, gpu stages {avg}, k_mean {kmean:.2f}, wls {wr}")
 
-    # Roofline of the dominant kernel (the WLS pass, one launch per step).
-    wls_bytes = n * wls_bytes_per_particle(kmean)
-    wls_gbs = wls_bytes / (med["wls"] * 1e-3) / 1e9
-    cnl_bytes = n * cnl_bytes_per_particle(kmean)
-    cnl_gbs = cnl_bytes / (med["cnl"] * 1e-3) / 1e9
+    # Roofline of the dominant kernel: algorithmic bytes of one launch over
+    # its average duration (HIP events around that launch on the library's
+    # stream).
+    per_particle = {"k_cnl_pass": cnl_bytes_per_particle(kmean), "k_wls_mixed": wls_bytes_per_particle(kmean)}
+    dom = max(per_particle, key=lambda k: avg[k])
+    dom_bytes = n * per_particle[dom]
+    dom_gbs = dom_bytes / (avg[dom] * 1e-3) / 1e9
+    kernel_gbs = {k: round(n * per_particle[k] / (avg[k] * 1e-3) / 1e9, 1) for k in per_particle if avg[k] > 0}
 
-    extra = {"stage_ms": med, "neighbor_mean": kmean, "pairs": cs["pairs"], "types": ts["count"],
-             "wls_fp64_redo": wr["fp64_redo"], "cnl_algorithmic_GBs": round(cnl_gbs, 1),
-             "wls_algorithmic_GBs": round(wls_gbs, 1)}
+    extra = {"stage_ms": {k: round(v, 4) for k, v in avg.items()}, "neighbor_mean": kmean, "pairs": cs["pairs"],
+             "types": ts["count"], "wls_fp64_redo": wr["fp64_redo"], "kernel_algorithmic_GBs": kernel_gbs}
 
     if not args.no_solve:
         pr = p.assemble(re, 1 << 30, L.XSTAR_RANDOM, 11)
-        extra["ppe"] = {"unknowns": pr["unknowns"], "nnz": pr["nnz"], "assembly_ms": pr["ms"]}
+        extra["ppe"] = {"unknowns": pr["unknowns"], "nnz": pr["nnz"], "assembly_ms": round(pr["ms"], 3)}
         log(f"assembly {pr}")
         solves = {}
         for name, method in (("fp64_jacobi", L.SOLVE_JACOBI), ("mixed", L.SOLVE_MIXED)):
@@ -151,30 +237,29 @@ def main():
 
     out = {
         "metric": "CNL+WLS Mparticles/s (PPE time-to-1e-8 residual, mixed vs plain FP64, in ppe_time_to_1e-8)",
-        "value": round(n / gpu_step_ms / 1e3, 3),
+        "value": round(n / wall_ms / 1e3, 3),
         "unit": "Mparticles/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(gpu_step_ms, 4),
-        "wall_ms_per_step": round(wall_ms, 4),
+        "ms_per_step": round(wall_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 moments / fp64 positions, CSR and redo",
+        "dtype": "f32 moments, f64 positions, f64 redo of flagged rows",
         "data": "synthetic",
         "config": {"workload": f"industrial_mix {n} particles: 70% jittered bulk, 20% tilted wall film 2-4 l0, "
-                               f"10% Poisson-disk splash at 0.2/l0^3; l0=0.004, re=2.1 l0, max_neighbors=128",
+                               f"10% Poisson-disk splash at 0.2/l0^3; l0=0.004, re=2.1 l0, max_neighbors=128; "
+                               f"step = neighbor search + classification + WLS gradient/Laplacian",
                    "particles": n, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(wls_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(wls_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_wls_mixed (+ k_wls_exact_list redo)",
-                     "bytes_per_launch": int(wls_bytes)},
+        "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, n),
+                     "kernel": dom, "kernel_ms": round(avg[dom], 4), "algorithmic_bytes": int(dom_bytes)},
     }
     out.update(extra)
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(max(10_000, n // 10), l0, maxn)
+            out["cpu_baseline"] = cpu_baseline(xyz, types, l0, maxn)
         except Exception as e:  # the oracle library may be absent on a bare checkout
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LSMPS_HIP_ABI_VERSION 1
+#define LSMPS_HIP_ABI_VERSION 2
 
 enum {
   LSMPS_OK = 0,
@@ -68,6 +68,7 @@ typedef struct {
   int32_t max_count;
   int32_t grid[3];       /* cells per axis of the dense grid */
   float ms;              /* GPU time of the whole stage */
+  float kernel_ms;       /* GPU time of the neighbor pass kernel alone */
 } lsmps_cnl_stats;
 /* Lists are CSR (row_ptr int32[n+1], col int32[pairs]) in the reference's
  * visiting order, truncated to max_neighbors (<= 1024). */
@@ -113,6 +114,7 @@ typedef struct {
   uint64_t ok, regularized, failed, skipped;
   uint64_t fp64_redo; /* MIXED: rows redone in FP64 */
   float ms;
+  float kernel_ms;    /* GPU time of the main WLS kernel alone (no field, no FP64 redo) */
 } lsmps_wls_result;
 /* Gradient and Laplacian for every particle except Wall and Splash. */
 int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out);

@@ -68,7 +68,7 @@ struct GraphCache {
 struct lsmps_ctx {
   int device = 0;
   hipStream_t st = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;  // stage / kernel
   std::string err;
 
   // particles (SoA FP64 positions, types)
@@ -149,6 +149,14 @@ float elapsed(lsmps_ctx* ctx) {
   return ms;
 }
 
+// Time between e2 and e3, recorded around a stage's main kernel.
+float kernel_elapsed(lsmps_ctx* ctx) {
+  float ms = 0.0f;
+  if (hipEventSynchronize(ctx->e3) != hipSuccess) return -1.0f;
+  if (hipEventElapsedTime(&ms, ctx->e2, ctx->e3) != hipSuccess) return -1.0f;
+  return ms;
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -186,6 +194,7 @@ int lsmps_create(int device, lsmps_ctx** out) {
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->e0) != hipSuccess || hipEventCreate(&ctx->e1) != hipSuccess ||
+      hipEventCreate(&ctx->e2) != hipSuccess || hipEventCreate(&ctx->e3) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->host_sc), sizeof(SolverScalars)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->host_small), 64 * sizeof(int64_t)) != hipSuccess ||
       ctx->counters.ensure(16) != hipSuccess || ctx->sc.ensure(1) != hipSuccess ||
@@ -229,6 +238,8 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   if (ctx->host_small) (void)hipHostFree(ctx->host_small);
   if (ctx->e0) (void)hipEventDestroy(ctx->e0);
   if (ctx->e1) (void)hipEventDestroy(ctx->e1);
+  if (ctx->e2) (void)hipEventDestroy(ctx->e2);
+  if (ctx->e3) (void)hipEventDestroy(ctx->e3);
   if (ctx->st) (void)hipStreamDestroy(ctx->st);
   delete ctx;
 }
@@ -378,7 +389,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, int32_t maxn, bool classify, doub
   zero_counters(ctx, 16);
   launch_fill_int(ctx->hist.p, maxn + 1, 0, st);
   launch_cnl_pass(B, n, re * re, maxn, classify, 1.5 * l0, ctx->type.p, ctx->type_s.p, ctx->counters.p, ctx->hist.p,
-                  ctx->counters.p + 8, st);
+                  ctx->counters.p + 8, st, ctx->e2, ctx->e3);
   if (int rc = check_launch(ctx, "cnl_pass")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
   std::vector<int> hist((size_t)maxn + 1);
@@ -390,6 +401,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, int32_t maxn, bool classify, doub
   fill_cnl_stats(ctx, hist, c[0], &cs);
   if (cs.pairs > (uint64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "neighbor pairs beyond int32 indexing");
   cs.ms = elapsed(ctx);
+  cs.kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
   ctx->re = re;
   ctx->maxn = maxn;
   ctx->pairs = cs.pairs;
@@ -499,11 +511,12 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   const int* nc = ctx->cnt.p;
   if (p->precision == LSMPS_WLS_FP64) {
     launch_wls_exact(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
-                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->counters.p, st);
+                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->counters.p, st, ctx->e2, ctx->e3);
   } else {
     launch_fill_int(ctx->nredo.p, 1, 0, st);
     launch_wls_mixed(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
-                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, st);
+                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, st, ctx->e2,
+                     ctx->e3);
   }
   if (int rc = check_launch(ctx, "wls")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
@@ -518,6 +531,7 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
     out->skipped = c[WLS_SKIPPED];
     out->fp64_redo = c[WLS_REDO_SLOT];
     out->ms = elapsed(ctx);
+    out->kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
   }
   return LSMPS_OK;
   GUARD_END

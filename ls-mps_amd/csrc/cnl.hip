@@ -334,16 +334,18 @@ void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t 
 
 void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classify, double zlim, uint8_t* type,
                      uint8_t* type_s, unsigned long long* trunc_total, int* hist, unsigned long long* tcount,
-                     hipStream_t st) {
+                     hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
   if (n <= 0) return;
   const size_t lds = (size_t)(WAVE / CNL_GROUP) * (size_t)(maxn > 0 ? maxn : 1) * sizeof(int);
   const unsigned grid = blocks(n, WAVE / CNL_GROUP);
+  if (ev_begin) (void)hipEventRecord(ev_begin, st);
   if (classify)
     hipLaunchKernelGGL(k_cnl_pass<true>, dim3(grid), dim3(WAVE), lds, st, B.spos, B.perm, B.cstart, B.scell, n, B.gx,
                        B.gy, B.gz, re2, maxn, B.nbr, B.cnt, B.cnt_orig, type, type_s, zlim, trunc_total);
   else
     hipLaunchKernelGGL(k_cnl_pass<false>, dim3(grid), dim3(WAVE), lds, st, B.spos, B.perm, B.cstart, B.scell, n, B.gx,
                        B.gy, B.gz, re2, maxn, B.nbr, B.cnt, B.cnt_orig, type, type_s, zlim, trunc_total);
+  if (ev_end) (void)hipEventRecord(ev_end, st);
   const unsigned gs = blocks(n, 256) < 1024u ? blocks(n, 256) : 1024u;
   hipLaunchKernelGGL(k_cnl_stats, dim3(gs), dim3(256), (size_t)(maxn + 6) * sizeof(int), st, B.cnt, type_s, n, maxn,
                      hist, classify ? tcount : nullptr);

@@ -42,7 +42,7 @@ void launch_fill_int(int* p, int64_t n, int v, hipStream_t st);
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t st);
 void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classify, double zlim, uint8_t* type,
                      uint8_t* type_s, unsigned long long* trunc_total, int* hist, unsigned long long* tcount,
-                     hipStream_t st);
+                     hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* type, uint8_t* type_s,
                      unsigned long long* tcount, hipStream_t st);
 void launch_ell_to_csr(const CnlBufs& B, int n, int maxn, const int* row_ptr, int* col, hipStream_t st);
@@ -54,10 +54,12 @@ void launch_field_quadratic(double4* spos, int n, const QuadField& q, hipStream_
 void launch_gather_field(const int* perm, const double* v, int n, double4* spos, hipStream_t st);
 void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status,
-                      unsigned long long* counts, hipStream_t st);
+                      unsigned long long* counts, hipStream_t st, hipEvent_t ev_begin = nullptr,
+                      hipEvent_t ev_end = nullptr);
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
-                      int* nredo, unsigned long long* counts, hipStream_t st);
+                      int* nredo, unsigned long long* counts, hipStream_t st, hipEvent_t ev_begin = nullptr,
+                      hipEvent_t ev_end = nullptr);
 void launch_wls_sample_errors(const double4* spos, const uint8_t* type, const double* grad, const double* lap,
                               const uint8_t* status, const int* iperm, int64_t stride, int ns, const QuadField& q,
                               double* ge, double* le, uint8_t* st_out, uint8_t* ty_out, hipStream_t st);

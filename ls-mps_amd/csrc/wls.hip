@@ -447,18 +447,23 @@ void launch_field_quadratic(double4* spos, int n, const QuadField& q, hipStream_
 
 void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status,
-                      unsigned long long* counts, hipStream_t st) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_wls_exact, dim3(blocks(n, 128)), dim3(128), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, reg,
-                       grad, lap, status, counts);
+                      unsigned long long* counts, hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
+  if (n <= 0) return;
+  if (ev_begin) (void)hipEventRecord(ev_begin, st);
+  hipLaunchKernelGGL(k_wls_exact, dim3(blocks(n, 128)), dim3(128), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, reg,
+                     grad, lap, status, counts);
+  if (ev_end) (void)hipEventRecord(ev_end, st);
 }
 
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
-                      int* nredo, unsigned long long* counts, hipStream_t st) {
+                      int* nredo, unsigned long long* counts, hipStream_t st, hipEvent_t ev_begin,
+                      hipEvent_t ev_end) {
   if (n <= 0) return;
+  if (ev_begin) (void)hipEventRecord(ev_begin, st);
   hipLaunchKernelGGL(k_wls_mixed, dim3(blocks(n, WAVE)), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, grad,
                      lap, status, redo, nredo, counts);
+  if (ev_end) (void)hipEventRecord(ev_end, st);
   // The redo list length is only known on the device: a bounded grid
   // strides over it (typical lists are empty or a few thousand entries).
   const unsigned g = blocks(n, 128) < 512u ? blocks(n, 128) : 512u;

@@ -34,7 +34,7 @@ class LsmpsError(RuntimeError):
 
 class CnlStats(C.Structure):
     _fields_ = [("pairs", _u64), ("truncated", _u64), ("mean", _d), ("p50", _i32), ("p90", _i32), ("p99", _i32),
-                ("max_count", _i32), ("grid", _i32 * 3), ("ms", _f)]
+                ("max_count", _i32), ("grid", _i32 * 3), ("ms", _f), ("kernel_ms", _f)]
 
 
 class TypeStats(C.Structure):
@@ -51,7 +51,7 @@ class WlsParams(C.Structure):
 
 class WlsResult(C.Structure):
     _fields_ = [("ok", _u64), ("regularized", _u64), ("failed", _u64), ("skipped", _u64), ("fp64_redo", _u64),
-                ("ms", _f)]
+                ("ms", _f), ("kernel_ms", _f)]
 
 
 class WlsStats(C.Structure):
