@@ -140,6 +140,8 @@ typedef struct {
 typedef struct {
   uint64_t unknowns, nnz, dirichlet, neumann, failed_rows; /* PpeStats fields */
   float ms;
+  float kernel_ms;     /* GPU time of the fill kernel alone */
+  uint64_t padded_nnz; /* stored entries of the sliced-ELL matrix (nnz + padding) */
 } lsmps_ppe_result;
 int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_result* out);
 

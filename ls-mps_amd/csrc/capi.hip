@@ -75,6 +75,7 @@ struct lsmps_ctx {
   uint64_t n = 0;
   DBuf<double> xyz_in, x, y, z;
   DBuf<uint8_t> type;
+  bool types_stale = false;  // classification has updated type_s only
 
   // cell list: particles in cell-major ("sorted") order and the ELL
   // neighbor table in sorted-slot indices; CSR only for downloads
@@ -84,9 +85,12 @@ struct lsmps_ctx {
   uint64_t pairs = 0;
   DBuf<int> cx, cy, cz, box, cell, slot, ccount, cstart, perm, iperm, scell, nbr, cnt, cnt_orig, row_ptr, col, hist;
   DBuf<double4> spos;
+  DBuf<float4> sposf;
+  double ox = 0.0, oy = 0.0, oz = 0.0;  // grid origin of the FP32 offsets
   DBuf<uint8_t> type_s;
   DBuf<int64_t> scan_scr;
   DBuf<unsigned long long> counters;  // 16 general-purpose counters
+  DBuf<unsigned> bpart;               // per-block count partials
 
   // field and WLS (results in sorted order; original order on download)
   bool have_field = false, field_quadratic = false, have_wls = false;
@@ -98,19 +102,29 @@ struct lsmps_ctx {
   // PPE
   bool have_ppe = false;
   uint64_t m = 0, nnz = 0;
-  DBuf<int> flag, pos, smap, rev, rcnt, prow, pcol;
-  DBuf<double> pval, b, xe, diag;
-
-  // solver (iteration matrix in sliced-ELL form)
-  DBuf<double> scale, yv, xsol, x0, vec64, partials, sell_v64, sell_vs64;
-  DBuf<float> vec32, sell_v32;
+  // Rows u are the unknowns in sorted-slot order, assembled straight into
+  // sliced-ELL (sell_*); rr numbers them as the reference does.
+  DBuf<int> flag, pos, fs, upos, uid, uslot, urr, rru, rlen;
+  DBuf<double> b, xe, diag;  // row order u
   DBuf<int> sell_w, sell_ptr, sell_col;
+  DBuf<double> sell_v64;
   uint64_t sell_nnz = 0;
+  // reference-order CSR and vectors, made on download / upload
+  DBuf<int> rcnt, prow, pcol, rev;
+  DBuf<double> pval, vrr;
+
+  // solver (vectors in row order u)
+  DBuf<double> scale, yv, xsol, x0, vec64, partials, sell_vs64;
+  DBuf<float> vec32, sell_v32;
   DBuf<SolverScalars> sc;
   SolverScalars* host_sc = nullptr;  // pinned
   int* host_small = nullptr;         // pinned scratch for small readbacks (64 ints)
   bool have_x0 = false, have_solution = false;
-  GraphCache graph;
+  GraphCache graph[2];  // captured iterations, one per solve precision (FP64, FP32)
+  void reset_graphs() {
+    graph[0].reset();
+    graph[1].reset();
+  }
 };
 
 namespace {
@@ -211,17 +225,20 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->st) (void)hipStreamSynchronize(ctx->st);
-  ctx->graph.reset();
+  ctx->reset_graphs();
   ctx->spos.release();
-  DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x, &ctx->y, &ctx->z, &ctx->f,
-                       &ctx->grad_s, &ctx->lap_s, &ctx->grad, &ctx->lap, &ctx->ge, &ctx->le, &ctx->pval, &ctx->b,
-                       &ctx->xe, &ctx->diag, &ctx->scale, &ctx->yv, &ctx->xsol, &ctx->x0, &ctx->vec64, &ctx->partials,
-                       &ctx->sell_v64, &ctx->sell_vs64};
+  ctx->sposf.release();
+  DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x,   &ctx->y,     &ctx->z,      &ctx->f,        &ctx->grad_s,
+                       &ctx->lap_s,  &ctx->grad, &ctx->lap, &ctx->ge,    &ctx->le,     &ctx->pval,     &ctx->vrr,
+                       &ctx->b,      &ctx->xe,   &ctx->diag, &ctx->scale, &ctx->yv,    &ctx->xsol,     &ctx->x0,
+                       &ctx->vec64,  &ctx->partials, &ctx->sell_v64, &ctx->sell_vs64};
   for (auto* p : d) p->release();
-  DBuf<int>* i[] = {&ctx->cx, &ctx->cy, &ctx->cz, &ctx->box, &ctx->cell, &ctx->slot, &ctx->ccount, &ctx->cstart,
-                    &ctx->perm, &ctx->iperm, &ctx->scell, &ctx->nbr, &ctx->cnt, &ctx->cnt_orig, &ctx->row_ptr,
-                    &ctx->col, &ctx->hist, &ctx->redo, &ctx->nredo, &ctx->flag, &ctx->pos, &ctx->smap, &ctx->rev,
-                    &ctx->rcnt, &ctx->prow, &ctx->pcol, &ctx->sell_w, &ctx->sell_ptr, &ctx->sell_col};
+  DBuf<int>* i[] = {&ctx->cx,    &ctx->cy,     &ctx->cz,    &ctx->box,      &ctx->cell,     &ctx->slot,
+                    &ctx->ccount, &ctx->cstart, &ctx->perm,  &ctx->iperm,    &ctx->scell,    &ctx->nbr,
+                    &ctx->cnt,   &ctx->cnt_orig, &ctx->row_ptr, &ctx->col,   &ctx->hist,     &ctx->redo,
+                    &ctx->nredo, &ctx->flag,   &ctx->pos,   &ctx->fs,       &ctx->upos,     &ctx->uid,
+                    &ctx->uslot, &ctx->urr,    &ctx->rru,   &ctx->rlen,     &ctx->rcnt,     &ctx->prow,
+                    &ctx->pcol,  &ctx->rev,    &ctx->sell_w, &ctx->sell_ptr, &ctx->sell_col};
   for (auto* p : i) p->release();
   ctx->type.release();
   ctx->type_s.release();
@@ -231,6 +248,7 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->ty_s.release();
   ctx->scan_scr.release();
   ctx->counters.release();
+  ctx->bpart.release();
   ctx->sell_v32.release();
   ctx->vec32.release();
   ctx->sc.release();
@@ -272,9 +290,10 @@ int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   }
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->n = n;
+  ctx->types_stale = false;
   ctx->have_cnl = ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_x0 = false;
   ctx->have_field = false;
-  ctx->graph.reset();
+  ctx->reset_graphs();
   return LSMPS_OK;
   GUARD_END
 }
@@ -336,10 +355,20 @@ CnlBufs cnl_bufs(lsmps_ctx* ctx) {
   B.iperm = ctx->iperm.p;
   B.scell = ctx->scell.p;
   B.spos = ctx->spos.p;
+  B.sposf = ctx->sposf.p;
+  B.ox = ctx->ox;
+  B.oy = ctx->oy;
+  B.oz = ctx->oz;
   B.nbr = ctx->nbr.p;
   B.cnt = ctx->cnt.p;
   B.cnt_orig = ctx->cnt_orig.p;
   return B;
+}
+
+// Original-order types from the sorted ones, after a classification.
+void sync_types(lsmps_ctx* ctx) {
+  if (ctx->types_stale) launch_unsort_u8(ctx->iperm.p, ctx->type_s.p, (int)ctx->n, ctx->type.p, ctx->st);
+  ctx->types_stale = false;
 }
 
 // Cell sort + one neighbor pass (+ classification when classify).
@@ -352,10 +381,12 @@ int build_cell_list(lsmps_ctx* ctx, double re, int32_t maxn, bool classify, doub
   hipStream_t st = ctx->st;
   if ((uint64_t)n * (uint64_t)std::max(1, maxn) > ((uint64_t)1 << 40))
     return fail(ctx, LSMPS_E_LIMIT, "neighbor table too large");
+  sync_types(ctx);  // with the previous order, before it is replaced
   DBuf<int>* per_particle[] = {&ctx->cx,   &ctx->cy,    &ctx->cz,    &ctx->cell, &ctx->slot,
                                &ctx->perm, &ctx->iperm, &ctx->scell, &ctx->cnt,  &ctx->cnt_orig};
   for (auto* b : per_particle) HIPCHK(b->ensure(n));
   HIPCHK(ctx->spos.ensure(n));
+  HIPCHK(ctx->sposf.ensure(n));
   HIPCHK(ctx->type_s.ensure(n));
   HIPCHK(ctx->nbr.ensure((size_t)n * (size_t)std::max(1, maxn)));
   HIPCHK(ctx->hist.ensure((size_t)maxn + 1));
@@ -379,6 +410,9 @@ int build_cell_list(lsmps_ctx* ctx, double re, int32_t maxn, bool classify, doub
   ctx->gx = (int)g[0];
   ctx->gy = (int)g[1];
   ctx->gz = (int)g[2];
+  ctx->ox = (double)bx[0] * re;
+  ctx->oy = (double)bx[1] * re;
+  ctx->oz = (double)bx[2] * re;
   HIPCHK(ctx->ccount.ensure((size_t)ncells));
   HIPCHK(ctx->cstart.ensure((size_t)ncells + 1));
   HIPCHK(ctx->scan_scr.ensure(std::max(scan_scratch_elems(ncells), scan_scratch_elems((int64_t)n + 1))));
@@ -388,7 +422,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, int32_t maxn, bool classify, doub
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
   zero_counters(ctx, 16);
   launch_fill_int(ctx->hist.p, maxn + 1, 0, st);
-  launch_cnl_pass(B, n, re * re, maxn, classify, 1.5 * l0, ctx->type.p, ctx->type_s.p, ctx->counters.p, ctx->hist.p,
+  launch_cnl_pass(B, n, re * re, maxn, classify, 1.5 * l0, ctx->type_s.p, ctx->counters.p, ctx->hist.p,
                   ctx->counters.p + 8, st, ctx->e2, ctx->e3);
   if (int rc = check_launch(ctx, "cnl_pass")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
@@ -404,6 +438,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, int32_t maxn, bool classify, doub
   cs.kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
   ctx->re = re;
   ctx->maxn = maxn;
+  ctx->types_stale = classify;
   ctx->pairs = cs.pairs;
   ctx->have_cnl = true;
   ctx->have_wls = ctx->have_ppe = ctx->have_solution = false;
@@ -442,8 +477,9 @@ int lsmps_classify(lsmps_ctx* ctx, double l0, lsmps_type_stats* out) {
   hipStream_t st = ctx->st;
   HIPCHK(hipEventRecord(ctx->e0, st));
   zero_counters(ctx, 16);
-  launch_classify(cnl_bufs(ctx), (int)ctx->n, ctx->maxn, 1.5 * l0, ctx->type.p, ctx->type_s.p, ctx->counters.p, st);
+  launch_classify(cnl_bufs(ctx), (int)ctx->n, ctx->maxn, 1.5 * l0, ctx->type_s.p, ctx->counters.p, st);
   if (int rc = check_launch(ctx, "classify")) return rc;
+  ctx->types_stale = true;
   HIPCHK(hipEventRecord(ctx->e1, st));
   unsigned long long c[5];
   HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
@@ -499,6 +535,7 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   HIPCHK(ctx->lap_s.ensure(n));
   HIPCHK(ctx->wstatus_s.ensure(n));
   if (p->precision == LSMPS_WLS_MIXED) HIPCHK(ctx->redo.ensure(n));
+  HIPCHK(ctx->bpart.ensure(6 * ((size_t)n / 64 + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
   // Field into the sorted records: evaluated there for a quadratic,
   // gathered for values.
@@ -511,12 +548,12 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   const int* nc = ctx->cnt.p;
   if (p->precision == LSMPS_WLS_FP64) {
     launch_wls_exact(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
-                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->counters.p, st, ctx->e2, ctx->e3);
+                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->counters.p, ctx->bpart.p, st, ctx->e2, ctx->e3);
   } else {
     launch_fill_int(ctx->nredo.p, 1, 0, st);
     launch_wls_mixed(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
-                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, st, ctx->e2,
-                     ctx->e3);
+                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, ctx->bpart.p, st,
+                     ctx->e2, ctx->e3);
   }
   if (int rc = check_launch(ctx, "wls")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
@@ -551,7 +588,7 @@ int lsmps_wls_sampled_stats(lsmps_ctx* ctx, int32_t samples, lsmps_wls_stats* ou
   HIPCHK(ctx->le.ensure(ns));
   HIPCHK(ctx->st_s.ensure(ns));
   HIPCHK(ctx->ty_s.ensure(ns));
-  launch_wls_sample_errors(ctx->spos.p, ctx->type.p, ctx->grad_s.p, ctx->lap_s.p,
+  launch_wls_sample_errors(ctx->spos.p, ctx->type_s.p, ctx->grad_s.p, ctx->lap_s.p,
                            ctx->wstatus_s.p, ctx->iperm.p, stride, ns, ctx->qf, ctx->ge.p, ctx->le.p, ctx->st_s.p,
                            ctx->ty_s.p, ctx->st);
   if (int rc = check_launch(ctx, "wls_sample_errors")) return rc;
@@ -598,16 +635,20 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
   if (!ctx || !p) return LSMPS_E_ARG;
   if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "PPE assembly needs neighbors");
   if (p->xmode != LSMPS_XSTAR_QUADRATIC && p->xmode != LSMPS_XSTAR_RANDOM) return fail(ctx, LSMPS_E_ARG, "bad xmode");
+  if (!(p->re > 0) || !std::isfinite(p->re)) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   const int n = (int)ctx->n;
   hipStream_t st = ctx->st;
   HIPCHK(ctx->flag.ensure(n));
   HIPCHK(ctx->pos.ensure((size_t)n + 1));
-  HIPCHK(ctx->smap.ensure(n));
+  HIPCHK(ctx->fs.ensure(n));
+  HIPCHK(ctx->upos.ensure((size_t)n + 1));
+  HIPCHK(ctx->uid.ensure(n));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
-  launch_unknown_flags(ctx->type.p, n, ctx->flag.p, st);
+  // Reference numbering: unknowns by particle index, the first cap of them.
+  launch_unknown_flags(ctx->type_s.p, ctx->iperm.p, n, ctx->flag.p, st);
   launch_exclusive_scan(ctx->flag.p, n, ctx->pos.p, ctx->scan_scr.p, st);
   if (int rc = check_launch(ctx, "unknown_scan")) return rc;
   HIPCHK(hipMemcpyAsync(ctx->host_small, ctx->pos.p + n, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -615,45 +656,58 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
   const int total = ctx->host_small[0];
   const int cap = std::max(0, std::min(p->max_unknowns, n));
   const int m = std::min(total, cap);
-  HIPCHK(ctx->rev.ensure(m));
-  HIPCHK(ctx->rcnt.ensure(m));
-  HIPCHK(ctx->prow.ensure((size_t)m + 1));
-  HIPCHK(ctx->xe.ensure(m));
+  // Row numbering: the same unknowns in sorted-slot order.
+  launch_unknown_slots(ctx->perm.p, ctx->flag.p, ctx->pos.p, n, cap, ctx->fs.p, st);
+  launch_exclusive_scan(ctx->fs.p, n, ctx->upos.p, ctx->scan_scr.p, st);
+  const int ns = ppe_slices(m);
+  HIPCHK(ctx->uslot.ensure(m));
+  HIPCHK(ctx->urr.ensure(m));
+  HIPCHK(ctx->rru.ensure(m));
+  HIPCHK(ctx->rlen.ensure(m));
   HIPCHK(ctx->b.ensure(m));
+  HIPCHK(ctx->xe.ensure(m));
   HIPCHK(ctx->diag.ensure(m));
-  const uint64_t nnz_cap = ctx->pairs + (uint64_t)m;
-  if (nnz_cap > (uint64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "PPE nonzeros beyond int32 indexing");
-  HIPCHK(ctx->pcol.ensure(nnz_cap));
-  HIPCHK(ctx->pval.ensure(nnz_cap));
-  HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)m + 1)));
-  launch_unknown_map(ctx->flag.p, ctx->pos.p, n, cap, ctx->iperm.p, ctx->smap.p, ctx->rev.p, st);
+  HIPCHK(ctx->sell_w.ensure(ns));
+  HIPCHK(ctx->sell_ptr.ensure((size_t)ns + 1));
+  launch_unknown_maps(ctx->perm.p, ctx->fs.p, ctx->upos.p, ctx->pos.p, n, ctx->uid.p, ctx->uslot.p, ctx->urr.p,
+                      ctx->rru.p, st);
+  launch_ppe_count(ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, m, ctx->sell_w.p, st);
+  HIPCHK(ctx->scan_scr.ensure(std::max(scan_scratch_elems((int64_t)n + 1), scan_scratch_elems((int64_t)ns + 1))));
+  launch_exclusive_scan(ctx->sell_w.p, ns, ctx->sell_ptr.p, ctx->scan_scr.p, st);
+  if (int rc = check_launch(ctx, "ppe_count")) return rc;
+  int64_t* hs = reinterpret_cast<int64_t*>(ctx->host_small);
+  const int ntiles = (ns + 2047) / 2048;
+  hs[0] = 0;
+  if (ns > 0) HIPCHK(hipMemcpyAsync(hs, ctx->scan_scr.p + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (hs[0] > (int64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "PPE matrix beyond int32 indexing");
+  ctx->sell_nnz = (uint64_t)hs[0];
+  HIPCHK(ctx->sell_col.ensure(ctx->sell_nnz));
+  HIPCHK(ctx->sell_v64.ensure(ctx->sell_nnz));
+  HIPCHK(ctx->bpart.ensure(3 * (size_t)ns + 1));
   zero_counters(ctx, 16);
-  launch_ppe_row_count(ctx->spos.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, ctx->smap.p, ctx->rev.p,
-                       ctx->iperm.p, m, p->re, ctx->rcnt.p, st);
-  launch_exclusive_scan(ctx->rcnt.p, m, ctx->prow.p, ctx->scan_scr.p, st);
-  launch_ppe_row_fill(ctx->spos.p, ctx->type.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, ctx->smap.p,
-                      ctx->rev.p, ctx->iperm.p, m, p->re, p->shift, p->xmode, p->xseed, ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->xe.p, ctx->diag.p,
-                      ctx->counters.p, st);
-  launch_ppe_rhs(ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->xe.p, m, ctx->b.p, st);
-  if (int rc = check_launch(ctx, "ppe")) return rc;
+  launch_ppe_fill(ctx->spos.p, ctx->type_s.p, ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn,
+                  m, p->re, p->shift, p->xmode, p->xseed, ctx->sell_ptr.p, ctx->sell_col.p, ctx->sell_v64.p,
+                  ctx->rlen.p, ctx->diag.p, ctx->xe.p, ctx->b.p, ctx->counters.p, ctx->bpart.p, st, ctx->e2, ctx->e3);
+  if (int rc = check_launch(ctx, "ppe_fill")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
-  unsigned long long c[2];
-  int nnz = 0;
+  unsigned long long c[3];
   HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&nnz, ctx->prow.p + m, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   ctx->m = (uint64_t)m;
-  ctx->nnz = (uint64_t)nnz;
+  ctx->nnz = m > 0 ? (uint64_t)c[2] : 0;
   ctx->have_ppe = true;
   ctx->have_solution = ctx->have_x0 = false;
-  ctx->graph.reset();
+  ctx->reset_graphs();
   if (out) {
     out->unknowns = (uint64_t)m;
-    out->nnz = (uint64_t)nnz;
+    out->nnz = ctx->nnz;
     out->dirichlet = (uint64_t)n - (uint64_t)m;
     out->neumann = c[1];
     out->failed_rows = c[0];
     out->ms = elapsed(ctx);
+    out->kernel_ms = m > 0 ? kernel_elapsed(ctx) : 0.0f;
+    out->padded_nnz = ctx->sell_nnz;
   }
   return LSMPS_OK;
   GUARD_END
@@ -666,7 +720,13 @@ int lsmps_upload_x0(lsmps_ctx* ctx, const double* x0, uint64_t n) {
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(ctx->x0.ensure(n));
-  if (n) HIPCHK(hipMemcpy(ctx->x0.p, x0, n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(ctx->vrr.ensure(2 * n));
+  if (n) {
+    HIPCHK(hipMemcpyAsync(ctx->vrr.p, x0, n * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    launch_from_rr(ctx->urr.p, ctx->vrr.p, (int)n, ctx->x0.p, ctx->st);
+    if (int rc = check_launch(ctx, "from_rr")) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->have_x0 = true;
   return LSMPS_OK;
   GUARD_END
@@ -685,7 +745,7 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
   if (int rc = check_launch(ctx, "solver_init")) return rc;
   // Capture `chunk` iterations once per buffer set; replay until the device
   // state leaves RUNNING (kernels past convergence return at once).
-  GraphCache& G = ctx->graph;
+  GraphCache& G = ctx->graph[sizeof(T) == 4 ? 1 : 0];
   const void* key[6] = {B.val, B.r, B.y, B.scale, B.sptr, B.val64};
   const bool hit = G.exec && G.m == B.m && G.chunk == chunk && G.with_replace == (int)with_replace &&
                    G.fp32 == (int)(sizeof(T) == 4) && std::equal(key, key + 6, G.key);
@@ -741,29 +801,6 @@ extern "C" {
 
 namespace {
 
-// Sliced-ELL structure of the assembled PPE (slice offsets, padded columns)
-// and its unscaled FP64 values.
-int build_sell_structure(lsmps_ctx* ctx) {
-  const int m = (int)ctx->m;
-  hipStream_t st = ctx->st;
-  const int ns = sell_slices(m);
-  HIPCHK(ctx->sell_w.ensure(ns));
-  HIPCHK(ctx->sell_ptr.ensure((size_t)ns + 1));
-  HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)ns + 1)));
-  launch_sell_widths(ctx->prow.p, m, ctx->sell_w.p, st);
-  launch_exclusive_scan(ctx->sell_w.p, ns, ctx->sell_ptr.p, ctx->scan_scr.p, st);
-  if (int rc = check_launch(ctx, "sell_widths")) return rc;
-  int64_t* hs = reinterpret_cast<int64_t*>(ctx->host_small);
-  const int ntiles = (ns + 2047) / 2048;
-  HIPCHK(hipMemcpyAsync(hs, ctx->scan_scr.p + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  if (hs[0] > (int64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "padded matrix beyond int32 indexing");
-  ctx->sell_nnz = (uint64_t)hs[0];
-  HIPCHK(ctx->sell_col.ensure(ctx->sell_nnz));
-  HIPCHK(ctx->sell_v64.ensure(ctx->sell_nnz));
-  return LSMPS_OK;
-}
-
 template <typename T>
 void bind_vectors(SolverBufs<T>& B, T* v, int m) {
   B.r = v;
@@ -781,8 +818,7 @@ int solve_fp64_jacobi(lsmps_ctx* ctx, const double* x0, int max_iter, double tol
   HIPCHK(ctx->vec64.ensure(6 * (size_t)m));
   HIPCHK(ctx->sell_vs64.ensure(ctx->sell_nnz));
   launch_inv_diag(ctx->diag.p, m, ctx->scale.p, st);
-  launch_sell_fill<double>(ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->scale.p, m, ctx->maxn + 1, ctx->sell_ptr.p, ctx->sell_col.p,
-                           ctx->sell_vs64.p, ctx->sell_v64.p, st);
+  launch_sell_scale<double>(ctx->sell_col.p, ctx->sell_v64.p, ctx->scale.p, (int64_t)ctx->sell_nnz, ctx->sell_vs64.p, st);
   SolverBufs<double> B;
   B.m = m;
   B.sptr = ctx->sell_ptr.p;
@@ -817,15 +853,13 @@ int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_res
   HIPCHK(ctx->yv.ensure(m));
   HIPCHK(ctx->scale.ensure(m));
   HIPCHK(hipEventRecord(ctx->e0, st));
-  if (int rc = build_sell_structure(ctx)) return rc;
   const double* x0 = p->use_x0 ? ctx->x0.p : nullptr;
   int rc = LSMPS_OK;
   if (p->method == LSMPS_SOLVE_MIXED) {
     HIPCHK(ctx->sell_v32.ensure(ctx->sell_nnz));
     HIPCHK(ctx->vec32.ensure(6 * (size_t)m));
     launch_inv_diag(ctx->diag.p, m, ctx->scale.p, st);
-    launch_sell_fill<float>(ctx->prow.p, ctx->pcol.p, ctx->pval.p, ctx->scale.p, m, ctx->maxn + 1, ctx->sell_ptr.p, ctx->sell_col.p,
-                            ctx->sell_v32.p, ctx->sell_v64.p, st);
+    launch_sell_scale<float>(ctx->sell_col.p, ctx->sell_v64.p, ctx->scale.p, (int64_t)ctx->sell_nnz, ctx->sell_v32.p, st);
     SolverBufs<float> B;
     B.m = m;
     B.sptr = ctx->sell_ptr.p;
@@ -858,8 +892,6 @@ int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_res
     rc = solve_fp64_jacobi(ctx, x0, p->max_iter, p->tol, chunk, out);
   } else {
     HIPCHK(ctx->vec64.ensure(6 * (size_t)m));
-    launch_sell_fill<double>(ctx->prow.p, ctx->pcol.p, ctx->pval.p, nullptr, m, ctx->maxn + 1, ctx->sell_ptr.p, ctx->sell_col.p,
-                             ctx->sell_v64.p, nullptr, st);
     SolverBufs<double> B;
     B.m = m;
     B.sptr = ctx->sell_ptr.p;
@@ -883,6 +915,8 @@ int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_res
 int lsmps_download_types(lsmps_ctx* ctx, uint8_t* type) {
   if (!ctx || (ctx->n && !type)) return LSMPS_E_ARG;
   HIPCHK(hipSetDevice(ctx->device));
+  sync_types(ctx);
+  if (int rc = check_launch(ctx, "unsort_types")) return rc;
   if (ctx->n) HIPCHK(hipMemcpyAsync(type, ctx->type.p, ctx->n, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return LSMPS_OK;
@@ -897,6 +931,7 @@ int lsmps_download_neighbors(lsmps_ctx* ctx, int32_t* row_ptr, int32_t* col) {
   HIPCHK(ctx->row_ptr.ensure((size_t)n + 1));
   HIPCHK(ctx->col.ensure(ctx->pairs));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
+  launch_unsort_int(ctx->iperm.p, ctx->cnt.p, n, ctx->cnt_orig.p, ctx->st);
   launch_exclusive_scan(ctx->cnt_orig.p, n, ctx->row_ptr.p, ctx->scan_scr.p, ctx->st);
   launch_ell_to_csr(cnl_bufs(ctx), n, ctx->maxn, ctx->row_ptr.p, ctx->col.p, ctx->st);
   if (int rc = check_launch(ctx, "ell_to_csr")) return rc;
@@ -929,26 +964,51 @@ int lsmps_download_ppe(lsmps_ctx* ctx, int32_t* row, int32_t* col, double* val, 
                        int32_t* rev) {
   if (!ctx) return LSMPS_E_ARG;
   if (!ctx->have_ppe) return fail(ctx, LSMPS_E_STATE, "no PPE");
+  GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
-  const size_t m = ctx->m, nnz = ctx->nnz;
+  const int m = (int)ctx->m;
+  const size_t nnz = ctx->nnz;
   hipStream_t st = ctx->st;
-  if (row) HIPCHK(hipMemcpyAsync(row, ctx->prow.p, (m + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx->rcnt.ensure(m));
+  HIPCHK(ctx->prow.ensure((size_t)m + 1));
+  HIPCHK(ctx->pcol.ensure(nnz));
+  HIPCHK(ctx->pval.ensure(nnz));
+  HIPCHK(ctx->rev.ensure(m));
+  HIPCHK(ctx->vrr.ensure(2 * (size_t)m));
+  HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)m + 1)));
+  // Reference-order CSR: rows rr, off-diagonals in list order, diagonal last.
+  launch_rr_lengths(ctx->rru.p, ctx->rlen.p, m, ctx->rcnt.p, st);
+  launch_exclusive_scan(ctx->rcnt.p, m, ctx->prow.p, ctx->scan_scr.p, st);
+  launch_sell_to_csr(ctx->rru.p, ctx->urr.p, ctx->sell_ptr.p, ctx->sell_col.p, ctx->sell_v64.p, ctx->rlen.p,
+                     ctx->prow.p, m, ctx->pcol.p, ctx->pval.p, st);
+  launch_to_rr(ctx->rru.p, ctx->b.p, m, ctx->vrr.p, st);
+  launch_to_rr(ctx->rru.p, ctx->xe.p, m, ctx->vrr.p + m, st);
+  launch_rr_particle(ctx->rru.p, ctx->uslot.p, ctx->perm.p, m, ctx->rev.p, st);
+  if (int rc = check_launch(ctx, "ppe_to_csr")) return rc;
+  if (row) HIPCHK(hipMemcpyAsync(row, ctx->prow.p, ((size_t)m + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
   if (col && nnz) HIPCHK(hipMemcpyAsync(col, ctx->pcol.p, nnz * sizeof(int), hipMemcpyDeviceToHost, st));
   if (val && nnz) HIPCHK(hipMemcpyAsync(val, ctx->pval.p, nnz * sizeof(double), hipMemcpyDeviceToHost, st));
-  if (b && m) HIPCHK(hipMemcpyAsync(b, ctx->b.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
-  if (x_exact && m) HIPCHK(hipMemcpyAsync(x_exact, ctx->xe.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
-  if (rev && m) HIPCHK(hipMemcpyAsync(rev, ctx->rev.p, m * sizeof(int), hipMemcpyDeviceToHost, st));
+  if (b && m) HIPCHK(hipMemcpyAsync(b, ctx->vrr.p, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (x_exact && m) HIPCHK(hipMemcpyAsync(x_exact, ctx->vrr.p + m, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (rev && m) HIPCHK(hipMemcpyAsync(rev, ctx->rev.p, (size_t)m * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return LSMPS_OK;
+  GUARD_END
 }
 
 int lsmps_download_solution(lsmps_ctx* ctx, double* x) {
   if (!ctx) return LSMPS_E_ARG;
   if (!ctx->have_solution) return fail(ctx, LSMPS_E_STATE, "no solution");
+  GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
-  if (x && ctx->m) HIPCHK(hipMemcpyAsync(x, ctx->xsol.p, ctx->m * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  const int m = (int)ctx->m;
+  HIPCHK(ctx->vrr.ensure(2 * (size_t)m));
+  launch_to_rr(ctx->rru.p, ctx->xsol.p, m, ctx->vrr.p, ctx->st);
+  if (int rc = check_launch(ctx, "to_rr")) return rc;
+  if (x && m) HIPCHK(hipMemcpyAsync(x, ctx->vrr.p, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return LSMPS_OK;
+  GUARD_END
 }
 
 }  // extern "C"

@@ -23,6 +23,11 @@
 // The classification (14 direction sectors of the kept neighbors, then the
 // Wall / Splash / NearWall / FreeSurface / Interior rules) is fused into the
 // same pass: it needs exactly the offsets the pass has just computed.
+// Candidates are screened in FP32 first; only those whose FP32 distance is
+// too close to re to decide take the FP64 test, so lists are unchanged.
+#include <algorithm>
+#include <cmath>
+
 #include "kernels.hpp"
 
 namespace lsmps_hip {
@@ -113,15 +118,20 @@ __global__ __launch_bounds__(256) void k_cell_sort(const int* __restrict__ start
 }
 
 // Sorted-slot copies of positions ({x, y, z, field} records, the field
-// filled in later) and cell ids, and the inverse permutation.
+// filled in later), FP32 offsets from the grid origin (for the candidate
+// prefilter), cell ids, and the inverse permutation.
 __global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ perm, const int* __restrict__ cell,
                                                        const double* __restrict__ x, const double* __restrict__ y,
-                                                       const double* __restrict__ z, int n, double4* __restrict__ spos,
-                                                       int* __restrict__ scell, int* __restrict__ iperm) {
+                                                       const double* __restrict__ z, int n, double ox, double oy,
+                                                       double oz, double4* __restrict__ spos,
+                                                       float4* __restrict__ sposf, int* __restrict__ scell,
+                                                       int* __restrict__ iperm) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
-  spos[s] = make_double4(x[i], y[i], z[i], 0.0);
+  const double a = x[i], b = y[i], c = z[i];
+  spos[s] = make_double4(a, b, c, 0.0);
+  sposf[s] = make_float4((float)(a - ox), (float)(b - oy), (float)(c - oz), 0.0f);
   scell[s] = cell[i];
   iperm[i] = s;
 }
@@ -146,31 +156,49 @@ __device__ __forceinline__ uint8_t classify_rule(uint8_t t0, int kept, double z,
   return T_INTERIOR;
 }
 
-// One wavefront per 8 consecutive sorted particles (nearly always from the
-// same few cells), 8 lanes per particle. The lanes step through each
-// candidate run 8 at a time, so position loads are coalesced; a wavefront
-// ballot gives every hit its rank among the group's hits, which keeps the
-// reference's visiting order (and the cut at max_neighbors). Hits are
-// staged in LDS; afterwards the group writes the row out with coalesced
-// stores and, when CLASSIFY, takes the direction sectors of the kept
-// neighbors only (~30) rather than of every candidate (~250).
+// Eight lanes per particle, eight consecutive sorted particles (nearly
+// always from the same few cells) per wavefront. The lanes step through
+// each candidate run sixteen at a time (two loads in flight per lane), so
+// position loads are coalesced; a ballot gives every hit its rank among
+// the group's hits, which keeps the reference's visiting order (and the
+// cut at max_neighbors). Hits are staged in LDS; afterwards the group
+// writes the row out with coalesced stores and, when CLASSIFY, takes the
+// direction sectors of the kept neighbors only (~30) rather than of every
+// candidate (~250).
+//
+// Candidate test: the squared distance in FP32 from 16-byte offset records
+// decides every pair outside [lo, hi], a band around re^2 wider than the
+// FP32 error bound (see cnl_prefilter_band); pairs inside it take the
+// reference's FP64 test, so the result is exactly the FP64 predicate.
 constexpr int CNL_GROUP = 8;
+__device__ __forceinline__ bool cnl_near(const float4 q, const float4 pf, int s2, const double4 pi,
+                                         const double4* __restrict__ spos, double re2, float lo, float hi) {
+  const float dx = q.x - pf.x, dy = q.y - pf.y, dz = q.z - pf.z;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  if (d2 < lo) return true;
+  if (d2 > hi) return false;
+  const double4 o = spos[s2];
+  const double ddx = o.x - pi.x, ddy = o.y - pi.y, ddz = o.z - pi.z;
+  return ddx * ddx + ddy * ddy + ddz * ddz < re2;
+}
 template <bool CLASSIFY>
-__global__ __launch_bounds__(WAVE) void k_cnl_pass(const double4* __restrict__ spos, const int* __restrict__ perm,
-                                                   const int* __restrict__ start, const int* __restrict__ scell, int n,
-                                                   int gx, int gy, int gz, double re2, int maxn, int* __restrict__ nbr,
-                                                   int* __restrict__ cnt, int* __restrict__ cnt_orig,
-                                                   uint8_t* __restrict__ type, uint8_t* __restrict__ type_s,
-                                                   double zlim, unsigned long long* __restrict__ trunc_total) {
-  extern __shared__ int rowbuf[];  // (WAVE / CNL_GROUP) rows of maxn
-  const int lane = threadIdx.x, g = lane & (CNL_GROUP - 1), sub = lane / CNL_GROUP;
-  const int s = blockIdx.x * (WAVE / CNL_GROUP) + sub;
+__global__ __launch_bounds__(256) void k_cnl_pass(const double4* __restrict__ spos, const float4* __restrict__ sposf,
+                                                  const int* __restrict__ start, const int* __restrict__ scell, int n,
+                                                  int gx, int gy, int gz, double re2, float lo, float hi, int maxn,
+                                                  int* __restrict__ nbr, int* __restrict__ cnt,
+                                                  uint8_t* __restrict__ type_s, double zlim,
+                                                  unsigned long long* __restrict__ trunc_total) {
+  extern __shared__ int rowbuf[];  // (blockDim.x / CNL_GROUP) rows of maxn
+  const int lane = threadIdx.x & (WAVE - 1), g = lane & (CNL_GROUP - 1), sub = threadIdx.x / CNL_GROUP;
+  const int per_block = blockDim.x / CNL_GROUP;
+  const int s = (int)xcd_block(blockIdx.x, gridDim.x) * per_block + sub;
   const bool live = s < n;
   int* buf = rowbuf + sub * maxn;
   int found = 0;
   double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
   if (live) {
     pi = spos[s];
+    const float4 pf = sposf[s];
     const int c = scell[s];
     const int cxl = c % gx, t = c / gx, cyl = t % gy, czl = t / gy;
     const int x0 = cxl > 0 ? cxl - 1 : 0, x1 = cxl + 1 < gx ? cxl + 1 : gx - 1;
@@ -184,20 +212,25 @@ __global__ __launch_bounds__(WAVE) void k_cnl_pass(const double4* __restrict__ s
         if (yy < 0 || yy >= gy) continue;
         const int rowc = (zz * gy + yy) * gx;
         const int b = start[rowc + x0], e = start[rowc + x1 + 1];
-        for (int base = b; base < e; base += CNL_GROUP) {
-          const int s2 = base + g;
-          bool hit = false;
-          if (s2 < e && s2 != s) {
-            const double4 q = spos[s2];
-            const double ddx = q.x - pi.x, ddy = q.y - pi.y, ddz = q.z - pi.z;
-            hit = ddx * ddx + ddy * ddy + ddz * ddz < re2;
+        for (int base = b; base < e; base += 2 * CNL_GROUP) {
+          const int sa = base + g, sb = base + CNL_GROUP + g;
+          float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = qa;
+          if (sa < e) qa = sposf[sa];
+          if (sb < e) qb = sposf[sb];
+          const bool ha = sa < e && sa != s && cnl_near(qa, pf, sa, pi, spos, re2, lo, hi);
+          const bool hb = sb < e && sb != s && cnl_near(qb, pf, sb, pi, spos, re2, lo, hi);
+          const unsigned ma = (unsigned)(__ballot(ha) >> gshift) & 0xFFu;
+          const unsigned mb = (unsigned)(__ballot(hb) >> gshift) & 0xFFu;
+          if (ha) {
+            const int pos = found + __popc(ma & below);
+            if (pos < maxn) buf[pos] = sa;
           }
-          const unsigned gm = (unsigned)(__ballot(hit) >> gshift) & 0xFFu;
-          if (hit) {
-            const int pos = found + __popc(gm & below);
-            if (pos < maxn) buf[pos] = s2;
+          found += __popc(ma);
+          if (hb) {
+            const int pos = found + __popc(mb & below);
+            if (pos < maxn) buf[pos] = sb;
           }
-          found += __popc(gm);
+          found += __popc(mb);
         }
       }
     }
@@ -219,14 +252,8 @@ __global__ __launch_bounds__(WAVE) void k_cnl_pass(const double4* __restrict__ s
   if (CLASSIFY) mask = group8_or(mask);
   const bool leader = live && g == 0;
   if (leader) {
-    const int i = perm[s];
     cnt[s] = kept;
-    cnt_orig[i] = kept;
-    if (CLASSIFY) {
-      const uint8_t nt = classify_rule(type[i], kept, pi.z, zlim, mask);
-      type[i] = nt;
-      type_s[s] = nt;
-    }
+    if (CLASSIFY) type_s[s] = classify_rule(type_s[s], kept, pi.z, zlim, mask);
   }
   unsigned long long tr = leader ? (unsigned long long)(found - kept) : 0ull;
   tr = wave_sum(tr);
@@ -252,15 +279,15 @@ __global__ __launch_bounds__(256) void k_cnl_stats(const int* __restrict__ cnt, 
     atomicAdd(&tcount[threadIdx.x], (unsigned long long)bins[maxn + 1 + threadIdx.x]);
 }
 
-// Stand-alone classification over the ELL table (src/lsmps.cpp:27,73).
-__global__ __launch_bounds__(256) void k_classify(const double4* __restrict__ spos, const int* __restrict__ perm,
-                                                  const int* __restrict__ nbr, const int* __restrict__ cnt, int n,
-                                                  int maxn, double zlim, uint8_t* __restrict__ type,
-                                                  uint8_t* __restrict__ type_s, unsigned long long* __restrict__ tcount) {
+// Stand-alone classification over the ELL table (src/lsmps.cpp:27,73),
+// on sorted-order types.
+__global__ __launch_bounds__(256) void k_classify(const double4* __restrict__ spos, const int* __restrict__ nbr,
+                                                  const int* __restrict__ cnt, int n, int maxn, double zlim,
+                                                  uint8_t* __restrict__ type_s,
+                                                  unsigned long long* __restrict__ tcount) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   int out = -1;
   if (s < n) {
-    const int i = perm[s];
     const int k = cnt[s];
     const int* row = nbr + (size_t)s * maxn;
     unsigned mask = 0;
@@ -269,8 +296,7 @@ __global__ __launch_bounds__(256) void k_classify(const double4* __restrict__ sp
       const double4 o = spos[row[q]];
       mask |= sector_bits(o.x - pi.x, o.y - pi.y, o.z - pi.z);
     }
-    out = classify_rule(type[i], k, pi.z, zlim, mask);
-    type[i] = (uint8_t)out;
+    out = classify_rule(type_s[s], k, pi.z, zlim, mask);
     type_s[s] = (uint8_t)out;
   }
   __shared__ unsigned c5[5];
@@ -279,6 +305,14 @@ __global__ __launch_bounds__(256) void k_classify(const double4* __restrict__ sp
   if (out >= 0) atomicAdd(&c5[out], 1u);
   __syncthreads();
   if (threadIdx.x < 5 && c5[threadIdx.x]) atomicAdd(&tcount[threadIdx.x], (unsigned long long)c5[threadIdx.x]);
+}
+
+// Original-order copy of a sorted-order array: dst[i] = src[iperm[i]].
+template <typename T>
+__global__ __launch_bounds__(256) void k_unsort(const int* __restrict__ iperm, const T* __restrict__ src, int n,
+                                                T* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[iperm[i]];
 }
 
 // CSR in original indexing and reference order, from the ELL table.
@@ -328,34 +362,63 @@ void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t 
     hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart, n, B.perm);
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks(ncells, 256)), dim3(256), 0, st, B.cstart, (int)ncells, B.perm);
     hipLaunchKernelGGL(k_gather_sorted, dim3(blocks(n, 256)), dim3(256), 0, st, B.perm, B.cell, B.x, B.y, B.z, n,
-                       B.spos, B.scell, B.iperm);
+                       B.ox, B.oy, B.oz, B.spos, B.sposf, B.scell, B.iperm);
   }
 }
 
-void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classify, double zlim, uint8_t* type,
-                     uint8_t* type_s, unsigned long long* trunc_total, int* hist, unsigned long long* tcount,
-                     hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
+// Band [lo, hi] around re^2 outside which the FP32 squared distance of two
+// offset records decides the FP64 predicate d^2 < re^2. Each offset is
+// rounded once from FP64 (error <= E 2^-24 for offsets up to E, the grid
+// extent), a difference adds |d| 2^-24 with |d| <= 2 re near the boundary,
+// so each FP32 difference is within eta = 2 E 2^-24 + 2 re 2^-24 of the
+// exact one; the squared norm then errs by at most 2 sqrt(3) eta |d| +
+// 3 eta^2 plus four roundings of ~re^2 2^-24. Twice that, plus a relative
+// 1e-9 for the FP64 side, is the half-width.
+static void cnl_prefilter_band(double re2, double extent, float* lo, float* hi) {
+  const double u = 0x1.0p-24, re = std::sqrt(re2);
+  const double eta = 2.0 * extent * u + 2.0 * re * u;
+  const double m = 2.0 * (2.0 * std::sqrt(3.0) * eta * (1.01 * re) + 3.0 * eta * eta + 4.0 * 1.1 * re2 * u) + 1e-9 * re2;
+  *lo = (float)(re2 - m);
+  *hi = (float)(re2 + m);
+}
+
+void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classify, double zlim, uint8_t* type_s,
+                     unsigned long long* trunc_total, int* hist, unsigned long long* tcount, hipStream_t st,
+                     hipEvent_t ev_begin, hipEvent_t ev_end) {
   if (n <= 0) return;
-  const size_t lds = (size_t)(WAVE / CNL_GROUP) * (size_t)(maxn > 0 ? maxn : 1) * sizeof(int);
-  const unsigned grid = blocks(n, WAVE / CNL_GROUP);
+  // 256 threads (32 particles) per block while the LDS rows stay <= 32 KB.
+  const int threads = maxn <= 256 ? 256 : WAVE;
+  const size_t lds = (size_t)(threads / CNL_GROUP) * (size_t)(maxn > 0 ? maxn : 1) * sizeof(int);
+  const unsigned grid = blocks(n, threads / CNL_GROUP);
+  const double extent = std::sqrt(re2) * (double)(std::max(B.gx, std::max(B.gy, B.gz)) + 1);
+  float lo, hi;
+  cnl_prefilter_band(re2, extent, &lo, &hi);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
   if (classify)
-    hipLaunchKernelGGL(k_cnl_pass<true>, dim3(grid), dim3(WAVE), lds, st, B.spos, B.perm, B.cstart, B.scell, n, B.gx,
-                       B.gy, B.gz, re2, maxn, B.nbr, B.cnt, B.cnt_orig, type, type_s, zlim, trunc_total);
+    hipLaunchKernelGGL(k_cnl_pass<true>, dim3(grid), dim3(threads), lds, st, B.spos, B.sposf, B.cstart, B.scell, n,
+                       B.gx, B.gy, B.gz, re2, lo, hi, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
   else
-    hipLaunchKernelGGL(k_cnl_pass<false>, dim3(grid), dim3(WAVE), lds, st, B.spos, B.perm, B.cstart, B.scell, n, B.gx,
-                       B.gy, B.gz, re2, maxn, B.nbr, B.cnt, B.cnt_orig, type, type_s, zlim, trunc_total);
+    hipLaunchKernelGGL(k_cnl_pass<false>, dim3(grid), dim3(threads), lds, st, B.spos, B.sposf, B.cstart, B.scell, n,
+                       B.gx, B.gy, B.gz, re2, lo, hi, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
   if (ev_end) (void)hipEventRecord(ev_end, st);
   const unsigned gs = blocks(n, 256) < 1024u ? blocks(n, 256) : 1024u;
   hipLaunchKernelGGL(k_cnl_stats, dim3(gs), dim3(256), (size_t)(maxn + 6) * sizeof(int), st, B.cnt, type_s, n, maxn,
                      hist, classify ? tcount : nullptr);
 }
 
-void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* type, uint8_t* type_s,
-                     unsigned long long* tcount, hipStream_t st) {
+void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* type_s, unsigned long long* tcount,
+                     hipStream_t st) {
   if (n > 0)
-    hipLaunchKernelGGL(k_classify, dim3(blocks(n, 256)), dim3(256), 0, st, B.spos, B.perm, B.nbr, B.cnt, n, maxn, zlim,
-                       type, type_s, tcount);
+    hipLaunchKernelGGL(k_classify, dim3(blocks(n, 256)), dim3(256), 0, st, B.spos, B.nbr, B.cnt, n, maxn, zlim, type_s,
+                       tcount);
+}
+
+void launch_unsort_u8(const int* iperm, const uint8_t* src, int n, uint8_t* dst, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_unsort<uint8_t>, dim3(blocks(n, 256)), dim3(256), 0, st, iperm, src, n, dst);
+}
+
+void launch_unsort_int(const int* iperm, const int* src, int n, int* dst, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_unsort<int>, dim3(blocks(n, 256)), dim3(256), 0, st, iperm, src, n, dst);
 }
 
 void launch_ell_to_csr(const CnlBufs& B, int n, int maxn, const int* row_ptr, int* col, hipStream_t st) {

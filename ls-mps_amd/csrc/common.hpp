@@ -59,6 +59,17 @@ __device__ __forceinline__ float group8_sum(float v) {
   v += __int_as_float(dpp_i<DPP_HALF_MIRROR>(__float_as_int(v)));
   return v;
 }
+__device__ __forceinline__ int group8_sum_int(int v) {
+  v += dpp_i<DPP_QUAD_SWAP1>(v);
+  v += dpp_i<DPP_QUAD_SWAP2>(v);
+  v += dpp_i<DPP_HALF_MIRROR>(v);
+  return v;
+}
+// Bits of `pred` over the aligned 8-lane group whose first lane (within the
+// wavefront) is gbase, bit j for lane gbase + j.
+__device__ __forceinline__ unsigned group8_ballot(bool pred, int gbase) {
+  return (unsigned)(__ballot(pred) >> gbase) & 0xFFu;
+}
 __device__ __forceinline__ unsigned group8_or(unsigned v) {
   v |= (unsigned)dpp_i<DPP_QUAD_SWAP1>((int)v);
   v |= (unsigned)dpp_i<DPP_QUAD_SWAP2>((int)v);
@@ -97,6 +108,45 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
     }
   }
   __syncthreads();
+}
+
+// The GPU hands workgroup b to XCD b % 8, and each XCD has its own L2.
+// Mapping the hardware block index to a logical one so that each XCD gets a
+// contiguous range of logical blocks keeps spatially neighboring work (and
+// the data it gathers) in one L2 instead of spreading it over all eight.
+// A bijection of [0, nblk).
+constexpr unsigned NUM_XCD = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblk) {
+  const unsigned q = nblk / NUM_XCD, r = nblk % NUM_XCD, x = b % NUM_XCD, i = b / NUM_XCD;
+  return x * q + (x < r ? x : r) + i;
+}
+
+// Event counts of a grid (statuses, failed rows, nonzeros). Same-address
+// global atomics from every wavefront serialise across the XCDs, so each
+// block writes its K counts to partials[k * gridDim.x + block] and one
+// small launch (launch_sum_counts) adds them. v holds this thread's counts;
+// lds needs K * (blockDim.x / 64) words.
+template <int K>
+__device__ __forceinline__ void block_count_partials(unsigned (&v)[K], unsigned* __restrict__ partials,
+                                                     unsigned* lds) {
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if (nw == 1) {
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < K; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
+    return;
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[k * nw + wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x < K) {
+    unsigned s = 0;
+    for (int w = 0; w < nw; ++w) s += lds[threadIdx.x * nw + w];
+    partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+  }
 }
 
 // Grid-wide reductions are two launches: every block stores its partial

@@ -24,6 +24,13 @@ struct QuadField {
 // scan.hip
 size_t scan_scratch_elems(int64_t n);
 void launch_exclusive_scan(const int* in, int64_t n, int* out, int64_t* scratch, hipStream_t st);
+// Adds per-block count partials (block_count_partials) into out[slot[k]].
+struct CountSlots {
+  int k;
+  int slot[8];
+};
+void launch_sum_counts(const unsigned* partials, int nblk, CountSlots slots, unsigned long long* out,
+                       hipStream_t st);
 
 // cnl.hip: buffers of the cell list (device pointers).
 struct CnlBufs {
@@ -34,17 +41,23 @@ struct CnlBufs {
   int64_t* scan_scratch = nullptr;
   int *perm = nullptr, *iperm = nullptr, *scell = nullptr;  // sorted slot <-> particle
   double4* spos = nullptr;                                  // sorted {x, y, z, field}
+  float4* sposf = nullptr;                                  // sorted FP32 offsets from (ox, oy, oz)
+  double ox = 0.0, oy = 0.0, oz = 0.0;                      // grid origin
   int *nbr = nullptr, *cnt = nullptr, *cnt_orig = nullptr;  // ELL table
 };
 void launch_cell_coords(const double* x, const double* y, const double* z, int n, double re, int* cx, int* cy,
                         int* cz, int* box, hipStream_t st);
 void launch_fill_int(int* p, int64_t n, int v, hipStream_t st);
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t st);
-void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classify, double zlim, uint8_t* type,
-                     uint8_t* type_s, unsigned long long* trunc_total, int* hist, unsigned long long* tcount,
-                     hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
-void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* type, uint8_t* type_s,
-                     unsigned long long* tcount, hipStream_t st);
+// Neighbor lists and classification work on sorted-order types only; the
+// original-order arrays are produced on demand by the unsort launchers.
+void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classify, double zlim, uint8_t* type_s,
+                     unsigned long long* trunc_total, int* hist, unsigned long long* tcount, hipStream_t st,
+                     hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* type_s, unsigned long long* tcount,
+                     hipStream_t st);
+void launch_unsort_u8(const int* iperm, const uint8_t* src, int n, uint8_t* dst, hipStream_t st);
+void launch_unsort_int(const int* iperm, const int* src, int n, int* dst, hipStream_t st);
 void launch_ell_to_csr(const CnlBufs& B, int n, int maxn, const int* row_ptr, int* col, hipStream_t st);
 void launch_gather_types(const int* perm, const uint8_t* type, int n, uint8_t* type_s, hipStream_t st);
 
@@ -52,31 +65,41 @@ void launch_gather_types(const int* perm, const uint8_t* type, int n, uint8_t* t
 // outputs in sorted order)
 void launch_field_quadratic(double4* spos, int n, const QuadField& q, hipStream_t st);
 void launch_gather_field(const int* perm, const double* v, int n, double4* spos, hipStream_t st);
+// bpart: per-block count partials, 6 * ceil(n / 128) (exact) or
+// 2 * ceil(n / 64) (mixed) words.
 void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status,
-                      unsigned long long* counts, hipStream_t st, hipEvent_t ev_begin = nullptr,
+                      unsigned long long* counts, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin = nullptr,
                       hipEvent_t ev_end = nullptr);
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
-                      int* nredo, unsigned long long* counts, hipStream_t st, hipEvent_t ev_begin = nullptr,
-                      hipEvent_t ev_end = nullptr);
+                      int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,
+                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 void launch_wls_sample_errors(const double4* spos, const uint8_t* type, const double* grad, const double* lap,
                               const uint8_t* status, const int* iperm, int64_t stride, int ns, const QuadField& q,
                               double* ge, double* le, uint8_t* st_out, uint8_t* ty_out, hipStream_t st);
 void launch_wls_unsort(const int* perm, const double* gs, const double* ls, const uint8_t* ss, int n, double* g,
                        double* l, uint8_t* st, hipStream_t stream);
 
-// ppe.hip (rows in unknown order; positions and neighbor rows in sorted order)
-void launch_unknown_flags(const uint8_t* type, int n, int* flag, hipStream_t st);
-void launch_unknown_map(const int* flag, const int* pos, int n, int cap, const int* iperm, int* smap, int* rev,
-                        hipStream_t st);
-void launch_ppe_row_count(const double4* spos, const int* nbr, const int* ncnt, int maxn, const int* smap,
-                          const int* rev, const int* iperm, int m, double re, int* cnt, hipStream_t st);
-void launch_ppe_row_fill(const double4* spos, const uint8_t* type, const int* nbr, const int* ncnt, int maxn,
-                         const int* smap, const int* rev, const int* iperm, int m, double re, double shift, int xmode,
-                         uint32_t xseed, const int* prow, int* pcol, double* pval, double* xe, double* diag,
-                         unsigned long long* counters, hipStream_t st);
-void launch_ppe_rhs(const int* prow, const int* pcol, const double* pval, const double* xe, int m, double* b,
-                    hipStream_t st);
+// ppe.hip. Rows u are the unknowns in sorted-slot order; rr is the
+// reference's row numbering (unknowns in particle index order).
+int ppe_slices(int m);
+void launch_unknown_flags(const uint8_t* type_s, const int* iperm, int n, int* flag, hipStream_t st);
+void launch_unknown_slots(const int* perm, const int* flag, const int* pos, int n, int cap, int* fs, hipStream_t st);
+void launch_unknown_maps(const int* perm, const int* fs, const int* upos, const int* pos, int n, int* uid, int* uslot,
+                         int* urr, int* rru, hipStream_t st);
+void launch_ppe_count(const int* uslot, const int* urr, const int* uid, const int* nbr, const int* ncnt, int maxn,
+                      int m, int* wsz, hipStream_t st);
+void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr, const int* uid,
+                     const int* nbr, const int* ncnt, int maxn, int m, double re, double shift, int xmode,
+                     uint32_t xseed, const int* sptr, int* col, double* val, int* rlen, double* diag, double* xe,
+                     double* b, unsigned long long* counters, unsigned* bpart, hipStream_t st,
+                     hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+void launch_rr_lengths(const int* rru, const int* rlen, int m, int* cnt_rr, hipStream_t st);
+void launch_sell_to_csr(const int* rru, const int* urr, const int* sptr, const int* scol, const double* sval,
+                        const int* rlen, const int* prow, int m, int* pcol, double* pval, hipStream_t st);
+void launch_to_rr(const int* rru, const double* src, int m, double* dst, hipStream_t st);
+void launch_rr_particle(const int* rru, const int* uslot, const int* perm, int m, int* rev, hipStream_t st);
+void launch_from_rr(const int* urr, const double* src, int m, double* dst, hipStream_t st);
 
 }  // namespace lsmps_hip

@@ -1,38 +1,78 @@
-// CSR pressure-Poisson (PPE) assembly on gfx950.
+// Pressure-Poisson (PPE) assembly on gfx950.
 //
 // Semantics of the reference's assemble_ppe (src/lsmps.cpp:29-64):
 // unknowns are the first `cap` particles, in index order, of type Interior
-// or NearWall; row 0 is the identity pin; every other row holds, in
-// neighbor-list order, -a_ij = -w/(r*r) for each unknown neighbor with
-// w > 0, then the diagonal sum(a_ij) last (1 with a failed-row count when
-// it is not positive and finite); b = A x* summed in row order. Optional
-// extension: the diagonal is scaled by (1 + shift) after the check
-// (shift = 0 is the reference).
+// or NearWall (row rr = their rank); row 0 is the identity pin; every other
+// row holds, in neighbor-list order, -a_ij = -w/(r*r) for each unknown
+// neighbor with w > 0, then the diagonal sum(a_ij) last (1 with a
+// failed-row count when it is not positive and finite); b = A x* summed
+// left to right in that order. Optional extension: the diagonal is scaled
+// by (1 + shift) after the check (shift = 0 is the reference).
 //
-// Two passes (count, scan, fill), one thread per row, each row's
-// diagonal summed sequentially in list order with no contraction (this file
-// is built with -ffp-contract=off), so values, b and structure are
-// bit-identical to the reference's.
+// Layout. The matrix is written once, straight into the sliced-ELL form
+// the solver streams: rows are the unknowns in cell-sorted order (row u =
+// rank of the unknown among sorted slots), slices of 64 rows stored
+// column-major and padded with (u, 0). Neighbor lists in sorted slots are
+// ascending (the reference visits cells z, y, x-major and members by
+// index, which is exactly the slot order), so each row's columns come out
+// ascending with no sort; the diagonal is placed at its rank among them.
+// The reference-order CSR (rows rr, diagonal last) is only materialised
+// when a caller downloads it.
+//
+// Kernels: unknown flags over particle index, a scan (reference numbering
+// rr), flags over sorted slots, a scan (sorted numbering u), the maps
+// between them; then a count pass (unknown neighbors per row -> slice
+// widths) and the fill pass. Count and fill run one 512-thread block per
+// slice, eight lanes per row: the lanes walk the row's ELL neighbor list
+// eight entries at a time, and ordered ballots give each kept entry its
+// position. The diagonal and b are sums in list order, accumulated by
+// broadcasting each kept entry to the row's lanes in order, so they are
+// bit-identical to the reference (this file is built with
+// -ffp-contract=off).
 #include "kernels.hpp"
 
 namespace lsmps_hip {
 
-__global__ __launch_bounds__(256) void k_unknown_flags(const uint8_t* __restrict__ type, int n, int* __restrict__ flag) {
+constexpr int PPE_SLICE = 64;
+constexpr int PPE_GROUP = 8;
+constexpr int PPE_THREADS = PPE_SLICE * PPE_GROUP;  // one slice per block
+
+__device__ __forceinline__ bool is_unknown_type(uint8_t t) { return t == T_INTERIOR || t == T_NEARWALL; }
+
+// flag[i] over particle index i, from the sorted-order types.
+__global__ __launch_bounds__(256) void k_unknown_flags(const uint8_t* __restrict__ type_s,
+                                                       const int* __restrict__ iperm, int n, int* __restrict__ flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) flag[i] = (type[i] == T_INTERIOR || type[i] == T_NEARWALL) ? 1 : 0;
+  if (i < n) flag[i] = is_unknown_type(type_s[iperm[i]]) ? 1 : 0;
 }
 
-// Unknown numbering in particle index order; smap is the same map indexed
-// by sorted slot (what the row kernels look up for each neighbor).
-__global__ __launch_bounds__(256) void k_unknown_map(const int* __restrict__ flag, const int* __restrict__ pos, int n,
-                                                     int cap, const int* __restrict__ iperm, int* __restrict__ smap,
-                                                     int* __restrict__ rev) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int p = pos[i];
-  const bool u = flag[i] && p < cap;
-  smap[iperm[i]] = u ? p : -1;
-  if (u) rev[p] = i;
+// fs[s]: slot s holds an unknown (flagged and among the first cap).
+__global__ __launch_bounds__(256) void k_unknown_slots(const int* __restrict__ perm, const int* __restrict__ flag,
+                                                       const int* __restrict__ pos, int n, int cap,
+                                                       int* __restrict__ fs) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  fs[s] = flag[i] && pos[i] < cap ? 1 : 0;
+}
+
+// uid[s] = row u of slot s or -1; uslot[u] = s; urr[u] = reference row;
+// rru[rr] = u.
+__global__ __launch_bounds__(256) void k_unknown_maps(const int* __restrict__ perm, const int* __restrict__ fs,
+                                                      const int* __restrict__ upos, const int* __restrict__ pos, int n,
+                                                      int* __restrict__ uid, int* __restrict__ uslot,
+                                                      int* __restrict__ urr, int* __restrict__ rru) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  if (!fs[s]) {
+    uid[s] = -1;
+    return;
+  }
+  const int u = upos[s], rr = pos[perm[s]];
+  uid[s] = u;
+  uslot[u] = s;
+  urr[u] = rr;
+  rru[rr] = u;
 }
 
 // Weight test of one pair, as the reference computes it (src/lsmps.cpp:22,45).
@@ -46,36 +86,10 @@ __device__ __forceinline__ bool pair_coef(double dx, double dy, double dz, doubl
   return true;
 }
 
-// Positions (x, y, z), neighbor rows and smap are in sorted-slot order.
-__global__ __launch_bounds__(256) void k_ppe_row_count(const double4* __restrict__ spos, const int* __restrict__ nbr,
-                                                       const int* __restrict__ ncnt, int maxn,
-                                                       const int* __restrict__ smap, const int* __restrict__ rev,
-                                                       const int* __restrict__ iperm, int m, double re,
-                                                       int* __restrict__ cnt) {
-  const int rr = blockIdx.x * blockDim.x + threadIdx.x;
-  if (rr >= m) return;
-  if (rr == 0) {
-    cnt[0] = 1;
-    return;
-  }
-  const int s = iperm[rev[rr]];
-  const double4 pi = spos[s];
-  const int* row = nbr + (size_t)s * maxn;
-  const int k = ncnt[s];
-  int c = 1;
-  for (int q = 0; q < k; ++q) {
-    const int s2 = row[q];
-    if (smap[s2] < 0) continue;
-    const double4 pj = spos[s2];
-    double a;
-    if (pair_coef(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, re, &a)) ++c;
-  }
-  cnt[rr] = c;
-}
-
 // Seeded manufactured solution x*_k ~ U(-1, 1): the top 53 bits of the
 // splitmix64 finaliser of seed * 2^64/phi + k, scaled to [0, 1) and mapped
-// to [-1, 1). Counter-based, so every row draws its value independently.
+// to [-1, 1). Counter-based on the reference row k, so every row draws its
+// value independently of the layout.
 __device__ __forceinline__ double xstar_random(uint32_t seed, int k) {
   uint64_t z = (uint64_t)seed * 0x9E3779B97F4A7C15ull + (uint64_t)k;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -84,101 +98,258 @@ __device__ __forceinline__ double xstar_random(uint32_t seed, int k) {
   return -1.0 + 2.0 * ((double)(z >> 11) * 0x1.0p-53);
 }
 
-// counters: [0] failed rows, [1] NearWall rows except row 0.
-__global__ __launch_bounds__(256) void k_ppe_row_fill(const double4* __restrict__ spos, const uint8_t* __restrict__ type,
-                                                      const int* __restrict__ nbr, const int* __restrict__ ncnt,
-                                                      int maxn, const int* __restrict__ smap,
-                                                      const int* __restrict__ rev, const int* __restrict__ iperm, int m,
-                                                      double re, double shift, int xmode, uint32_t xseed,
-                                                      const int* __restrict__ prow,
-                                                      int* __restrict__ pcol, double* __restrict__ pval,
-                                                      double* __restrict__ xe, double* __restrict__ diag_out,
-                                                      unsigned long long* __restrict__ counters) {
-  const int rr = blockIdx.x * blockDim.x + threadIdx.x;
-  int failed = 0, neumann = 0;
-  if (rr < m) {
-    const int i = rev[rr];
-    const int s = iperm[i];
+// Slice-local row of this thread's group and the slice index.
+struct PpeRow {
+  int slice, u, g, gbase;
+  bool live;
+};
+__device__ __forceinline__ PpeRow ppe_row(int m) {
+  PpeRow R;
+  R.slice = (int)xcd_block(blockIdx.x, gridDim.x);
+  R.g = threadIdx.x & (PPE_GROUP - 1);
+  R.gbase = threadIdx.x & ~(PPE_GROUP - 1) & (WAVE - 1);
+  R.u = R.slice * PPE_SLICE + (int)(threadIdx.x / PPE_GROUP);
+  R.live = R.u < m;
+  return R;
+}
+
+// Upper bound of each row's length (unknown neighbors + diagonal; exact
+// except for coincident or boundary-rounded pairs, which become padding)
+// and the slice widths wsz[slice] = 64 * longest row.
+__global__ __launch_bounds__(PPE_THREADS) void k_ppe_count(const int* __restrict__ uslot, const int* __restrict__ urr,
+                                                           const int* __restrict__ uid, const int* __restrict__ nbr,
+                                                           const int* __restrict__ ncnt, int maxn, int m,
+                                                           int* __restrict__ wsz) {
+  __shared__ int wmax;
+  if (threadIdx.x == 0) wmax = 0;
+  __syncthreads();
+  const PpeRow R = ppe_row(m);
+  int c = 0;
+  if (R.live && urr[R.u] != 0) {
+    const int s = uslot[R.u];
+    const int* row = nbr + (size_t)s * maxn;
+    const int k = ncnt[s];
+    for (int q = R.g; q < k; q += PPE_GROUP) c += uid[row[q]] >= 0 ? 1 : 0;
+  }
+  c = group8_sum_int(c);
+  if (R.live && R.g == 0) atomicMax(&wmax, c + 1);
+  __syncthreads();
+  if (threadIdx.x == 0) wsz[R.slice] = wmax * PPE_SLICE;
+}
+
+// The fill pass. Outputs in row order u: the sliced-ELL columns (sorted
+// numbering) and FP64 values, the row lengths, the diagonal, x* and b.
+// Counts per block into bpart (then counters): [0] failed rows, [1] NearWall
+// rows except row 0, [2] nonzeros.
+__global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
+    const double4* __restrict__ spos, const uint8_t* __restrict__ type_s, const int* __restrict__ uslot,
+    const int* __restrict__ urr, const int* __restrict__ uid, const int* __restrict__ nbr,
+    const int* __restrict__ ncnt, int maxn, int m, double re, double shift, int xmode, uint32_t xseed,
+    const int* __restrict__ sptr, int* __restrict__ col, double* __restrict__ val, int* __restrict__ rlen,
+    double* __restrict__ diag, double* __restrict__ xe, double* __restrict__ b, unsigned* __restrict__ bpart) {
+  __shared__ unsigned lds[3 * PPE_THREADS / WAVE];
+  const PpeRow R = ppe_row(m);
+  const unsigned below = (1u << R.g) - 1u;
+  int failed = 0, neumann = 0, len = 0;
+  if (R.live) {
+    const int u = R.u;
+    const int s = uslot[u];
+    const int rr = urr[u];
     const double4 pi = spos[s];
-    xe[rr] = xmode == 0 ? pi.x * pi.x + pi.y * pi.y  // x* = x^2 + y^2 (src/lsmps.cpp:29)
-                        : xstar_random(xseed, rr);
-    int at = prow[rr];
-    if (rr == 0) {
-      pcol[at] = 0;
-      pval[at] = 1.0;
-      diag_out[0] = 1.0;
-    } else {
-      double d = 0.0;
+    const int sb = sptr[R.slice];
+    const int width = (sptr[R.slice + 1] - sb) / PPE_SLICE;
+    const int base = sb + (u - R.slice * PPE_SLICE);
+    const double xu = xmode == 0 ? pi.x * pi.x + pi.y * pi.y  // x* = x^2 + y^2 (src/lsmps.cpp:29)
+                                 : xstar_random(xseed, rr);
+    double d = 0.0, bs = 0.0;
+    int kept = 0, below_u = 0;
+    if (rr != 0) {
       const int* row = nbr + (size_t)s * maxn;
-      const int kn = ncnt[s];
-      for (int q = 0; q < kn; ++q) {
-        const int s2 = row[q];
-        const int cj = smap[s2];
-        if (cj < 0) continue;
-        const double4 pj = spos[s2];
-        double a;
-        if (!pair_coef(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, re, &a)) continue;
-        d += a;
-        pcol[at] = cj;
-        pval[at] = -a;
-        ++at;
+      const int k = ncnt[s];
+      for (int c0 = 0; c0 < k; c0 += PPE_GROUP) {
+        const int q = c0 + R.g;
+        int cj = -1;
+        double a = 0.0, xj = 0.0;
+        bool keep = false;
+        if (q < k) {
+          const int s2 = row[q];
+          cj = uid[s2];
+          if (cj >= 0) {
+            const double4 pj = spos[s2];
+            keep = pair_coef(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, re, &a);
+            if (keep) xj = xmode == 0 ? pj.x * pj.x + pj.y * pj.y : xstar_random(xseed, urr[cj]);
+          }
+        }
+        const unsigned gm = group8_ballot(keep, R.gbase);
+        const unsigned lt = group8_ballot(keep && cj < u, R.gbase);
+        if (keep) {
+          // Entries with a column above u sit one slot later: the diagonal
+          // takes slot below_u + popc(lt).
+          const int slot = kept + __popc(gm & below) + (cj > u ? 1 : 0);
+          col[base + slot * PPE_SLICE] = cj;
+          val[base + slot * PPE_SLICE] = -a;
+        }
+        // Sums in list order: each kept entry is broadcast to the group in turn.
+        for (int j = 0; j < PPE_GROUP; ++j) {
+          if ((gm >> j) & 1u) {
+            const double aj = __shfl(a, R.gbase + j);
+            const double xjj = __shfl(xj, R.gbase + j);
+            d += aj;
+            bs += -aj * xjj;
+          }
+        }
+        kept += __popc(gm);
+        below_u += __popc(lt);
       }
       if (d <= 0 || !isfinite(d)) {
         failed = 1;
         d = 1.0;
       }
       if (shift != 0.0) d += shift * d;
-      pcol[at] = rr;
-      pval[at] = d;
-      diag_out[rr] = d;
-      neumann = type[i] == T_NEARWALL ? 1 : 0;
+      neumann = type_s[s] == T_NEARWALL ? 1 : 0;
+    } else {
+      d = 1.0;  // the identity pin of row 0
+    }
+    bs += d * xu;
+    len = kept + 1;
+    if (R.g == 0) {
+      col[base + below_u * PPE_SLICE] = u;
+      val[base + below_u * PPE_SLICE] = d;
+      rlen[u] = len;
+      diag[u] = d;
+      xe[u] = xu;
+      b[u] = bs;
+    }
+    for (int k2 = len + R.g; k2 < width; k2 += PPE_GROUP) {
+      col[base + k2 * PPE_SLICE] = u;
+      val[base + k2 * PPE_SLICE] = 0.0;
+    }
+  } else {
+    // Rows past m in the last slice: every stored entry is defined, as
+    // (0, 0), so passes over all stored entries stay in bounds.
+    const int sb = sptr[R.slice];
+    const int width = (sptr[R.slice + 1] - sb) / PPE_SLICE;
+    const int base = sb + (R.u - R.slice * PPE_SLICE);
+    for (int k2 = R.g; k2 < width; k2 += PPE_GROUP) {
+      col[base + k2 * PPE_SLICE] = 0;
+      val[base + k2 * PPE_SLICE] = 0.0;
     }
   }
-  const unsigned long long fs = wave_sum((unsigned long long)failed), ns = wave_sum((unsigned long long)neumann);
-  if ((threadIdx.x & (WAVE - 1)) == 0) {
-    if (fs) atomicAdd(&counters[0], fs);
-    if (ns) atomicAdd(&counters[1], ns);
-  }
+  const bool leader = R.live && R.g == 0;
+  unsigned v[3] = {leader ? (unsigned)failed : 0u, leader ? (unsigned)neumann : 0u, leader ? (unsigned)len : 0u};
+  block_count_partials<3>(v, bpart, lds);
 }
 
-// b = A x* with each row summed left to right (src/lsmps.cpp:59-61).
-__global__ __launch_bounds__(256) void k_ppe_rhs(const int* __restrict__ prow, const int* __restrict__ pcol,
-                                                 const double* __restrict__ pval, const double* __restrict__ xe, int m,
-                                                 double* __restrict__ b) {
+// ---- reference-order views, for downloads and uploads -----------------
+
+// cnt_rr[rr] = rlen[rru[rr]] (CSR row lengths in reference order).
+__global__ __launch_bounds__(256) void k_rr_lengths(const int* __restrict__ rru, const int* __restrict__ rlen, int m,
+                                                    int* __restrict__ cnt_rr) {
+  const int rr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rr < m) cnt_rr[rr] = rlen[rru[rr]];
+}
+
+// CSR row rr: the off-diagonal entries of sliced-ELL row u = rru[rr] in
+// order with columns renumbered to reference rows, then the diagonal.
+__global__ __launch_bounds__(256) void k_sell_to_csr(const int* __restrict__ rru, const int* __restrict__ urr,
+                                                     const int* __restrict__ sptr, const int* __restrict__ scol,
+                                                     const double* __restrict__ sval, const int* __restrict__ rlen,
+                                                     const int* __restrict__ prow, int m, int* __restrict__ pcol,
+                                                     double* __restrict__ pval) {
   const int rr = blockIdx.x * blockDim.x + threadIdx.x;
   if (rr >= m) return;
-  double s = 0.0;
-  for (int k = prow[rr]; k < prow[rr + 1]; ++k) s += pval[k] * xe[pcol[k]];
-  b[rr] = s;
+  const int u = rru[rr];
+  const int sl = u / PPE_SLICE;
+  const int base = sptr[sl] + (u - sl * PPE_SLICE);
+  const int len = rlen[u];
+  int at = prow[rr];
+  double dv = 0.0;
+  for (int k = 0; k < len; ++k) {
+    const int c = scol[base + k * PPE_SLICE];
+    const double v = sval[base + k * PPE_SLICE];
+    if (c == u) {
+      dv = v;
+      continue;
+    }
+    pcol[at] = urr[c];
+    pval[at] = v;
+    ++at;
+  }
+  pcol[at] = rr;
+  pval[at] = dv;
+}
+
+// dst[rr] = src[rru[rr]] (row-order vector to reference order) and the
+// particle index of each reference row.
+__global__ __launch_bounds__(256) void k_to_rr(const int* __restrict__ rru, const double* __restrict__ src, int m,
+                                               double* __restrict__ dst) {
+  const int rr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rr < m) dst[rr] = src[rru[rr]];
+}
+__global__ __launch_bounds__(256) void k_rr_particle(const int* __restrict__ rru, const int* __restrict__ uslot,
+                                                     const int* __restrict__ perm, int m, int* __restrict__ rev) {
+  const int rr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rr < m) rev[rr] = perm[uslot[rru[rr]]];
+}
+// dst[u] = src[urr[u]] (reference order to row order).
+__global__ __launch_bounds__(256) void k_from_rr(const int* __restrict__ urr, const double* __restrict__ src, int m,
+                                                 double* __restrict__ dst) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < m) dst[u] = src[urr[u]];
 }
 
 static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
-void launch_unknown_flags(const uint8_t* type, int n, int* flag, hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_unknown_flags, dim3(blocks(n, 256)), dim3(256), 0, st, type, n, flag);
+int ppe_slices(int m) { return (m + PPE_SLICE - 1) / PPE_SLICE; }
+
+void launch_unknown_flags(const uint8_t* type_s, const int* iperm, int n, int* flag, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_unknown_flags, dim3(blocks(n, 256)), dim3(256), 0, st, type_s, iperm, n, flag);
 }
-void launch_unknown_map(const int* flag, const int* pos, int n, int cap, const int* iperm, int* smap, int* rev,
-                        hipStream_t st) {
+void launch_unknown_slots(const int* perm, const int* flag, const int* pos, int n, int cap, int* fs, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_unknown_slots, dim3(blocks(n, 256)), dim3(256), 0, st, perm, flag, pos, n, cap, fs);
+}
+void launch_unknown_maps(const int* perm, const int* fs, const int* upos, const int* pos, int n, int* uid, int* uslot,
+                         int* urr, int* rru, hipStream_t st) {
   if (n > 0)
-    hipLaunchKernelGGL(k_unknown_map, dim3(blocks(n, 256)), dim3(256), 0, st, flag, pos, n, cap, iperm, smap, rev);
+    hipLaunchKernelGGL(k_unknown_maps, dim3(blocks(n, 256)), dim3(256), 0, st, perm, fs, upos, pos, n, uid, uslot, urr,
+                       rru);
 }
-void launch_ppe_row_count(const double4* spos, const int* nbr, const int* ncnt, int maxn, const int* smap,
-                          const int* rev, const int* iperm, int m, double re, int* cnt, hipStream_t st) {
+void launch_ppe_count(const int* uslot, const int* urr, const int* uid, const int* nbr, const int* ncnt, int maxn,
+                      int m, int* wsz, hipStream_t st) {
   if (m > 0)
-    hipLaunchKernelGGL(k_ppe_row_count, dim3(blocks(m, 256)), dim3(256), 0, st, spos, nbr, ncnt, maxn, smap, rev, iperm,
-                       m, re, cnt);
+    hipLaunchKernelGGL(k_ppe_count, dim3(ppe_slices(m)), dim3(PPE_THREADS), 0, st, uslot, urr, uid, nbr, ncnt, maxn, m,
+                       wsz);
 }
-void launch_ppe_row_fill(const double4* spos, const uint8_t* type, const int* nbr, const int* ncnt, int maxn,
-                         const int* smap, const int* rev, const int* iperm, int m, double re, double shift, int xmode,
-                         uint32_t xseed, const int* prow, int* pcol, double* pval, double* xe, double* diag,
-                         unsigned long long* counters, hipStream_t st) {
+void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr, const int* uid,
+                     const int* nbr, const int* ncnt, int maxn, int m, double re, double shift, int xmode,
+                     uint32_t xseed, const int* sptr, int* col, double* val, int* rlen, double* diag, double* xe,
+                     double* b, unsigned long long* counters, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin,
+                     hipEvent_t ev_end) {
+  if (m <= 0) return;
+  const int g = ppe_slices(m);
+  if (ev_begin) (void)hipEventRecord(ev_begin, st);
+  hipLaunchKernelGGL(k_ppe_fill, dim3(g), dim3(PPE_THREADS), 0, st, spos, type_s, uslot, urr, uid, nbr, ncnt, maxn, m,
+                     re, shift, xmode, xseed, sptr, col, val, rlen, diag, xe, b, bpart);
+  if (ev_end) (void)hipEventRecord(ev_end, st);
+  launch_sum_counts(bpart, g, CountSlots{3, {0, 1, 2, 0, 0, 0, 0, 0}}, counters, st);
+}
+void launch_rr_lengths(const int* rru, const int* rlen, int m, int* cnt_rr, hipStream_t st) {
+  if (m > 0) hipLaunchKernelGGL(k_rr_lengths, dim3(blocks(m, 256)), dim3(256), 0, st, rru, rlen, m, cnt_rr);
+}
+void launch_sell_to_csr(const int* rru, const int* urr, const int* sptr, const int* scol, const double* sval,
+                        const int* rlen, const int* prow, int m, int* pcol, double* pval, hipStream_t st) {
   if (m > 0)
-    hipLaunchKernelGGL(k_ppe_row_fill, dim3(blocks(m, 256)), dim3(256), 0, st, spos, type, nbr, ncnt, maxn, smap, rev,
-                       iperm, m, re, shift, xmode, xseed, prow, pcol, pval, xe, diag, counters);
+    hipLaunchKernelGGL(k_sell_to_csr, dim3(blocks(m, 256)), dim3(256), 0, st, rru, urr, sptr, scol, sval, rlen, prow,
+                       m, pcol, pval);
 }
-void launch_ppe_rhs(const int* prow, const int* pcol, const double* pval, const double* xe, int m, double* b,
-                    hipStream_t st) {
-  if (m > 0) hipLaunchKernelGGL(k_ppe_rhs, dim3(blocks(m, 256)), dim3(256), 0, st, prow, pcol, pval, xe, m, b);
+void launch_to_rr(const int* rru, const double* src, int m, double* dst, hipStream_t st) {
+  if (m > 0) hipLaunchKernelGGL(k_to_rr, dim3(blocks(m, 256)), dim3(256), 0, st, rru, src, m, dst);
+}
+void launch_rr_particle(const int* rru, const int* uslot, const int* perm, int m, int* rev, hipStream_t st) {
+  if (m > 0) hipLaunchKernelGGL(k_rr_particle, dim3(blocks(m, 256)), dim3(256), 0, st, rru, uslot, perm, m, rev);
+}
+void launch_from_rr(const int* urr, const double* src, int m, double* dst, hipStream_t st) {
+  if (m > 0) hipLaunchKernelGGL(k_from_rr, dim3(blocks(m, 256)), dim3(256), 0, st, urr, src, m, dst);
 }
 
 }  // namespace lsmps_hip

@@ -104,4 +104,38 @@ void launch_exclusive_scan(const int* in, int64_t n, int* out, int64_t* scratch,
   if (ntiles > 0) hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(SCAN_THREADS), 0, st, in, n, scratch, out);
 }
 
+// out[slot[k]] += sum over b of partials[k * nblk + b]: block k adds row k
+// with four independent loads in flight per thread (a row is a few hundred
+// KB at most, so the time is load latency, not bandwidth).
+constexpr int SUM_THREADS = 1024;
+__global__ __launch_bounds__(SUM_THREADS) void k_sum_counts(const unsigned* __restrict__ partials, int nblk,
+                                                            CountSlots slots, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long lds[SUM_THREADS / WAVE];
+  const int k = blockIdx.x;
+  const unsigned* row = partials + (size_t)k * nblk;
+  unsigned long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int b = threadIdx.x;
+  for (; b + 3 * SUM_THREADS < nblk; b += 4 * SUM_THREADS) {
+    s0 += row[b];
+    s1 += row[b + SUM_THREADS];
+    s2 += row[b + 2 * SUM_THREADS];
+    s3 += row[b + 3 * SUM_THREADS];
+  }
+  for (; b < nblk; b += SUM_THREADS) s0 += row[b];
+  const unsigned long long s = wave_sum((s0 + s1) + (s2 + s3));
+  if ((threadIdx.x & (WAVE - 1)) == 0) lds[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < SUM_THREADS / WAVE; ++w) t += lds[w];
+    out[slots.slot[k]] += t;
+  }
+}
+
+void launch_sum_counts(const unsigned* partials, int nblk, CountSlots slots, unsigned long long* out,
+                       hipStream_t st) {
+  if (nblk > 0 && slots.k > 0)
+    hipLaunchKernelGGL(k_sum_counts, dim3(slots.k), dim3(SUM_THREADS), 0, st, partials, nblk, slots, out);
+}
+
 }  // namespace lsmps_hip

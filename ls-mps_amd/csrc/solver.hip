@@ -9,7 +9,8 @@
 //
 // The system solved is A S y = b, x = S y, where S is a diagonal scaling:
 // identity (the reference), 1/diag(A) (right Jacobi) or a learned diagonal.
-// The scaled matrix is built once per solve in the solve precision T.
+// The scaled matrix is made once per solve in the solve precision T by one
+// streaming pass over the assembled FP64 matrix.
 //
 // The iteration matrix is stored sliced-ELL: rows in slices of 64 (one
 // wavefront), each slice padded to its longest row and stored column-major,
@@ -399,61 +400,15 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_final(const double* __re
   }
 }
 
-// ---- sliced-ELL construction from the assembled CSR -------------------
-__global__ __launch_bounds__(256) void k_sell_widths(const int* __restrict__ prow, int m, int nslices,
-                                                     int* __restrict__ wsz) {
-  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sl >= nslices) return;
-  int w = 0;
-  for (int l = 0; l < SLICE; ++l) {
-    const int row = sl * SLICE + l;
-    if (row < m) w = max(w, prow[row + 1] - prow[row]);
-  }
-  wsz[sl] = w * SLICE;
-}
-
-// Copies the CSR rows into their slices in ascending column order, scaled
-// by S at the columns (values in T), with the unscaled FP64 values when
-// val64 is given, and pads with (row, 0). One wavefront per row: the row's
-// columns are staged in LDS and each entry's slot is its rank (columns in a
-// row are distinct).
-constexpr int FILL_WAVES = 4;
+// ---- iteration matrix from the assembled sliced-ELL -------------------
+// val[e] = A[e] * S[col[e]] in precision T, over every stored entry
+// (padding entries hold 0 and stay 0).
 template <typename T>
-__global__ __launch_bounds__(FILL_WAVES * WAVE) void k_sell_fill(const int* __restrict__ prow,
-                                                                  const int* __restrict__ pcol,
-                                                                  const double* __restrict__ pval,
-                                                                  const double* __restrict__ scale, int m, int maxlen,
-                                                                  const int* __restrict__ sptr, int* __restrict__ col,
-                                                                  T* __restrict__ val, double* __restrict__ val64) {
-  extern __shared__ int sh[];  // FILL_WAVES * maxlen columns
-  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  int* rc = sh + w * maxlen;
-  // Block-uniform trip count, so the barriers below are reached by all.
-  for (int base = blockIdx.x * FILL_WAVES; base < m; base += gridDim.x * FILL_WAVES) {
-    const int row = base + w;
-    const bool live = row < m;
-    const SellRow R = live ? sell_row(sptr, row) : SellRow{0, 0};
-    const int b = live ? prow[row] : 0, len = live ? prow[row + 1] - b : 0;
-    for (int q = lane; q < len; q += WAVE) rc[q] = pcol[b + q];
-    __syncthreads();
-    for (int q = lane; q < len; q += WAVE) {
-      const int c = rc[q];
-      int rank = 0;
-      for (int u = 0; u < len; ++u) rank += rc[u] < c ? 1 : 0;
-      const int e = R.base + rank * SLICE;
-      const double a = pval[b + q];
-      col[e] = c;
-      val[e] = (T)(scale ? a * scale[c] : a);
-      if (val64) val64[e] = a;
-    }
-    for (int k = len + lane; k < R.width; k += WAVE) {
-      const int e = R.base + k * SLICE;
-      col[e] = row;
-      val[e] = (T)0;
-      if (val64) val64[e] = 0.0;
-    }
-    __syncthreads();
-  }
+__global__ __launch_bounds__(256) void k_sell_scale(const int* __restrict__ col, const double* __restrict__ val64,
+                                                    const double* __restrict__ scale, int64_t nnz,
+                                                    T* __restrict__ val) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
+    val[e] = (T)(scale ? val64[e] * scale[col[e]] : val64[e]);
 }
 
 __global__ __launch_bounds__(256) void k_inv_diag(const double* __restrict__ diag, int m, double* __restrict__ s) {
@@ -486,27 +441,16 @@ unsigned solver_grid(int m) {
   return g < 1 ? 1 : (g > SOLVER_MAX_GRID ? SOLVER_MAX_GRID : g);
 }
 
-int sell_slices(int m) { return (m + SLICE - 1) / SLICE; }
-
-void launch_sell_widths(const int* prow, int m, int* wsz, hipStream_t st) {
-  const int ns = sell_slices(m);
-  if (ns > 0) hipLaunchKernelGGL(k_sell_widths, dim3(blocks(ns, 256)), dim3(256), 0, st, prow, m, ns, wsz);
-}
-
 template <typename T>
-void launch_sell_fill(const int* prow, const int* pcol, const double* pval, const double* scale, int m, int maxlen,
-                      const int* sptr, int* col, T* val, double* val64, hipStream_t st) {
-  if (m <= 0) return;
-  unsigned g = blocks(m, FILL_WAVES);
-  g = g > 8192u ? 8192u : g;
-  const size_t lds = (size_t)FILL_WAVES * (size_t)(maxlen > 0 ? maxlen : 1) * sizeof(int);
-  hipLaunchKernelGGL(k_sell_fill<T>, dim3(g), dim3(FILL_WAVES * WAVE), lds, st, prow, pcol, pval, scale, m, maxlen,
-                     sptr, col, val, val64);
+void launch_sell_scale(const int* col, const double* val64, const double* scale, int64_t nnz, T* val,
+                       hipStream_t st) {
+  if (nnz <= 0) return;
+  const int64_t want = (nnz + 255) / 256;
+  const unsigned g = (unsigned)(want < 8192 ? want : 8192);
+  hipLaunchKernelGGL(k_sell_scale<T>, dim3(g), dim3(256), 0, st, col, val64, scale, nnz, val);
 }
-template void launch_sell_fill<float>(const int*, const int*, const double*, const double*, int, int, const int*,
-                                      int*, float*, double*, hipStream_t);
-template void launch_sell_fill<double>(const int*, const int*, const double*, const double*, int, int, const int*,
-                                       int*, double*, double*, hipStream_t);
+template void launch_sell_scale<float>(const int*, const double*, const double*, int64_t, float*, hipStream_t);
+template void launch_sell_scale<double>(const int*, const double*, const double*, int64_t, double*, hipStream_t);
 
 void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st) {
   if (m > 0) hipLaunchKernelGGL(k_inv_diag, dim3(blocks(m, 256)), dim3(256), 0, st, diag, m, s);

@@ -55,13 +55,10 @@ struct SolverBufs {
 };
 
 unsigned solver_grid(int m);
-int sell_slices(int m);
-// wsz[s] = 64 * (longest row of slice s); scan it into slice offsets.
-void launch_sell_widths(const int* prow, int m, int* wsz, hipStream_t st);
-// maxlen: an upper bound of the row lengths (sizes the LDS staging).
+// val = A S over all nnz stored entries (padding included), in precision T.
 template <typename T>
-void launch_sell_fill(const int* prow, const int* pcol, const double* pval, const double* scale, int m, int maxlen,
-                      const int* sptr, int* col, T* val, double* val64, hipStream_t st);
+void launch_sell_scale(const int* col, const double* val64, const double* scale, int64_t nnz, T* val,
+                       hipStream_t st);
 void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st);
 template <typename T>
 void launch_solver_init(const SolverBufs<T>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,

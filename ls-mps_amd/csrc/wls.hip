@@ -142,13 +142,14 @@ __device__ __noinline__ uint8_t wls_exact_one(int i, const double4* __restrict__
 
 // Block-aggregated status histogram (statuses 0..4 and the redo slot);
 // every thread of the block must call it, with slot < 0 for "nothing".
-__device__ __forceinline__ void count_status(int slot, unsigned long long* __restrict__ counts) {
+// Block counts of each status slot into bpart[slot * gridDim.x + block].
+__device__ __forceinline__ void count_status(int slot, unsigned* __restrict__ bpart) {
   __shared__ unsigned c[WLS_REDO_SLOT + 1];
   if (threadIdx.x <= WLS_REDO_SLOT) c[threadIdx.x] = 0u;
   __syncthreads();
   if (slot >= 0) atomicAdd(&c[slot], 1u);
   __syncthreads();
-  if (threadIdx.x <= WLS_REDO_SLOT && c[threadIdx.x]) atomicAdd(&counts[threadIdx.x], (unsigned long long)c[threadIdx.x]);
+  if (threadIdx.x <= WLS_REDO_SLOT) bpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = c[threadIdx.x];
 }
 
 __device__ __forceinline__ void store_result(int i, uint8_t st, const double* g3, double lap, double* __restrict__ grad,
@@ -164,9 +165,8 @@ __global__ __launch_bounds__(128) void k_wls_exact(const double4* __restrict__ s
                                                    const uint8_t* __restrict__ type, const int* __restrict__ nbr,
                                                    const int* __restrict__ cnt, int maxn, int n, double re, double l0,
                                                    double reg, double* __restrict__ grad, double* __restrict__ lap,
-                                                   uint8_t* __restrict__ status,
-                                                   unsigned long long* __restrict__ counts) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   uint8_t* __restrict__ status, unsigned* __restrict__ bpart) {
+  const int i = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   int slot = -1;
   if (i < n) {
     const uint8_t t = type[i];
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(128) void k_wls_exact(const double4* __restrict__ s
     store_result(i, st, g3, L, grad, lap, status);
     slot = st;
   }
-  count_status(slot, counts);
+  count_status(slot, bpart);
 }
 
 // Redo in FP64 the particles the FP32 kernel could not trust.
@@ -302,14 +302,14 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
                                                     const int* __restrict__ cnt, int maxn, int n, double re, double l0,
                                                     double* __restrict__ grad, double* __restrict__ lap,
                                                     uint8_t* __restrict__ status, int* __restrict__ redo,
-                                                    int* __restrict__ nredo, unsigned long long* __restrict__ counts) {
+                                                    int* __restrict__ nredo, unsigned* __restrict__ bpart) {
   __shared__ float sums[WLS_HALF * WLS_STRIDE];
   const int lane = threadIdx.x;
   const double inv_re = 1.0 / re;
   const float l0s = (float)(l0 * inv_re);
   const float inv_h2 = 1.0f / (2.0f * l0s), inv_g = 1.0f / l0s;
   const float rmin = (float)(1e-12 * inv_re), rmin2 = rmin * rmin;
-  const int s0 = blockIdx.x * WAVE;
+  const int s0 = (int)xcd_block(blockIdx.x, gridDim.x) * WAVE;
   int n_ok = 0, n_skip = 0;
   for (int half = 0; half < WAVE / WLS_HALF; ++half) {
     // phase 1: 8 lanes per particle, 8 particles per pass, 4 passes.
@@ -361,12 +361,9 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
     }
     __syncthreads();
   }
-  // Per-wave status counts (rows sent to the FP64 list are counted there).
-  unsigned long long c_ok = wave_sum((unsigned long long)n_ok), c_skip = wave_sum((unsigned long long)n_skip);
-  if (lane == 0) {
-    if (c_ok) atomicAdd(&counts[WLS_OK], c_ok);
-    if (c_skip) atomicAdd(&counts[WLS_SKIPPED], c_skip);
-  }
+  // Per-block status counts (rows sent to the FP64 list are counted there).
+  unsigned v[2] = {(unsigned)n_ok, (unsigned)n_skip};
+  block_count_partials<2>(v, bpart, nullptr);
 }
 
 // Quadratic field f = c0 + cx x + cy y + cz z + cxx x^2 + cyy y^2 + czz z^2
@@ -395,7 +392,7 @@ __global__ __launch_bounds__(256) void k_wls_sample_errors(const double4* __rest
                                                            uint8_t* __restrict__ ty_out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ns) return;
-  const int64_t io = (int64_t)k * stride;  // original index; arrays below are in sorted order
+  const int64_t io = (int64_t)k * stride;  // original index; all arrays here are in sorted order
   const int64_t i = iperm[io];
   const double a = spos[i].x, b = spos[i].y, c = spos[i].z;
   const double ex = q.c[1] + 2 * q.c[4] * a + q.c[7] * b + q.c[8] * c;
@@ -405,7 +402,7 @@ __global__ __launch_bounds__(256) void k_wls_sample_errors(const double4* __rest
   ge[k] = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
   le[k] = fabs(lap[i] - q.lap_exact);
   st_out[k] = status[i];
-  ty_out[k] = type[io];
+  ty_out[k] = type[i];
 }
 
 // WLS results from sorted order back to original particle order.
@@ -447,23 +444,30 @@ void launch_field_quadratic(double4* spos, int n, const QuadField& q, hipStream_
 
 void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status,
-                      unsigned long long* counts, hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
+                      unsigned long long* counts, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin,
+                      hipEvent_t ev_end) {
   if (n <= 0) return;
+  const unsigned g = blocks(n, 128);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
-  hipLaunchKernelGGL(k_wls_exact, dim3(blocks(n, 128)), dim3(128), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, reg,
-                     grad, lap, status, counts);
+  hipLaunchKernelGGL(k_wls_exact, dim3(g), dim3(128), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, reg, grad, lap,
+                     status, bpart);
   if (ev_end) (void)hipEventRecord(ev_end, st);
+  CountSlots cs{WLS_REDO_SLOT + 1, {0, 1, 2, 3, 4, 5, 0, 0}};
+  launch_sum_counts(bpart, (int)g, cs, counts, st);
 }
 
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
-                      int* nredo, unsigned long long* counts, hipStream_t st, hipEvent_t ev_begin,
-                      hipEvent_t ev_end) {
+                      int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,
+                      hipEvent_t ev_begin, hipEvent_t ev_end) {
   if (n <= 0) return;
+  const unsigned gm = blocks(n, WAVE);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
-  hipLaunchKernelGGL(k_wls_mixed, dim3(blocks(n, WAVE)), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, grad,
-                     lap, status, redo, nredo, counts);
+  hipLaunchKernelGGL(k_wls_mixed, dim3(gm), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, grad, lap, status,
+                     redo, nredo, bpart);
   if (ev_end) (void)hipEventRecord(ev_end, st);
+  CountSlots cs{2, {WLS_OK, WLS_SKIPPED, 0, 0, 0, 0, 0, 0}};
+  launch_sum_counts(bpart, (int)gm, cs, counts, st);
   // The redo list length is only known on the device: a bounded grid
   // strides over it (typical lists are empty or a few thousand entries).
   const unsigned g = blocks(n, 128) < 512u ? blocks(n, 128) : 512u;

@@ -66,7 +66,7 @@ class PpeParams(C.Structure):
 
 class PpeResult(C.Structure):
     _fields_ = [("unknowns", _u64), ("nnz", _u64), ("dirichlet", _u64), ("neumann", _u64), ("failed_rows", _u64),
-                ("ms", _f)]
+                ("ms", _f), ("kernel_ms", _f), ("padded_nnz", _u64)]
 
 
 class SolveParams(C.Structure):
