@@ -12,9 +12,11 @@ K steps, `value` = particles / step time (Mparticles/s).
 
 Also measured, outside the timed steps: PPE assembly over all fluid
 unknowns, and the PPE solve to a true relative residual of 1e-8 with plain
-FP64 right-Jacobi BiCGStab and with mixed-precision BiCGStab (FP32 Krylov
-vectors and matrix, FP64 reliable updates), both on the same matrix and
-right-hand side (seeded x* ~ U(-1, 1)).
+FP64 right-Jacobi BiCGStab, with mixed-precision BiCGStab (FP32 Krylov
+vectors and matrix, FP64 reliable updates), and with the learned start and
+diagonal inside the mixed solver (network weights from
+ls-mps_amd/weights/, trained by python -m lsmps_amd.learn), all on the same
+matrix and right-hand side (seeded x* ~ U(-1, 1)).
 
 The CPU baseline is the reference's own src/lsmps.cpp (built into
 oracle/_ref where the reference tree was present) running neighbors +
@@ -240,15 +242,24 @@ def main():
                         "algorithmic_GBs": round(asm_bytes / (pr["ms"] * 1e-3) / 1e9, 1)}
         log(f"assembly {pr}")
         solves = {}
-        for name, method in (("fp64_jacobi", L.SOLVE_JACOBI), ("mixed", L.SOLVE_MIXED)):
+        methods = [("fp64_jacobi", L.SOLVE_JACOBI), ("mixed", L.SOLVE_MIXED)]
+        weights = os.path.join(ROOT, "ls-mps_amd", "weights", "mlp_bulk_lattice.npy")
+        if os.path.exists(weights):
+            from lsmps_amd import learn
+
+            p.set_params(learn.load_params(weights))
+            methods.append(("learned_mixed", L.SOLVE_LEARNED))
+        for name, method in methods:
             st0 = p.solve(method, args.solve_max_iter, 1e-8, chunk=32)
             st = p.solve(method, args.solve_max_iter, 1e-8, chunk=32)
             log(f"solve {name}: {st}")
             solves[name] = {"ms": round(st["ms"], 3), "iterations": st["iterations"], "converged": st["converged"],
                             "final_residual": st["final_residual"], "replacements": st["replacements"],
+                            "fallback": st["fallback"], "initial_residual": st["initial_residual"],
                             "first_call_ms": round(st0["ms"], 3)}
-        if solves["mixed"]["ms"] > 0:
-            solves["speedup_mixed_vs_fp64"] = round(solves["fp64_jacobi"]["ms"] / solves["mixed"]["ms"], 3)
+        for name in ("mixed", "learned_mixed"):
+            if name in solves and solves[name]["ms"] > 0:
+                solves[f"speedup_{name}_vs_fp64"] = round(solves["fp64_jacobi"]["ms"] / solves[name]["ms"], 3)
         extra["ppe_time_to_1e-8"] = solves
     p.close()
 

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LSMPS_HIP_ABI_VERSION 2
+#define LSMPS_HIP_ABI_VERSION 3
 
 enum {
   LSMPS_OK = 0,
@@ -150,8 +150,9 @@ enum {
   LSMPS_SOLVE_PLAIN = 0,   /* FP64, unpreconditioned, the reference's stopping rule */
   LSMPS_SOLVE_JACOBI = 1,  /* FP64, right Jacobi, true-residual check */
   LSMPS_SOLVE_MIXED = 2,   /* FP32 Krylov + FP64 reliable updates, right Jacobi */
-  LSMPS_SOLVE_LEARNED = 3, /* MIXED with the MLP diagonal and initial guess,
-                              Jacobi fallback when the guess raises the residual */
+  LSMPS_SOLVE_LEARNED = 3, /* MIXED with the MLP diagonal and initial guess
+                              (lsmps_mlp_set_params first); plain MIXED when
+                              the guess raises the residual */
 };
 typedef struct {
   int32_t method;
@@ -168,6 +169,48 @@ typedef struct {
 } lsmps_solve_result;
 int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_result* out);
 int lsmps_upload_x0(lsmps_ctx* ctx, const double* x0, uint64_t n);
+
+/* ---- learned preconditioner (no reference counterpart) -----------------
+ * A per-row MLP 16 -> 64 -> 64 -> 2 (tanh hidden units; BF16 weight
+ * matrices, FP32 biases and accumulation) on features of the assembled PPE.
+ * Output 0 times s_b (the RMS of b/diag) is the initial guess; output 1,
+ * clamped to [-0.5, 0.5], is the log of a correction of the Jacobi
+ * diagonal. LSMPS_SOLVE_LEARNED uses both inside the mixed solver and falls
+ * back to the plain mixed method when the guess has a larger true residual
+ * than the zero start; convergence is always decided on the true FP64
+ * residual. Parameters are FP32, laid out as W1 (64x16, one row per output
+ * unit), b1 (64), W2 (64x64), b2 (64), W3 (2x64), b3 (2). */
+#define LSMPS_MLP_FEATURES 16
+#define LSMPS_MLP_OUTPUTS 2
+#define LSMPS_MLP_PARAMS 5378
+/* Features of every unknown (reference row order, m x 16) and s_b; either
+ * pointer may be NULL. Needs an assembled PPE. */
+int lsmps_features(lsmps_ctx* ctx, float* feat, double* rhs_scale);
+int lsmps_mlp_set_params(lsmps_ctx* ctx, const float* params, uint64_t n);
+int lsmps_mlp_get_params(lsmps_ctx* ctx, float* params, uint64_t n);
+/* The network on n host rows of features; out[2n]. */
+int lsmps_mlp_predict(lsmps_ctx* ctx, const float* feat, uint64_t n, float* out);
+/* Loss 0.5/n * sum_rows sum_k wk[k] (out_k - target_k)^2 and its gradient
+ * (LSMPS_MLP_PARAMS floats), computed by the training kernels. */
+int lsmps_mlp_loss_grad(lsmps_ctx* ctx, const float* feat, const float* target, uint64_t n, const float* wk,
+                        float* grad, double* loss);
+/* Appends every stride-th row of the current solved PPE to the context's
+ * training set: features, and targets x / s_b and the log of the clamped
+ * least-squares ratio sum x_j q_j / sum q_j^2 (q = b / diag) over the
+ * row's columns. */
+int lsmps_train_collect(lsmps_ctx* ctx, int32_t stride, uint64_t* total);
+int lsmps_train_clear(lsmps_ctx* ctx);
+typedef struct {
+  int32_t steps, batch; /* Adam steps, rows per minibatch (drawn with replacement) */
+  float lr, wk0, wk1;   /* learning rate, loss weights of the two outputs */
+  uint32_t seed;
+} lsmps_train_params;
+typedef struct {
+  double loss_first, loss_last; /* minibatch loss at the first and the last step */
+  uint64_t samples;
+  float ms;
+} lsmps_train_result;
+int lsmps_mlp_train(lsmps_ctx* ctx, const lsmps_train_params* p, lsmps_train_result* out);
 
 /* ---- downloads (tests, parity, callers that want the arrays) ----------- */
 int lsmps_download_types(lsmps_ctx* ctx, uint8_t* type);

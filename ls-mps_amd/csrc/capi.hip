@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "learn.hpp"
 #include "solver.hpp"
 
 using namespace lsmps_hip;
@@ -102,6 +103,7 @@ struct lsmps_ctx {
   // PPE
   bool have_ppe = false;
   uint64_t m = 0, nnz = 0;
+  double ppe_re = 0.0;
   // Rows u are the unknowns in sorted-slot order, assembled straight into
   // sliced-ELL (sell_*); rr numbers them as the reference does.
   DBuf<int> flag, pos, fs, upos, uid, uslot, urr, rru, rlen;
@@ -121,6 +123,17 @@ struct lsmps_ctx {
   int* host_small = nullptr;         // pinned scratch for small readbacks (64 ints)
   bool have_x0 = false, have_solution = false;
   GraphCache graph[2];  // captured iterations, one per solve precision (FP64, FP32)
+
+  // learned preconditioner: features of the current PPE (row order u), the
+  // network (FP32 master, BF16 copy, Adam moments) and the training set
+  bool have_features = false, have_params = false;
+  DBuf<float> feat, pred, feat_rr, master, adam1, adam2, mgrad, gpart, dfeat, dtarget, tfeat, ttarget;
+  DBuf<uint16_t> wbf;
+  DBuf<double> scal, lx0, lscale;
+  DBuf<unsigned long long> fbits;
+  DBuf<int> bidx;
+  int64_t dcount = 0;
+  int adam_step = 0;
   void reset_graphs() {
     graph[0].reset();
     graph[1].reset();
@@ -246,6 +259,15 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->wstatus.release();
   ctx->st_s.release();
   ctx->ty_s.release();
+  DBuf<float>* fl[] = {&ctx->feat,  &ctx->pred, &ctx->feat_rr, &ctx->master, &ctx->adam1,  &ctx->adam2,
+                       &ctx->mgrad, &ctx->gpart, &ctx->dfeat, &ctx->dtarget, &ctx->tfeat, &ctx->ttarget};
+  for (auto* p : fl) p->release();
+  ctx->wbf.release();
+  ctx->scal.release();
+  ctx->lx0.release();
+  ctx->lscale.release();
+  ctx->fbits.release();
+  ctx->bidx.release();
   ctx->scan_scr.release();
   ctx->counters.release();
   ctx->bpart.release();
@@ -291,6 +313,7 @@ int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->n = n;
   ctx->types_stale = false;
+  ctx->have_features = false;
   ctx->have_cnl = ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_x0 = false;
   ctx->have_field = false;
   ctx->reset_graphs();
@@ -695,9 +718,10 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
   HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   ctx->m = (uint64_t)m;
+  ctx->ppe_re = p->re;
   ctx->nnz = m > 0 ? (uint64_t)c[2] : 0;
   ctx->have_ppe = true;
-  ctx->have_solution = ctx->have_x0 = false;
+  ctx->have_solution = ctx->have_x0 = ctx->have_features = false;
   ctx->reset_graphs();
   if (out) {
     out->unknowns = (uint64_t)m;
@@ -735,6 +759,30 @@ int lsmps_upload_x0(lsmps_ctx* ctx, const double* x0, uint64_t n) {
 }  // extern "C"
 
 namespace {
+
+// Features of every row of the assembled PPE (row order u) and s_b.
+int compute_features(lsmps_ctx* ctx) {
+  const int m = (int)ctx->m;
+  hipStream_t st = ctx->st;
+  HIPCHK(ctx->feat.ensure((size_t)m * LEARN_F));
+  HIPCHK(ctx->scal.ensure(4));
+  HIPCHK(ctx->fbits.ensure((size_t)m / 64 + 1));
+  launch_rhs_scale(ctx->b.p, ctx->diag.p, m, ctx->partials.p, ctx->scal.p, st);
+  launch_features(ctx->spos.p, ctx->type_s.p, ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn,
+                  m, ctx->ppe_re, ctx->diag.p, ctx->b.p, ctx->rlen.p, ctx->fbits.p, ctx->scal.p, ctx->feat.p, st);
+  if (int rc = check_launch(ctx, "features")) return rc;
+  ctx->have_features = true;
+  return LSMPS_OK;
+}
+
+int ensure_mlp_buffers(lsmps_ctx* ctx) {
+  HIPCHK(ctx->master.ensure(LEARN_P));
+  HIPCHK(ctx->adam1.ensure(LEARN_P));
+  HIPCHK(ctx->adam2.ensure(LEARN_P));
+  HIPCHK(ctx->mgrad.ensure(LEARN_P + 1));
+  HIPCHK(ctx->wbf.ensure(LEARN_P));
+  return LSMPS_OK;
+}
 
 template <typename T>
 int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_iter, double tol, int flags, int chunk,
@@ -833,6 +881,58 @@ int solve_fp64_jacobi(lsmps_ctx* ctx, const double* x0, int max_iter, double tol
   return run_bicgstab<double>(ctx, B, x0, max_iter, tol, SOLVE_TRUE_CHECK, chunk, out);
 }
 
+// Mixed-precision BiCGStab with scaling `scale` from x0 (or zero). With
+// precheck, returns 1 (and does nothing else) when x0 has a larger true
+// residual than the zero start. When the mixed solve does not converge,
+// FP64 right-Jacobi BiCGStab continues from its iterate (fallback = 1).
+int solve_mixed(lsmps_ctx* ctx, const double* scale, const double* x0, bool precheck, int max_iter, double tol,
+                int chunk, lsmps_solve_result* out) {
+  const int m = (int)ctx->m;
+  hipStream_t st = ctx->st;
+  HIPCHK(ctx->sell_v32.ensure(ctx->sell_nnz));
+  HIPCHK(ctx->vec32.ensure(6 * (size_t)m));
+  launch_sell_scale<float>(ctx->sell_col.p, ctx->sell_v64.p, scale, (int64_t)ctx->sell_nnz, ctx->sell_v32.p, st);
+  SolverBufs<float> B;
+  B.m = m;
+  B.sptr = ctx->sell_ptr.p;
+  B.col = ctx->sell_col.p;
+  B.val = ctx->sell_v32.p;
+  B.val64 = ctx->sell_v64.p;
+  B.b = ctx->b.p;
+  B.scale = scale;
+  B.y = ctx->yv.p;
+  B.partials = ctx->partials.p;
+  bind_vectors(B, ctx->vec32.p, m);
+  const int flags = SOLVE_RELIABLE | SOLVE_TRUE_CHECK | SOLVE_RESTART;
+  if (precheck && x0) {
+    launch_solver_init<float>(B, x0, ctx->sc.p, max_iter, tol, flags, 0.1, st);
+    if (int rc = check_launch(ctx, "solver_init")) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const SolverScalars& s0 = *ctx->host_sc;
+    // The residual norm is taken of the FP32 residual vector: allow its
+    // rounding before calling the start worse than zero.
+    if (!(s0.init_res <= s0.bnorm * (1.0 + 1e-6))) return 1;
+  }
+  int rc = run_bicgstab<float>(ctx, B, x0, max_iter, tol, flags, chunk, out);
+  if (rc == LSMPS_OK && !out->converged) {
+    // Fallback and judge: FP64 right-Jacobi BiCGStab, continuing from the
+    // mixed iterate when it lowered the residual, else from zero.
+    const lsmps_solve_result first = *out;
+    const bool resume = std::isfinite(first.final_residual) && first.final_residual < first.initial_residual;
+    if (resume) {
+      HIPCHK(ctx->x0.ensure(m));
+      HIPCHK(hipMemcpyAsync(ctx->x0.p, ctx->xsol.p, (size_t)m * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    rc = solve_fp64_jacobi(ctx, resume ? ctx->x0.p : nullptr, max_iter, tol, chunk, out);
+    out->iterations += first.iterations;
+    out->replacements += first.replacements;
+    out->initial_residual = first.initial_residual;
+    out->fallback = 1;
+  }
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -841,7 +941,7 @@ int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_res
   if (!ctx || !p || !out) return LSMPS_E_ARG;
   if (!ctx->have_ppe) return fail(ctx, LSMPS_E_STATE, "solve needs an assembled PPE");
   if (p->use_x0 && !ctx->have_x0) return fail(ctx, LSMPS_E_STATE, "use_x0 without lsmps_upload_x0");
-  if (p->method < LSMPS_SOLVE_PLAIN || p->method > LSMPS_SOLVE_MIXED)
+  if (p->method < LSMPS_SOLVE_PLAIN || p->method > LSMPS_SOLVE_LEARNED)
     return fail(ctx, LSMPS_E_ARG, "unknown solve method");
   const int chunk = p->chunk > 0 ? p->chunk : 16;
   GUARD_BEGIN
@@ -856,36 +956,23 @@ int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_res
   const double* x0 = p->use_x0 ? ctx->x0.p : nullptr;
   int rc = LSMPS_OK;
   if (p->method == LSMPS_SOLVE_MIXED) {
-    HIPCHK(ctx->sell_v32.ensure(ctx->sell_nnz));
-    HIPCHK(ctx->vec32.ensure(6 * (size_t)m));
     launch_inv_diag(ctx->diag.p, m, ctx->scale.p, st);
-    launch_sell_scale<float>(ctx->sell_col.p, ctx->sell_v64.p, ctx->scale.p, (int64_t)ctx->sell_nnz, ctx->sell_v32.p, st);
-    SolverBufs<float> B;
-    B.m = m;
-    B.sptr = ctx->sell_ptr.p;
-    B.col = ctx->sell_col.p;
-    B.val = ctx->sell_v32.p;
-    B.val64 = ctx->sell_v64.p;
-    B.b = ctx->b.p;
-    B.scale = ctx->scale.p;
-    B.y = ctx->yv.p;
-    B.partials = ctx->partials.p;
-    bind_vectors(B, ctx->vec32.p, m);
-    rc = run_bicgstab<float>(ctx, B, x0, p->max_iter, p->tol, SOLVE_RELIABLE | SOLVE_TRUE_CHECK | SOLVE_RESTART,
-                             chunk, out);
-    if (rc == LSMPS_OK && !out->converged) {
-      // Fallback and judge: FP64 right-Jacobi BiCGStab, continuing from the
-      // mixed iterate when it lowered the residual, else from zero.
-      const lsmps_solve_result first = *out;
-      const bool resume = std::isfinite(first.final_residual) && first.final_residual < first.initial_residual;
-      if (resume) {
-        HIPCHK(ctx->x0.ensure(m));
-        HIPCHK(hipMemcpyAsync(ctx->x0.p, ctx->xsol.p, (size_t)m * sizeof(double), hipMemcpyDeviceToDevice, st));
-      }
-      rc = solve_fp64_jacobi(ctx, resume ? ctx->x0.p : nullptr, p->max_iter, p->tol, chunk, out);
-      out->iterations += first.iterations;
-      out->replacements += first.replacements;
-      out->initial_residual = first.initial_residual;
+    rc = solve_mixed(ctx, ctx->scale.p, x0, false, p->max_iter, p->tol, chunk, out);
+  } else if (p->method == LSMPS_SOLVE_LEARNED) {
+    if (!ctx->have_params) return fail(ctx, LSMPS_E_STATE, "LEARNED solve needs network parameters");
+    if (!ctx->have_features)
+      if (int r2 = compute_features(ctx)) return r2;
+    HIPCHK(ctx->pred.ensure((size_t)m * LEARN_O));
+    HIPCHK(ctx->lx0.ensure(m));
+    HIPCHK(ctx->lscale.ensure(m));
+    launch_mlp_forward(ctx->feat.p, m, ctx->wbf.p, ctx->master.p, ctx->pred.p, st);
+    launch_learned_apply(ctx->pred.p, ctx->diag.p, m, ctx->scal.p, ctx->lx0.p, ctx->lscale.p, st);
+    if (int r2 = check_launch(ctx, "learned_apply")) return r2;
+    rc = solve_mixed(ctx, ctx->lscale.p, ctx->lx0.p, true, p->max_iter, p->tol, chunk, out);
+    if (rc == 1) {
+      // The predicted start raised the residual: the plain method instead.
+      launch_inv_diag(ctx->diag.p, m, ctx->scale.p, st);
+      rc = solve_mixed(ctx, ctx->scale.p, x0, false, p->max_iter, p->tol, chunk, out);
       out->fallback = 1;
     }
   } else if (p->method == LSMPS_SOLVE_JACOBI) {
@@ -1007,6 +1094,186 @@ int lsmps_download_solution(lsmps_ctx* ctx, double* x) {
   if (int rc = check_launch(ctx, "to_rr")) return rc;
   if (x && m) HIPCHK(hipMemcpyAsync(x, ctx->vrr.p, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
+  return LSMPS_OK;
+  GUARD_END
+}
+
+// ---- learned preconditioner -------------------------------------------------
+
+int lsmps_features(lsmps_ctx* ctx, float* feat, double* rhs_scale) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (!ctx->have_ppe) return fail(ctx, LSMPS_E_STATE, "features need an assembled PPE");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  const int m = (int)ctx->m;
+  if (int rc = compute_features(ctx)) return rc;
+  if (feat && m) {
+    HIPCHK(ctx->feat_rr.ensure((size_t)m * LEARN_F));
+    launch_rows_to_rr(ctx->rru.p, ctx->feat.p, m, LEARN_F, ctx->feat_rr.p, ctx->st);
+    if (int rc = check_launch(ctx, "rows_to_rr")) return rc;
+    HIPCHK(hipMemcpyAsync(feat, ctx->feat_rr.p, (size_t)m * LEARN_F * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+  }
+  double sc = 1.0;
+  HIPCHK(hipMemcpyAsync(&sc, ctx->scal.p, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (rhs_scale) *rhs_scale = sc;
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_mlp_set_params(lsmps_ctx* ctx, const float* params, uint64_t n) {
+  if (!ctx || !params) return LSMPS_E_ARG;
+  if (n != (uint64_t)LEARN_P) return fail(ctx, LSMPS_E_ARG, "parameter count differs from LSMPS_MLP_PARAMS");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  if (int rc = ensure_mlp_buffers(ctx)) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->master.p, params, LEARN_P * sizeof(float), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemsetAsync(ctx->adam1.p, 0, LEARN_P * sizeof(float), ctx->st));
+  HIPCHK(hipMemsetAsync(ctx->adam2.p, 0, LEARN_P * sizeof(float), ctx->st));
+  launch_mlp_cast(ctx->master.p, ctx->wbf.p, ctx->st);
+  if (int rc = check_launch(ctx, "mlp_cast")) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->adam_step = 0;
+  ctx->have_params = true;
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_mlp_get_params(lsmps_ctx* ctx, float* params, uint64_t n) {
+  if (!ctx || !params) return LSMPS_E_ARG;
+  if (!ctx->have_params) return fail(ctx, LSMPS_E_STATE, "no network parameters");
+  if (n != (uint64_t)LEARN_P) return fail(ctx, LSMPS_E_ARG, "parameter count differs from LSMPS_MLP_PARAMS");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemcpyAsync(params, ctx->master.p, LEARN_P * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return LSMPS_OK;
+}
+
+int lsmps_mlp_predict(lsmps_ctx* ctx, const float* feat, uint64_t n, float* out) {
+  if (!ctx || (n && (!feat || !out))) return LSMPS_E_ARG;
+  if (!ctx->have_params) return fail(ctx, LSMPS_E_STATE, "no network parameters");
+  if (n > (uint64_t)INT_MAX / LEARN_F) return fail(ctx, LSMPS_E_LIMIT, "too many rows");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ctx->tfeat.ensure(n * LEARN_F));
+  HIPCHK(ctx->ttarget.ensure(n * LEARN_O));
+  if (n) {
+    HIPCHK(hipMemcpyAsync(ctx->tfeat.p, feat, n * LEARN_F * sizeof(float), hipMemcpyHostToDevice, ctx->st));
+    launch_mlp_forward(ctx->tfeat.p, (int)n, ctx->wbf.p, ctx->master.p, ctx->ttarget.p, ctx->st);
+    if (int rc = check_launch(ctx, "mlp_forward")) return rc;
+    HIPCHK(hipMemcpyAsync(out, ctx->ttarget.p, n * LEARN_O * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_mlp_loss_grad(lsmps_ctx* ctx, const float* feat, const float* target, uint64_t n, const float* wk,
+                        float* grad, double* loss) {
+  if (!ctx || !wk || (n && (!feat || !target))) return LSMPS_E_ARG;
+  if (!ctx->have_params) return fail(ctx, LSMPS_E_STATE, "no network parameters");
+  if (n > (uint64_t)INT_MAX / LEARN_F) return fail(ctx, LSMPS_E_LIMIT, "too many rows");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ctx->tfeat.ensure(n * LEARN_F));
+  HIPCHK(ctx->ttarget.ensure(n * LEARN_O));
+  HIPCHK(ctx->bidx.ensure(n));
+  HIPCHK(ctx->gpart.ensure((size_t)mlp_grad_blocks((int)n) * (LEARN_P + 1)));
+  if (n) {
+    HIPCHK(hipMemcpyAsync(ctx->tfeat.p, feat, n * LEARN_F * sizeof(float), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->ttarget.p, target, n * LEARN_O * sizeof(float), hipMemcpyHostToDevice, ctx->st));
+    launch_iota((int)n, ctx->bidx.p, ctx->st);
+    launch_mlp_grad(ctx->tfeat.p, ctx->ttarget.p, ctx->bidx.p, (int)n, ctx->wbf.p, ctx->master.p, wk[0], wk[1],
+                    ctx->gpart.p, ctx->mgrad.p, ctx->st);
+    if (int rc = check_launch(ctx, "mlp_grad")) return rc;
+  }
+  std::vector<float> g(LEARN_P + 1, 0.0f);
+  if (n) HIPCHK(hipMemcpyAsync(g.data(), ctx->mgrad.p, (LEARN_P + 1) * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (grad) std::memcpy(grad, g.data(), LEARN_P * sizeof(float));
+  if (loss) *loss = g[LEARN_P];
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_train_collect(lsmps_ctx* ctx, int32_t stride, uint64_t* total) {
+  if (!ctx || stride < 1) return LSMPS_E_ARG;
+  if (!ctx->have_solution) return fail(ctx, LSMPS_E_STATE, "collecting samples needs a solved PPE");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  const int m = (int)ctx->m;
+  if (!ctx->have_features)
+    if (int rc = compute_features(ctx)) return rc;
+  const int start = (int)(ctx->dcount % stride);
+  const int c = collect_count(m, start, stride);
+  const int64_t need = ctx->dcount + c;
+  if (need > (int64_t)INT_MAX / LEARN_F) return fail(ctx, LSMPS_E_LIMIT, "training set beyond int32 indexing");
+  if ((size_t)need * LEARN_F > ctx->dfeat.cap) {
+    // Grow (keeping the samples already collected).
+    const size_t cap = std::max<size_t>((size_t)need, 2 * ctx->dfeat.cap / LEARN_F);
+    DBuf<float> nf, nt;
+    HIPCHK(nf.ensure(cap * LEARN_F));
+    HIPCHK(nt.ensure(cap * LEARN_O));
+    if (ctx->dcount) {
+      HIPCHK(hipMemcpyAsync(nf.p, ctx->dfeat.p, ctx->dcount * LEARN_F * sizeof(float), hipMemcpyDeviceToDevice,
+                            ctx->st));
+      HIPCHK(hipMemcpyAsync(nt.p, ctx->dtarget.p, ctx->dcount * LEARN_O * sizeof(float), hipMemcpyDeviceToDevice,
+                            ctx->st));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->dfeat.release();
+    ctx->dtarget.release();
+    ctx->dfeat = nf;
+    ctx->dtarget = nt;
+  }
+  launch_collect(ctx->feat.p, ctx->xsol.p, ctx->b.p, ctx->diag.p, ctx->sell_ptr.p, ctx->sell_col.p, ctx->rlen.p, m,
+                 start, stride, ctx->scal.p, ctx->dcount, ctx->dfeat.p, ctx->dtarget.p, ctx->st);
+  if (int rc = check_launch(ctx, "collect")) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->dcount = need;
+  if (total) *total = (uint64_t)ctx->dcount;
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_train_clear(lsmps_ctx* ctx) {
+  if (!ctx) return LSMPS_E_ARG;
+  ctx->dcount = 0;
+  return LSMPS_OK;
+}
+
+int lsmps_mlp_train(lsmps_ctx* ctx, const lsmps_train_params* p, lsmps_train_result* out) {
+  if (!ctx || !p || !out) return LSMPS_E_ARG;
+  if (!ctx->have_params) return fail(ctx, LSMPS_E_STATE, "training needs initial network parameters");
+  if (ctx->dcount <= 0) return fail(ctx, LSMPS_E_STATE, "no training samples (lsmps_train_collect)");
+  if (p->steps < 0 || p->batch < 1 || !(p->lr > 0)) return fail(ctx, LSMPS_E_ARG, "bad training parameters");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  std::memset(out, 0, sizeof(*out));
+  hipStream_t st = ctx->st;
+  const int nb = p->batch;
+  HIPCHK(ctx->bidx.ensure(nb));
+  HIPCHK(ctx->gpart.ensure((size_t)mlp_grad_blocks(nb) * (LEARN_P + 1)));
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  float first = 0.0f, last = 0.0f;
+  for (int k = 0; k < p->steps; ++k) {
+    launch_batch_indices(p->seed, (uint32_t)k, nb, (int)ctx->dcount, ctx->bidx.p, st);
+    launch_mlp_grad(ctx->dfeat.p, ctx->dtarget.p, ctx->bidx.p, nb, ctx->wbf.p, ctx->master.p, p->wk0, p->wk1,
+                    ctx->gpart.p, ctx->mgrad.p, st);
+    if (k == 0) HIPCHK(hipMemcpyAsync(&first, ctx->mgrad.p + LEARN_P, sizeof(float), hipMemcpyDeviceToHost, st));
+    ctx->adam_step += 1;
+    launch_adam(ctx->master.p, ctx->adam1.p, ctx->adam2.p, ctx->mgrad.p, p->lr, 0.9f, 0.999f, 1e-8f, ctx->adam_step,
+                ctx->wbf.p, st);
+    if (k == p->steps - 1)
+      HIPCHK(hipMemcpyAsync(&last, ctx->mgrad.p + LEARN_P, sizeof(float), hipMemcpyDeviceToHost, st));
+    if (int rc = check_launch(ctx, "mlp_train")) return rc;
+  }
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  out->loss_first = first;
+  out->loss_last = last;
+  out->samples = (uint64_t)ctx->dcount;
+  out->ms = elapsed(ctx);
   return LSMPS_OK;
   GUARD_END
 }

@@ -79,6 +79,17 @@ class SolveResult(C.Structure):
                 ("max_error", _d), ("ms", _f)]
 
 
+class TrainParams(C.Structure):
+    _fields_ = [("steps", _i32), ("batch", _i32), ("lr", _f), ("wk0", _f), ("wk1", _f), ("seed", C.c_uint32)]
+
+
+class TrainResult(C.Structure):
+    _fields_ = [("loss_first", _d), ("loss_last", _d), ("samples", _u64), ("ms", _f)]
+
+
+MLP_FEATURES, MLP_OUTPUTS, MLP_PARAMS = 16, 2, 5378
+
+
 def as_dict(s):
     out = {}
     for name, _ in s._fields_:
@@ -110,6 +121,14 @@ ABI = {
     "lsmps_download_wls": (C.c_int, [_p, _p, _p, _p]),
     "lsmps_download_ppe": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),
     "lsmps_download_solution": (C.c_int, [_p, _p]),
+    "lsmps_features": (C.c_int, [_p, _p, C.POINTER(_d)]),
+    "lsmps_mlp_set_params": (C.c_int, [_p, _p, _u64]),
+    "lsmps_mlp_get_params": (C.c_int, [_p, _p, _u64]),
+    "lsmps_mlp_predict": (C.c_int, [_p, _p, _u64, _p]),
+    "lsmps_mlp_loss_grad": (C.c_int, [_p, _p, _p, _u64, _p, _p, C.POINTER(_d)]),
+    "lsmps_train_collect": (C.c_int, [_p, _i32, C.POINTER(_u64)]),
+    "lsmps_train_clear": (C.c_int, [_p]),
+    "lsmps_mlp_train": (C.c_int, [_p, C.POINTER(TrainParams), C.POINTER(TrainResult)]),
 }
 
 _LIB = None
@@ -266,3 +285,50 @@ class Pipeline:
         x = np.empty(max(self.m, 1), np.float64)
         self._check(self.L.lsmps_download_solution(self.h, _ptr(x)), "download_solution")
         return x[: self.m]
+
+    # -- learned preconditioner --------------------------------------------------
+    def features(self):
+        """(m x 16 features in reference row order, s_b)."""
+        f = np.empty((max(self.m, 1), MLP_FEATURES), np.float32)
+        sb = _d()
+        self._check(self.L.lsmps_features(self.h, _ptr(f), C.byref(sb)), "features")
+        return f[: self.m], sb.value
+
+    def set_params(self, params):
+        w = np.ascontiguousarray(params, np.float32)
+        self._check(self.L.lsmps_mlp_set_params(self.h, _ptr(w), w.shape[0]), "mlp_set_params")
+
+    def get_params(self):
+        w = np.empty(MLP_PARAMS, np.float32)
+        self._check(self.L.lsmps_mlp_get_params(self.h, _ptr(w), MLP_PARAMS), "mlp_get_params")
+        return w
+
+    def predict(self, feat):
+        f = np.ascontiguousarray(feat, np.float32).reshape(-1, MLP_FEATURES)
+        out = np.empty((max(f.shape[0], 1), MLP_OUTPUTS), np.float32)
+        self._check(self.L.lsmps_mlp_predict(self.h, _ptr(f), f.shape[0], _ptr(out)), "mlp_predict")
+        return out[: f.shape[0]]
+
+    def loss_grad(self, feat, target, wk=(1.0, 1.0)):
+        f = np.ascontiguousarray(feat, np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1, MLP_OUTPUTS)
+        w = np.ascontiguousarray(wk, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = _d()
+        self._check(self.L.lsmps_mlp_loss_grad(self.h, _ptr(f), _ptr(t), f.shape[0], _ptr(w), _ptr(g), C.byref(loss)),
+                    "mlp_loss_grad")
+        return loss.value, g
+
+    def train_collect(self, stride=1):
+        tot = _u64()
+        self._check(self.L.lsmps_train_collect(self.h, stride, C.byref(tot)), "train_collect")
+        return tot.value
+
+    def train_clear(self):
+        self._check(self.L.lsmps_train_clear(self.h), "train_clear")
+
+    def train(self, steps, batch=4096, lr=1e-3, wk=(1.0, 1.0), seed=1):
+        p = TrainParams(steps, batch, lr, wk[0], wk[1], seed)
+        r = TrainResult()
+        self._check(self.L.lsmps_mlp_train(self.h, C.byref(p), C.byref(r)), "mlp_train")
+        return as_dict(r)

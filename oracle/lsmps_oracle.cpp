@@ -700,6 +700,60 @@ void orc_get_ppe(const orc_state* s, int32_t* row, int32_t* col, double* val, do
   if (rev) std::copy(s->rev.begin(), s->rev.end(), rev);
 }
 
+// Features of the learned preconditioner, row by row in reference order
+// (definitions in lsmps_oracle.h); plain sequential FP64 sums.
+void orc_features(const orc_state* s, double re, float* feat, double* rhs_scale) {
+  const Csr& A = s->A;
+  const size_t m = s->rev.size();
+  std::vector<int> map(s->r.size(), -1);
+  for (size_t k = 0; k < m; ++k) map[s->rev[k]] = (int)k;
+  std::vector<double> d(m);
+  for (size_t k = 0; k < m; ++k) d[k] = A.val[A.row[k + 1] - 1];  // diagonal is stored last
+  double ss = 0.0;
+  for (size_t k = 0; k < m; ++k) ss += (A.b[k] / d[k]) * (A.b[k] / d[k]);
+  double sb = m ? std::sqrt(ss / (double)m) : 0.0;
+  if (!(sb > 0.0) || !std::isfinite(sb)) sb = 1.0;
+  if (rhs_scale) *rhs_scale = sb;
+  for (size_t k = 0; k < m; ++k) {
+    const int i = s->rev[k];
+    const std::vector<int>& nb = s->nbr[i];
+    const size_t kn = nb.size();
+    double sr = 0, sr2 = 0, rmin = 1e300, wu = 0, wall = 0, nbs = 0;
+    int nu = 0, nw = 0;
+    for (int j : nb) {
+      const double dx = s->r[j].x - s->r[i].x, dy = s->r[j].y - s->r[i].y, dz = s->r[j].z - s->r[i].z;
+      const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+      const double w = kernel_weight(r, re), rn = r / re;
+      sr += rn;
+      sr2 += rn * rn;
+      rmin = std::min(rmin, rn);
+      const double a = w > 0 ? w / (r * r) : 0.0;
+      wall += a;
+      nw += s->type[j] == T_WALL ? 1 : 0;
+      if (map[j] >= 0 && w > 0) {
+        wu += a;
+        ++nu;
+        nbs += a * (A.b[map[j]] / d[map[j]]);
+      }
+    }
+    const double ik = kn ? 1.0 / (double)kn : 0.0;
+    const bool pinned = k == 0 || (A.row[k + 1] - A.row[k] == 1 && d[k] == 1.0);
+    float* f = feat + k * 16;
+    for (int t = 0; t < 5; ++t) f[t] = s->type[i] == t ? 1.0f : 0.0f;
+    f[5] = (float)(kn / 32.0);
+    f[6] = (float)(sr * ik);
+    f[7] = (float)(sr2 * ik);
+    f[8] = (float)(kn ? rmin : 0.0);
+    f[9] = (float)(d[k] * re * re / 32.0);
+    f[10] = (float)(wall > 0 ? wu / wall : 0.0);
+    f[11] = (float)(nu * ik);
+    f[12] = (float)(A.b[k] / d[k] / sb);
+    f[13] = pinned ? 0.0f : (float)(nbs / d[k] / sb);
+    f[14] = (float)(nw * ik);
+    f[15] = pinned ? 1.0f : 0.0f;
+  }
+}
+
 void orc_solve_ppe(orc_state* s, int max_iter, double tol, int precond, const double* x0, orc_ppe_stats* out,
                    double* x) {
   std::vector<double> sol;

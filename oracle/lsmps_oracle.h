@@ -115,6 +115,18 @@ void orc_run(orc_state* s, double l0, double re_ratio, int max_neighbors, int wl
              int max_unknowns, int max_iter, double tol, int do_solve,
              orc_run_stats* rs, orc_wls_stats* ws, orc_ppe_stats* ps);
 
+/* Features of the learned preconditioner for every unknown of the
+ * assembled PPE, reference row order, feat[16 m] (no reference
+ * counterpart; definitions follow the GPU kernel's):
+ *   0-4 one-hot type; 5 k/32; 6-8 mean r/re, mean (r/re)^2, min r/re over
+ *   the neighbor list; 9 diag re^2/32; 10 sum of w/r^2 over unknown
+ *   neighbors over the sum over all neighbors; 11 unknown neighbors / k;
+ *   12 (b/diag)/s_b; 13 sum_j (a_ij/diag)(b_j/diag_j)/s_b (0 on pinned
+ *   and failed rows, whose diagonal is not a sum of couplings); 14 Wall
+ *   neighbors / k; 15 pinned or failed row.
+ * s_b = sqrt(mean (b/diag)^2). */
+void orc_features(const orc_state* s, double re, float* feat, double* rhs_scale);
+
 /* MLP 16->64->64->2 (tanh hidden units), weights fp32 row-major
  * [W1(64x16) b1(64) W2(64x64) b2(64) W3(2x64) b3(2)]. Forward over n rows;
  * out[2n]. */

@@ -88,6 +88,7 @@ def lib():
         L.orc_solve_ppe.argtypes = [_p, C.c_int, _d, C.c_int, _p, C.POINTER(PpeStats), _p]
         L.orc_run.argtypes = [_p, _d, _d, C.c_int, C.c_int, C.c_int, C.c_int, _d, C.c_int,
                               C.POINTER(RunStats), C.POINTER(WlsStats), C.POINTER(PpeStats)]
+        L.orc_features.argtypes = [_p, _d, _p, C.POINTER(_d)]
         L.orc_mlp_forward.argtypes = [_p, _p, _u64, _p]
         L.orc_mlp_loss_grad.restype = _d
         L.orc_mlp_loss_grad.argtypes = [_p, _p, _p, _p, _u64, _p]
@@ -188,6 +189,14 @@ class Oracle:
         rev = np.empty(max(m, 1), np.int32)
         self.L.orc_get_ppe(self.h, _ptr(row), _ptr(col), _ptr(val), _ptr(b), _ptr(xe), _ptr(rev))
         return dict(row=row, col=col[:nnz], val=val[:nnz], b=b[:m], x_exact=xe[:m], rev=rev[:m])
+
+    def features(self, re):
+        """(m x 16 learned-preconditioner features in row order, s_b)."""
+        m = len(self.get_ppe()["b"])
+        f = np.empty((max(m, 1), 16), np.float32)
+        sb = _d()
+        self.L.orc_features(self.h, re, _ptr(f), C.byref(sb))
+        return f[:m], sb.value
 
     def solve(self, max_iter, tol, precond=0, x0=None):
         st = PpeStats()

@@ -258,3 +258,37 @@ def test_mlp_forward_and_gradient():
         lm = 0.5 * np.mean(((_np_mlp(Pd - e, X) - T) ** 2 * wk).sum(1))
         fd = (lp - lm) / 2e-6
         assert abs(fd - g[k]) <= 2e-4 * max(1.0, abs(fd)), (k, fd, g[k])
+
+
+def test_feature_invariants():
+    """Pins the feature normalisations of the learned preconditioner: the
+    type one-hot sums to 1 and matches the particle type, k/32 matches the
+    neighbor lists, rows whose neighbors are all unknowns keep all their
+    coupling (features 10 and 11 are 1), feature 12 is b/diag over its RMS
+    (mean square exactly 1), and only the pinned row 0 (and failed rows)
+    carry feature 15."""
+    l0, re = 0.01, 0.021
+    xyz, t = O.generate_case("wall_film", 3000, 5, l0)
+    o = O.Oracle(xyz, t)
+    o.neighbors(re, 96)
+    o.classify(l0)
+    o.assemble(re, 1 << 30, 1, 9)
+    f, sb = o.features(re)
+    A = o.get_ppe()
+    row, col = o.get_neighbors()
+    types = o.types()
+    rev = A["rev"]
+    assert np.allclose(f[:, :5].sum(1), 1.0)
+    assert np.array_equal(f[:, :5].argmax(1), types[rev])
+    assert np.allclose(f[:, 5], np.diff(row)[rev] / 32.0)
+    unknown = np.zeros(len(types), bool)
+    unknown[rev] = True
+    for k in range(len(rev)):
+        nb = col[row[rev[k]]:row[rev[k] + 1]]
+        if len(nb) and unknown[nb].all() and k != 0:
+            assert abs(f[k, 10] - 1.0) < 1e-6 and abs(f[k, 11] - 1.0) < 1e-6
+    d = A["val"][A["row"][1:] - 1]
+    q = A["b"] / d
+    assert abs(sb - np.sqrt(np.mean(q * q))) <= 1e-12 * sb
+    assert abs(np.mean(f[:, 12].astype(np.float64) ** 2) - 1.0) < 1e-5
+    assert f[0, 15] == 1.0 and f[0, 13] == 0.0
