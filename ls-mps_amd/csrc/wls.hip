@@ -326,7 +326,20 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
           const double4 pi = spos[s];
           const int* row = nbr + (size_t)s * maxn;
           const int kn = cnt[s];
-          for (int k = g; k < kn; k += WLS_GROUP) wls_accumulate(spos[row[k]], pi, inv_re, inv_h2, inv_g, rmin2, a);
+          // Four neighbors per lane in flight: all four indices, then all
+          // four records, then the four updates (the loads of one batch
+          // do not wait on each other).
+          for (int k0 = g; k0 < kn; k0 += 4 * WLS_GROUP) {
+            int j[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) j[u] = k0 + u * WLS_GROUP < kn ? row[k0 + u * WLS_GROUP] : -1;
+            double4 pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pj[u] = j[u] >= 0 ? spos[j[u]] : pi;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (j[u] >= 0) wls_accumulate(pj[u], pi, inv_re, inv_h2, inv_g, rmin2, a);
+          }
         }
       }
 #pragma unroll
