@@ -204,14 +204,21 @@ __global__ __launch_bounds__(256) void k_cnl_pass(const double4* __restrict__ sp
     const int x0 = cxl > 0 ? cxl - 1 : 0, x1 = cxl + 1 < gx ? cxl + 1 : gx - 1;
     const unsigned below = (1u << g) - 1u;
     const int gshift = lane & ~(CNL_GROUP - 1);
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int zz = czl + dz;
-      if (zz < 0 || zz >= gz) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int yy = cyl + dy;
-        if (yy < 0 || yy >= gy) continue;
-        const int rowc = (zz * gy + yy) * gx;
-        const int b = start[rowc + x0], e = start[rowc + x1 + 1];
+    // All nine run bounds first (independent loads, one latency), then the
+    // runs in visiting order.
+    int rb[9], re_[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const int zz = czl + r / 3 - 1, yy = cyl + r % 3 - 1;
+      const bool ok = zz >= 0 && zz < gz && yy >= 0 && yy < gy;
+      const int rowc = ok ? (zz * gy + yy) * gx : 0;
+      rb[r] = ok ? start[rowc + x0] : 0;
+      re_[r] = ok ? start[rowc + x1 + 1] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      {
+        const int b = rb[r], e = re_[r];
         for (int base = b; base < e; base += 2 * CNL_GROUP) {
           const int sa = base + g, sb = base + CNL_GROUP + g;
           float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = qa;
