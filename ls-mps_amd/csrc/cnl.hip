@@ -170,7 +170,7 @@ __device__ __forceinline__ uint8_t classify_rule(uint8_t t0, int kept, double z,
 // decides every pair outside [lo, hi], a band around re^2 wider than the
 // FP32 error bound (see cnl_prefilter_band); pairs inside it take the
 // reference's FP64 test, so the result is exactly the FP64 predicate.
-constexpr int CNL_GROUP = 8;
+constexpr int CNL_GROUP = 8, CNL_CHUNKS = 4;
 __device__ __forceinline__ bool cnl_near(const float4 q, const float4 pf, int s2, const double4 pi,
                                          const double4* __restrict__ spos, double re2, float lo, float hi) {
   const float dx = q.x - pf.x, dy = q.y - pf.y, dz = q.z - pf.z;
@@ -219,25 +219,27 @@ __global__ __launch_bounds__(256) void k_cnl_pass(const double4* __restrict__ sp
     for (int r = 0; r < 9; ++r) {
       {
         const int b = rb[r], e = re_[r];
-        for (int base = b; base < e; base += 2 * CNL_GROUP) {
-          const int sa = base + g, sb = base + CNL_GROUP + g;
-          float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = qa;
-          if (sa < e) qa = sposf[sa];
-          if (sb < e) qb = sposf[sb];
-          const bool ha = sa < e && sa != s && cnl_near(qa, pf, sa, pi, spos, re2, lo, hi);
-          const bool hb = sb < e && sb != s && cnl_near(qb, pf, sb, pi, spos, re2, lo, hi);
-          const unsigned ma = (unsigned)(__ballot(ha) >> gshift) & 0xFFu;
-          const unsigned mb = (unsigned)(__ballot(hb) >> gshift) & 0xFFu;
-          if (ha) {
-            const int pos = found + __popc(ma & below);
-            if (pos < maxn) buf[pos] = sa;
+        // Four chunks of eight per iteration: a run (three cells, ~28
+        // candidates at the usual density) mostly takes one iteration
+        // with all four loads in flight.
+        for (int base = b; base < e; base += CNL_CHUNKS * CNL_GROUP) {
+          float4 q[CNL_CHUNKS];
+#pragma unroll
+          for (int u = 0; u < CNL_CHUNKS; ++u) {
+            const int sc = base + u * CNL_GROUP + g;
+            q[u] = sc < e ? sposf[sc] : make_float4(0.f, 0.f, 0.f, 0.f);
           }
-          found += __popc(ma);
-          if (hb) {
-            const int pos = found + __popc(mb & below);
-            if (pos < maxn) buf[pos] = sb;
+#pragma unroll
+          for (int u = 0; u < CNL_CHUNKS; ++u) {
+            const int sc = base + u * CNL_GROUP + g;
+            const bool h = sc < e && sc != s && cnl_near(q[u], pf, sc, pi, spos, re2, lo, hi);
+            const unsigned mk = (unsigned)(__ballot(h) >> gshift) & 0xFFu;
+            if (h) {
+              const int pos = found + __popc(mk & below);
+              if (pos < maxn) buf[pos] = sc;
+            }
+            found += __popc(mk);
           }
-          found += __popc(mb);
         }
       }
     }
