@@ -53,6 +53,9 @@ void lsmps_destroy(lsmps_ctx* ctx);
 const char* lsmps_last_error(const lsmps_ctx* ctx);
 /* Waits for all work queued on the context's stream. */
 int lsmps_synchronize(lsmps_ctx* ctx);
+/* "name (arch, N CUs)" of the context's device into buf (NUL-terminated).
+ * No reference counterpart: for reports written next to the reference's. */
+int lsmps_device_name(lsmps_ctx* ctx, char* buf, int32_t cap);
 
 /* Copies n particles (xyz interleaved, 3n doubles; type n bytes) into HBM.
  * Replaces the std::vector<Particle> handed to run_benchmark

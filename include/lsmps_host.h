@@ -26,6 +26,16 @@ uint64_t lsmps_generate_case_c(const char* case_name, uint64_t particles, uint32
  * did not converge or WLS failures exceed a fifth of the samples). */
 int lsmps_run_config_c(const char* cfg_path, const char* out_dir);
 
+/* Writes, as JSON, what the GPU did in the last run_benchmark of the
+ * calling thread: device, and per stage the GPU time, the time of its main
+ * kernel, the stage's algorithmic bytes (inputs read once, outputs written
+ * once), the rate they imply and its fraction of the 8 TB/s HBM peak.
+ * The reference's write_json / write_csv files are left as they are; this
+ * report sits next to them (<case>.gpu.json from the command line).
+ * Returns 0, or 1 when no run happened on this thread or the file cannot
+ * be written. */
+int lsmps_write_gpu_report_c(const char* path);
+
 #ifdef __cplusplus
 }
 #endif

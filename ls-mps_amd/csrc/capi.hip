@@ -292,6 +292,14 @@ int lsmps_synchronize(lsmps_ctx* ctx) {
   return LSMPS_OK;
 }
 
+int lsmps_device_name(lsmps_ctx* ctx, char* buf, int32_t cap) {
+  if (!ctx || !buf || cap < 1) return LSMPS_E_ARG;
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+  std::snprintf(buf, (size_t)cap, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return LSMPS_OK;
+}
+
 int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* type, uint64_t n) {
   if (!ctx) return LSMPS_E_ARG;
   if ((n && (!xyz || !type))) return fail(ctx, LSMPS_E_ARG, "null particle arrays");

@@ -54,6 +54,20 @@ lsmps_ctx* thread_ctx() {
   return holder.ctx;
 }
 
+// What the GPU did in the last run_benchmark on this thread.
+struct GpuReport {
+  bool valid = false;
+  std::string device;
+  uint64_t n = 0, m = 0, nnz = 0, padded_nnz = 0, fp64_redo = 0;
+  double k_mean = 0.0;
+  float cnl_ms = 0, cnl_kernel_ms = 0, wls_ms = 0, wls_kernel_ms = 0, asm_ms = 0, asm_kernel_ms = 0, solve_ms = 0;
+  int iterations = 0;
+};
+GpuReport& gpu_report() {
+  thread_local GpuReport r;
+  return r;
+}
+
 void check(lsmps_ctx* ctx, int rc, const char* stage) {
   if (rc != LSMPS_OK) throw std::runtime_error(std::string("lsmps ") + stage + ": " + lsmps_last_error(ctx));
 }
@@ -182,6 +196,25 @@ Metrics run_benchmark(const Config& c, std::vector<Particle>& ps) {
     m.ppe.max_error = sr.max_error;
     m.ppe.solve_ms = sr.ms;
   }
+  GpuReport& g = gpu_report();
+  g = GpuReport{};
+  char name[256] = {0};
+  if (lsmps_device_name(ctx, name, (int32_t)sizeof(name)) == LSMPS_OK) g.device = name;
+  g.n = n;
+  g.m = pr.unknowns;
+  g.nnz = pr.nnz;
+  g.padded_nnz = pr.padded_nnz;
+  g.fp64_redo = wr.fp64_redo;
+  g.k_mean = cs.mean;
+  g.cnl_ms = cs.ms;
+  g.cnl_kernel_ms = cs.kernel_ms;
+  g.wls_ms = wr.ms;
+  g.wls_kernel_ms = wr.kernel_ms;
+  g.asm_ms = pr.ms;
+  g.asm_kernel_ms = pr.kernel_ms;
+  g.solve_ms = (float)m.ppe.solve_ms;
+  g.iterations = m.ppe.iterations;
+  g.valid = true;
   return m;
 }
 
@@ -223,6 +256,43 @@ void write_csv(const std::string& path, const Config& c, const Metrics& m) {
 }
 
 }  // namespace lsmps
+
+namespace {
+void stage_json(std::ostream& o, const char* name, double ms, double kernel_ms, double bytes, bool last) {
+  const double t = kernel_ms > 0 ? kernel_ms : ms;
+  const double gbs = t > 0 ? bytes / (t * 1e-3) / 1e9 : 0.0;
+  o << "    \"" << name << "\": {\"ms\": " << ms << ", \"kernel_ms\": " << kernel_ms
+    << ", \"algorithmic_bytes\": " << bytes << ", \"GBs\": " << gbs << ", \"hbm_fraction\": " << gbs / 8000.0
+    << "}" << (last ? "\n" : ",\n");
+}
+}  // namespace
+
+// Algorithmic bytes per stage (inputs read once, outputs written once;
+// k = mean neighbor count): neighbors + classification N (24 + 1 + 4k + 4
+// + 1); WLS N (24 + 8 + 4 + 4k + 1 + 24 + 8 + 1); assembly N + m (4k + 4 +
+// 24 + 4) + 12 nnz + 24 m; one BiCGStab iteration 24 nnz + 160 m (two FP64
+// SpMVs over the stored matrix and the vector updates).
+extern "C" int lsmps_write_gpu_report_c(const char* path) {
+  const lsmps::GpuReport& g = lsmps::gpu_report();
+  if (!g.valid || !path) return 1;
+  std::ofstream o(path);
+  if (!o) return 1;
+  const double n = (double)g.n, m = (double)g.m, k = g.k_mean;
+  o << std::setprecision(6);
+  o << "{\n  \"device\": \"" << g.device << "\",\n  \"hbm_peak_GBs\": 8000,\n";
+  o << "  \"particles\": " << g.n << ", \"unknowns\": " << g.m << ", \"nnz\": " << g.nnz
+    << ", \"stored_entries\": " << g.padded_nnz << ", \"neighbor_mean\": " << k
+    << ", \"wls_fp64_redo\": " << g.fp64_redo << ",\n";
+  o << "  \"stages\": {\n";
+  stage_json(o, "neighbors_classify", g.cnl_ms, g.cnl_kernel_ms, n * (24 + 1 + 4 * k + 4 + 1), false);
+  stage_json(o, "wls", g.wls_ms, g.wls_kernel_ms, n * (24 + 8 + 4 + 4 * k + 1 + 24 + 8 + 1), false);
+  stage_json(o, "ppe_assembly", g.asm_ms, g.asm_kernel_ms, n + m * (4 * k + 4 + 24 + 4) + 12.0 * g.nnz + 24 * m,
+                    false);
+  const double it = g.iterations > 0 ? g.iterations : 1;
+  stage_json(o, "ppe_solve_per_iteration", g.solve_ms / it, 0.0, 24.0 * g.nnz + 160 * m, true);
+  o << "  },\n  \"solve_iterations\": " << g.iterations << "\n}\n";
+  return o ? 0 : 1;
+}
 
 extern "C" uint64_t lsmps_generate_case_c(const char* case_name, uint64_t particles, uint32_t seed, double l0,
                                           int film_layers, double* xyz, uint8_t* type) {

@@ -21,6 +21,7 @@ int run(const std::string& cfg_path, const std::string& out_dir) {
   const std::string base = out_dir + "/" + c.case_name;
   lsmps::write_json(base + ".json", c, m);
   lsmps::write_csv(base + ".csv", c, m);
+  lsmps_write_gpu_report_c((base + ".gpu.json").c_str());  // GPU stage report beside the reference's files
   std::ostream& o = std::cout;
   o << "LS-MPS industrial benchmark v0.2\n";
   o << "case=" << c.case_name << " particles=" << m.particles << "\n";

@@ -105,6 +105,7 @@ ABI = {
     "lsmps_destroy": (None, [_p]),
     "lsmps_last_error": (C.c_char_p, [_p]),
     "lsmps_synchronize": (C.c_int, [_p]),
+    "lsmps_device_name": (C.c_int, [_p, C.c_char_p, _i32]),
     "lsmps_upload_particles": (C.c_int, [_p, _p, _p, _u64]),
     "lsmps_build_neighbors": (C.c_int, [_p, _d, _i32, C.POINTER(CnlStats)]),
     "lsmps_build_neighbors_classify": (C.c_int, [_p, _d, _i32, _d, C.POINTER(CnlStats), C.POINTER(TypeStats)]),

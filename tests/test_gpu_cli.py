@@ -64,3 +64,8 @@ def test_cli_matches_reference_output(ref_configs, which, tmp_path):
         # The untimed prefix up to the iteration count is deterministic.
         row, want_row = csv[1].split(","), rec["cli_csv_row_untimed"].split(",")
         assert row[:14] == want_row[:14], (name, row[:14])
+        if which == "own":
+            g = json.loads((out / f"{rec['config']['case']}.gpu.json").read_text())
+            assert g["particles"] == r["particles"] and g["nnz"] == r["ppe"]["nnz"]
+            for st in ("neighbors_classify", "wls", "ppe_assembly", "ppe_solve_per_iteration"):
+                assert g["stages"][st]["ms"] > 0 and g["stages"][st]["algorithmic_bytes"] > 0, st
