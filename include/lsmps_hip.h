@@ -74,7 +74,7 @@ typedef struct {
   float kernel_ms;       /* GPU time of the neighbor pass kernel alone */
 } lsmps_cnl_stats;
 /* Lists are CSR (row_ptr int32[n+1], col int32[pairs]) in the reference's
- * visiting order, truncated to max_neighbors (<= 1024). */
+ * visiting order, truncated to max_neighbors (<= 256). */
 int lsmps_build_neighbors(lsmps_ctx* ctx, double re, int32_t max_neighbors, lsmps_cnl_stats* out);
 
 /* Neighbors and classification in one pass over the candidates (the
@@ -173,6 +173,30 @@ typedef struct {
 int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_result* out);
 int lsmps_upload_x0(lsmps_ctx* ctx, const double* x0, uint64_t n);
 
+/* ---- time stepping (no counterpart in the reference's run_benchmark; the
+ * step timeline of its Python models, python/rain_plate_lsmps_cpu.py:311-381:
+ * injection, motion, outflow removal, neighbor rebuilds on a cadence) ------ */
+/* Neighbor lists holding every pair within re + skin (cell size re + skin),
+ * so they stay valid while particles move less than skin / 2; WLS, PPE
+ * assembly and features take only pairs within re (their weights vanish
+ * beyond it), and classification counts only neighbors within re. With
+ * skin = 0 this is lsmps_build_neighbors[_classify]. max_neighbors bounds
+ * the skin lists. */
+int lsmps_build_neighbors_skin(lsmps_ctx* ctx, double re, double skin, int32_t max_neighbors, double l0,
+                               int32_t classify, lsmps_cnl_stats* out, lsmps_type_stats* types);
+/* Velocities of all particles, 3 per particle, caller order. */
+int lsmps_set_velocities(lsmps_ctx* ctx, const double* uvw, uint64_t n);
+/* Appends particles (xyz 3 per particle, types, velocities or NULL) after
+ * the existing ones; they join the stages at the next neighbor build. */
+int lsmps_append_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* type, const double* uvw,
+                           uint64_t count);
+/* x += u dt for every particle (and the sorted copies the neighbor lists
+ * refer to); WLS, PPE and solution become stale. */
+int lsmps_advance(lsmps_ctx* ctx, double dt);
+/* Removes the particles outside [lo, hi) (per axis), keeping the order of
+ * the rest; the neighbor lists must be rebuilt afterwards. */
+int lsmps_remove_outside(lsmps_ctx* ctx, const double* lo, const double* hi, uint64_t* removed);
+
 /* ---- learned preconditioner (no reference counterpart) -----------------
  * A per-row MLP 16 -> 64 -> 64 -> 2 (tanh hidden units; BF16 weight
  * matrices, FP32 biases and accumulation) on features of the assembled PPE.
@@ -217,6 +241,8 @@ int lsmps_mlp_train(lsmps_ctx* ctx, const lsmps_train_params* p, lsmps_train_res
 
 /* ---- downloads (tests, parity, callers that want the arrays) ----------- */
 int lsmps_download_types(lsmps_ctx* ctx, uint8_t* type);
+/* Current positions, caller order, 3 per particle. */
+int lsmps_download_positions(lsmps_ctx* ctx, double* xyz);
 int lsmps_download_neighbors(lsmps_ctx* ctx, int32_t* row_ptr, int32_t* col);
 int lsmps_download_wls(lsmps_ctx* ctx, double* grad3, double* lap, uint8_t* status);
 /* Any pointer may be NULL. rev[k] = particle index of unknown k. */

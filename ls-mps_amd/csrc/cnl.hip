@@ -184,7 +184,8 @@ __device__ __forceinline__ bool cnl_near(const float4 q, const float4 pf, int s2
 template <bool CLASSIFY>
 __global__ __launch_bounds__(256) void k_cnl_pass(const double4* __restrict__ spos, const float4* __restrict__ sposf,
                                                   const int* __restrict__ start, const int* __restrict__ scell, int n,
-                                                  int gx, int gy, int gz, double re2, float lo, float hi, int maxn,
+                                                  int gx, int gy, int gz, double re2, double re2c, float lo, float hi,
+                                                  int maxn,
                                                   int* __restrict__ nbr, int* __restrict__ cnt,
                                                   uint8_t* __restrict__ type_s, double zlim,
                                                   unsigned long long* __restrict__ trunc_total) {
@@ -246,7 +247,11 @@ __global__ __launch_bounds__(256) void k_cnl_pass(const double4* __restrict__ sp
   }
   __syncthreads();
   const int kept = found < maxn ? found : maxn;
+  // Classification counts and takes sectors of the kept neighbors within
+  // the classification radius (the list radius unless the lists carry a
+  // skin, in which case it drops the skin-only entries).
   unsigned mask = 0;
+  int kin = 0;
   if (live) {
     int* out = nbr + (size_t)s * maxn;
     for (int k = g; k < kept; k += CNL_GROUP) {
@@ -254,15 +259,22 @@ __global__ __launch_bounds__(256) void k_cnl_pass(const double4* __restrict__ sp
       out[k] = s2;
       if (CLASSIFY) {
         const double4 q = spos[s2];
-        mask |= sector_bits(q.x - pi.x, q.y - pi.y, q.z - pi.z);
+        const double dx = q.x - pi.x, dy = q.y - pi.y, dz = q.z - pi.z;
+        if (dx * dx + dy * dy + dz * dz < re2c) {
+          mask |= sector_bits(dx, dy, dz);
+          ++kin;
+        }
       }
     }
   }
-  if (CLASSIFY) mask = group8_or(mask);
+  if (CLASSIFY) {
+    mask = group8_or(mask);
+    kin = group8_sum_int(kin);
+  }
   const bool leader = live && g == 0;
   if (leader) {
     cnt[s] = kept;
-    if (CLASSIFY) type_s[s] = classify_rule(type_s[s], kept, pi.z, zlim, mask);
+    if (CLASSIFY) type_s[s] = classify_rule(type_s[s], kin, pi.z, zlim, mask);
   }
   unsigned long long tr = leader ? (unsigned long long)(found - kept) : 0ull;
   tr = wave_sum(tr);
@@ -391,9 +403,9 @@ static void cnl_prefilter_band(double re2, double extent, float* lo, float* hi) 
   *hi = (float)(re2 + m);
 }
 
-void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classify, double zlim, uint8_t* type_s,
-                     unsigned long long* trunc_total, int* hist, unsigned long long* tcount, hipStream_t st,
-                     hipEvent_t ev_begin, hipEvent_t ev_end) {
+void launch_cnl_pass(const CnlBufs& B, int n, double re2, double re2c, int maxn, bool classify, double zlim,
+                     uint8_t* type_s, unsigned long long* trunc_total, int* hist, unsigned long long* tcount,
+                     hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
   if (n <= 0) return;
   // 256 threads (32 particles) per block while the LDS rows stay <= 32 KB.
   const int threads = maxn <= 256 ? 256 : WAVE;
@@ -405,10 +417,10 @@ void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classif
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
   if (classify)
     hipLaunchKernelGGL(k_cnl_pass<true>, dim3(grid), dim3(threads), lds, st, B.spos, B.sposf, B.cstart, B.scell, n,
-                       B.gx, B.gy, B.gz, re2, lo, hi, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
+                       B.gx, B.gy, B.gz, re2, re2c, lo, hi, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
   else
     hipLaunchKernelGGL(k_cnl_pass<false>, dim3(grid), dim3(threads), lds, st, B.spos, B.sposf, B.cstart, B.scell, n,
-                       B.gx, B.gy, B.gz, re2, lo, hi, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
+                       B.gx, B.gy, B.gz, re2, re2c, lo, hi, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
   if (ev_end) (void)hipEventRecord(ev_end, st);
   const unsigned gs = blocks(n, 256) < 1024u ? blocks(n, 256) : 1024u;
   hipLaunchKernelGGL(k_cnl_stats, dim3(gs), dim3(256), (size_t)(maxn + 6) * sizeof(int), st, B.cnt, type_s, n, maxn,

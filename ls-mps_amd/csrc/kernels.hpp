@@ -51,9 +51,11 @@ void launch_fill_int(int* p, int64_t n, int v, hipStream_t st);
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t st);
 // Neighbor lists and classification work on sorted-order types only; the
 // original-order arrays are produced on demand by the unsort launchers.
-void launch_cnl_pass(const CnlBufs& B, int n, double re2, int maxn, bool classify, double zlim, uint8_t* type_s,
-                     unsigned long long* trunc_total, int* hist, unsigned long long* tcount, hipStream_t st,
-                     hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+// re2: squared list radius; re2c: squared radius of the neighbors that
+// classification counts (equal unless the lists carry a skin).
+void launch_cnl_pass(const CnlBufs& B, int n, double re2, double re2c, int maxn, bool classify, double zlim,
+                     uint8_t* type_s, unsigned long long* trunc_total, int* hist, unsigned long long* tcount,
+                     hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* type_s, unsigned long long* tcount,
                      hipStream_t st);
 void launch_unsort_u8(const int* iperm, const uint8_t* src, int n, uint8_t* dst, hipStream_t st);

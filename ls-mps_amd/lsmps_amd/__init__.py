@@ -118,10 +118,16 @@ ABI = {
     "lsmps_solve_ppe": (C.c_int, [_p, C.POINTER(SolveParams), C.POINTER(SolveResult)]),
     "lsmps_upload_x0": (C.c_int, [_p, _p, _u64]),
     "lsmps_download_types": (C.c_int, [_p, _p]),
+    "lsmps_download_positions": (C.c_int, [_p, _p]),
     "lsmps_download_neighbors": (C.c_int, [_p, _p, _p]),
     "lsmps_download_wls": (C.c_int, [_p, _p, _p, _p]),
     "lsmps_download_ppe": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),
     "lsmps_download_solution": (C.c_int, [_p, _p]),
+    "lsmps_build_neighbors_skin": (C.c_int, [_p, _d, _d, _i32, _d, _i32, C.POINTER(CnlStats), C.POINTER(TypeStats)]),
+    "lsmps_set_velocities": (C.c_int, [_p, _p, _u64]),
+    "lsmps_append_particles": (C.c_int, [_p, _p, _p, _p, _u64]),
+    "lsmps_advance": (C.c_int, [_p, _d]),
+    "lsmps_remove_outside": (C.c_int, [_p, _p, _p, C.POINTER(_u64)]),
     "lsmps_features": (C.c_int, [_p, _p, C.POINTER(_d)]),
     "lsmps_mlp_set_params": (C.c_int, [_p, _p, _u64]),
     "lsmps_mlp_get_params": (C.c_int, [_p, _p, _u64]),
@@ -257,6 +263,11 @@ class Pipeline:
         self._check(self.L.lsmps_download_types(self.h, _ptr(t)), "download_types")
         return t
 
+    def positions(self):
+        xyz = np.empty((max(self.n, 1), 3), np.float64)
+        self._check(self.L.lsmps_download_positions(self.h, _ptr(xyz)), "download_positions")
+        return xyz[: self.n]
+
     def neighbor_lists(self):
         row = np.empty(self.n + 1, np.int32)
         col = np.empty(max(self.pairs, 1), np.int32)
@@ -286,6 +297,37 @@ class Pipeline:
         x = np.empty(max(self.m, 1), np.float64)
         self._check(self.L.lsmps_download_solution(self.h, _ptr(x)), "download_solution")
         return x[: self.m]
+
+    # -- time stepping -------------------------------------------------------------
+    def neighbors_skin(self, re, skin, max_neighbors, l0, classify=True):
+        s = CnlStats()
+        t = TypeStats()
+        self._check(self.L.lsmps_build_neighbors_skin(self.h, re, skin, max_neighbors, l0, 1 if classify else 0,
+                                                      C.byref(s), C.byref(t)), "build_neighbors_skin")
+        self.pairs = s.pairs
+        return as_dict(s), as_dict(t)
+
+    def set_velocities(self, uvw):
+        u = np.ascontiguousarray(uvw, np.float64).reshape(-1, 3)
+        self._check(self.L.lsmps_set_velocities(self.h, _ptr(u), u.shape[0]), "set_velocities")
+
+    def append(self, xyz, types, uvw=None):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(types, np.uint8)
+        u = None if uvw is None else np.ascontiguousarray(uvw, np.float64).reshape(-1, 3)
+        self._check(self.L.lsmps_append_particles(self.h, _ptr(xyz), _ptr(t), _ptr(u), xyz.shape[0]), "append")
+        self.n += xyz.shape[0]
+
+    def advance(self, dt):
+        self._check(self.L.lsmps_advance(self.h, dt), "advance")
+
+    def remove_outside(self, lo, hi):
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        r = _u64()
+        self._check(self.L.lsmps_remove_outside(self.h, _ptr(lo), _ptr(hi), C.byref(r)), "remove_outside")
+        self.n -= r.value
+        return r.value
 
     # -- learned preconditioner --------------------------------------------------
     def features(self):
