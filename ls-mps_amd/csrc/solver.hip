@@ -31,16 +31,19 @@
 // stored in ascending order, so the 64 rows of a slice (64 neighboring
 // particles) gather neighboring entries.
 //
-// Mixed precision (T = float): matrix and Krylov vectors in FP32, every dot
-// product accumulated in FP64, y in FP64, and reliable updates: when the
-// recurrence residual has dropped 10x since the last replacement, or when
-// it meets the tolerance, k_bicg_true_residual recomputes r = b - A S y from
-// the FP64 matrix and FP64 y, so the stopping test is on the true FP64
-// residual. With SOLVE_RESTART the FP32 Krylov space then restarts from the
-// true residual (r0 = r, p = v = 0): each restart is one step of iterative
-// refinement whose inner FP32 solve reduces the residual 10x. (Without the
-// restart, the FP32 recurrences drift away from the replaced residual and
-// BiCGStab stalls on these ill-conditioned Laplacians.)
+// Mixed precision (matrix type TM = float): the iteration matrix A S in
+// FP32 (half the value bytes of every SpMV), the Krylov vectors, dots and y
+// in FP64, and reliable updates: when the recurrence residual has dropped
+// by `delta` since the last replacement, or when it meets the tolerance,
+// k_bicg_true_residual recomputes r = b - A S y from the FP64 matrix and
+// FP64 y, so the stopping test is on the true FP64 residual. With
+// SOLVE_RESTART the Krylov space then restarts from the true residual
+// (r0 = r, p = v = 0): each restart is one step of iterative refinement
+// against the FP32 matrix. The PPE is a graph Laplacian pinned at one row,
+// so its conditioning grows with the scene; with the vectors in FP32 as
+// well, BiCGStab's recurrences there lose enough accuracy that iteration
+// counts scatter over 1x-8x of the FP64 solver's and some solves never
+// converge, while an FP32 matrix with FP64 vectors tracks the FP64 count.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -107,41 +110,43 @@ __device__ __forceinline__ SellRow sell_row(const int* __restrict__ sptr, int ro
   return {b + (row - sl * SLICE), (sptr[sl + 1] - b) / SLICE};
 }
 
-template <typename T>
-__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_p(int m, const T* __restrict__ r, const T* __restrict__ v,
-                                                           T* __restrict__ p, const SolverScalars* sc) {
+__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_p(int m, const double* __restrict__ r,
+                                                           const double* __restrict__ v, double* __restrict__ p,
+                                                           const SolverScalars* sc) {
   if (!iter_active(sc)) return;
-  const T beta = (T)sc->beta, omega = (T)sc->omega;
+  const double beta = sc->beta, omega = sc->omega;
   for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS)
     p[i] = fma(beta, fma(-omega, v[i], p[i]), r[i]);
 }
 
-// y = A x on the sliced-ELL matrix, one thread per row; returns the row's value.
-template <typename T>
-__device__ __forceinline__ T sell_dot(const int* __restrict__ sptr, const int* __restrict__ col,
-                                      const T* __restrict__ val, const T* __restrict__ x, int row) {
+// (A x)_row on the sliced-ELL matrix (values TM, FP64 x, FP64 sum), one
+// thread per row.
+template <typename TM>
+__device__ __forceinline__ double sell_dot(const int* __restrict__ sptr, const int* __restrict__ col,
+                                           const TM* __restrict__ val, const double* __restrict__ x, int row) {
   const SellRow R = sell_row(sptr, row);
-  T acc = 0;
+  double acc = 0.0;
 #pragma unroll 4
   for (int k = 0; k < R.width; ++k) {
     const int e = R.base + k * SLICE;
-    acc = fma(val[e], x[col[e]], acc);
+    acc = fma((double)val[e], x[col[e]], acc);
   }
   return acc;
 }
 
-template <typename T>
+template <typename TM>
 __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_v(const int* __restrict__ sptr, const int* __restrict__ col,
-                                                           const T* __restrict__ val, int m, const T* __restrict__ p,
-                                                           const T* __restrict__ r0, T* __restrict__ v,
-                                                           SolverScalars* sc, double* __restrict__ partials) {
+                                                           const TM* __restrict__ val, int m,
+                                                           const double* __restrict__ p, const double* __restrict__ r0,
+                                                           double* __restrict__ v, SolverScalars* sc,
+                                                           double* __restrict__ partials) {
   if (!iter_active(sc)) return;
   __shared__ double lds[SOLVER_THREADS / WAVE];
   double dot = 0.0;
   for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
-    const T acc = sell_dot(sptr, col, val, p, row);
+    const double acc = sell_dot(sptr, col, val, p, row);
     v[row] = acc;
-    dot += (double)r0[row] * (double)acc;
+    dot += r0[row] * acc;
   }
   double vv[1] = {dot};
   block_partials<1>(vv, partials, lds);
@@ -163,30 +168,30 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_alpha(const double* __re
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_s(int m, const T* __restrict__ r, const T* __restrict__ v,
-                                                           T* __restrict__ s, const SolverScalars* sc) {
+__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_s(int m, const double* __restrict__ r,
+                                                           const double* __restrict__ v, double* __restrict__ s,
+                                                           const SolverScalars* sc) {
   if (!iter_active(sc)) return;
-  const T alpha = (T)sc->alpha;
+  const double alpha = sc->alpha;
   for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS)
     s[i] = fma(-alpha, v[i], r[i]);
 }
 
-template <typename T>
+template <typename TM>
 __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_t(const int* __restrict__ sptr, const int* __restrict__ col,
-                                                           const T* __restrict__ val, int m, const T* __restrict__ s,
-                                                           T* __restrict__ t, SolverScalars* sc,
-                                                           double* __restrict__ partials) {
+                                                           const TM* __restrict__ val, int m,
+                                                           const double* __restrict__ s, double* __restrict__ t,
+                                                           SolverScalars* sc, double* __restrict__ partials) {
   if (!iter_active(sc)) return;
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
   double ss = 0.0, tt = 0.0, ts = 0.0;
   for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
-    const T acc = sell_dot(sptr, col, val, s, row);
-    const T si = s[row];
+    const double acc = sell_dot(sptr, col, val, s, row);
+    const double si = s[row];
     t[row] = acc;
-    ss += (double)si * (double)si;
-    tt += (double)acc * (double)acc;
-    ts += (double)acc * (double)si;
+    ss += si * si;
+    tt += acc * acc;
+    ts += acc * si;
   }
   double vv[3] = {ss, tt, ts};
   block_partials<3>(vv, partials, lds);
@@ -212,11 +217,11 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_omega(const double* __re
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const T* __restrict__ p, const T* __restrict__ s,
-                                                            const T* __restrict__ t, const T* __restrict__ r0,
-                                                            T* __restrict__ r, double* __restrict__ y,
-                                                            SolverScalars* sc, double* __restrict__ partials) {
+__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const double* __restrict__ p,
+                                                            const double* __restrict__ s, const double* __restrict__ t,
+                                                            const double* __restrict__ r0, double* __restrict__ r,
+                                                            double* __restrict__ y, SolverScalars* sc,
+                                                            double* __restrict__ partials) {
   const int state = sc->state;
   const bool sconv = state == SOLVE_S_CONVERGED;
   if (!(sconv || iter_active(sc))) return;
@@ -225,14 +230,14 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const T* __re
   double rr = 0.0, r0r = 0.0;
   for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS) {
     if (sconv) {
-      y[i] += alpha * (double)p[i];
+      y[i] += alpha * p[i];
     } else {
-      const T si = s[i];
-      y[i] += alpha * (double)p[i] + omega * (double)si;
-      const T ri = fma((T)-omega, t[i], si);
+      const double si = s[i];
+      y[i] += alpha * p[i] + omega * si;
+      const double ri = fma(-omega, t[i], si);
       r[i] = ri;
-      rr += (double)ri * (double)ri;
-      r0r += (double)r0[i] * (double)ri;
+      rr += ri * ri;
+      r0r += r0[i] * ri;
     }
   }
   double vv[2] = {rr, r0r};
@@ -260,11 +265,12 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_iter(const double* __res
 // = 0 and the scalars of a fresh solve. Otherwise runs only when a
 // replacement is pending: sets rho/beta, stops when the true residual meets
 // the tolerance, and with SOLVE_RESTART restarts the Krylov space.
-template <typename T, bool INIT>
+template <bool INIT>
 __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_true_residual(
     const int* __restrict__ sptr, const int* __restrict__ col, const double* __restrict__ val, int m,
-    const double* __restrict__ b, const double* __restrict__ scale, const double* __restrict__ y, T* __restrict__ r,
-    T* __restrict__ r0, T* __restrict__ p, T* __restrict__ v, SolverScalars* sc, double* __restrict__ partials) {
+    const double* __restrict__ b, const double* __restrict__ scale, const double* __restrict__ y,
+    double* __restrict__ r, double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v,
+    SolverScalars* sc, double* __restrict__ partials) {
   if (!INIT && sc->replace == 0) return;
   const bool restart = !INIT && (sc->flags & SOLVE_RESTART);
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
@@ -281,21 +287,20 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_true_residual(
     }
     const double bi = b[row];
     const double ri = bi - acc;
-    const T rt = (T)ri;
-    r[row] = rt;
+    r[row] = ri;
     if (INIT) {
-      r0[row] = rt;
-      p[row] = v[row] = (T)0;
+      r0[row] = ri;
+      p[row] = v[row] = 0.0;
       bb += bi * bi;
-      rr += (double)rt * (double)rt;
-    } else if (restart) {
-      r0[row] = rt;
-      p[row] = v[row] = (T)0;
       rr += ri * ri;
-      r0r += (double)rt * (double)rt;
+    } else if (restart) {
+      r0[row] = ri;
+      p[row] = v[row] = 0.0;
+      rr += ri * ri;
+      r0r += ri * ri;
     } else {
       rr += ri * ri;
-      r0r += (double)r0[row] * (double)rt;
+      r0r += r0[row] * ri;
     }
   }
   double vv[3] = {rr, r0r, bb};
@@ -456,40 +461,40 @@ void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st) {
   if (m > 0) hipLaunchKernelGGL(k_inv_diag, dim3(blocks(m, 256)), dim3(256), 0, st, diag, m, s);
 }
 
-template <typename T>
-void launch_solver_init(const SolverBufs<T>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,
+template <typename TM>
+void launch_solver_init(const SolverBufs<TM>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,
                         int flags, double delta, hipStream_t st) {
   const int m = B.m;
   hipLaunchKernelGGL(k_solver_setup, dim3(1), dim3(64), 0, st, sc, max_iter, tol, flags, delta);
   if (m > 0) hipLaunchKernelGGL(k_y_from_x, dim3(blocks(m, 256)), dim3(256), 0, st, x0, B.scale, m, B.y);
   const unsigned g = solver_grid(m);
-  hipLaunchKernelGGL((k_bicg_true_residual<T, true>), dim3(g), dim3(SOLVER_THREADS), 0, st, B.sptr, B.col, B.val64, m,
+  hipLaunchKernelGGL(k_bicg_true_residual<true>, dim3(g), dim3(SOLVER_THREADS), 0, st, B.sptr, B.col, B.val64, m,
                      B.b, B.scale, x0 ? (const double*)B.y : nullptr, B.r, B.r0, B.p, B.v, sc, B.partials);
   hipLaunchKernelGGL(k_fin_residual<true>, dim3(1), dim3(SOLVER_THREADS), 0, st, B.partials, (int)g, sc);
 }
 
-template <typename T>
-void launch_solver_iteration(const SolverBufs<T>& B, bool with_replace, SolverScalars* sc, hipStream_t st) {
+template <typename TM>
+void launch_solver_iteration(const SolverBufs<TM>& B, bool with_replace, SolverScalars* sc, hipStream_t st) {
   const int m = B.m;
   const unsigned g = solver_grid(m);
   const dim3 one(1), blk(SOLVER_THREADS), grid(g);
-  hipLaunchKernelGGL((k_bicg_p<T>), grid, blk, 0, st, m, B.r, B.v, B.p, sc);
-  hipLaunchKernelGGL((k_bicg_v<T>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.p, B.r0, B.v, sc, B.partials);
+  hipLaunchKernelGGL(k_bicg_p, grid, blk, 0, st, m, B.r, B.v, B.p, sc);
+  hipLaunchKernelGGL((k_bicg_v<TM>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.p, B.r0, B.v, sc, B.partials);
   hipLaunchKernelGGL(k_fin_alpha, one, blk, 0, st, B.partials, (int)g, sc);
-  hipLaunchKernelGGL((k_bicg_s<T>), grid, blk, 0, st, m, B.r, B.v, B.s, sc);
-  hipLaunchKernelGGL((k_bicg_t<T>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.s, B.t, sc, B.partials);
+  hipLaunchKernelGGL(k_bicg_s, grid, blk, 0, st, m, B.r, B.v, B.s, sc);
+  hipLaunchKernelGGL((k_bicg_t<TM>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.s, B.t, sc, B.partials);
   hipLaunchKernelGGL(k_fin_omega, one, blk, 0, st, B.partials, (int)g, sc);
-  hipLaunchKernelGGL((k_bicg_xr<T>), grid, blk, 0, st, m, B.p, B.s, B.t, B.r0, B.r, B.y, sc, B.partials);
+  hipLaunchKernelGGL(k_bicg_xr, grid, blk, 0, st, m, B.p, B.s, B.t, B.r0, B.r, B.y, sc, B.partials);
   hipLaunchKernelGGL(k_fin_iter, one, blk, 0, st, B.partials, (int)g, sc);
   if (with_replace) {
-    hipLaunchKernelGGL((k_bicg_true_residual<T, false>), grid, blk, 0, st, B.sptr, B.col, B.val64, m, B.b, B.scale,
+    hipLaunchKernelGGL(k_bicg_true_residual<false>, grid, blk, 0, st, B.sptr, B.col, B.val64, m, B.b, B.scale,
                        (const double*)B.y, B.r, B.r0, B.p, B.v, sc, B.partials);
     hipLaunchKernelGGL(k_fin_residual<false>, one, blk, 0, st, B.partials, (int)g, sc);
   }
 }
 
-template <typename T>
-void launch_solver_final(const SolverBufs<T>& B, const double* xe, double* x, SolverScalars* sc, hipStream_t st) {
+template <typename TM>
+void launch_solver_final(const SolverBufs<TM>& B, const double* xe, double* x, SolverScalars* sc, hipStream_t st) {
   const int m = B.m;
   const unsigned g = solver_grid(m);
   hipLaunchKernelGGL(k_bicg_final, dim3(g), dim3(SOLVER_THREADS), 0, st, B.sptr, B.col, B.val64, m, B.b, B.scale,

@@ -34,23 +34,26 @@ struct SolverScalars {
 constexpr unsigned SOLVER_MAX_GRID = 1024;  // 4 blocks of 256 per CU on 256 CUs, grid-stride
 
 // Iteration matrix in sliced-ELL form (slices of 64 rows, column-major,
-// padded with (row, 0)), plus the vectors of one solve.
-template <typename T>
+// padded with (row, 0)) with values in precision TM, plus the vectors of
+// one solve (always FP64: Krylov recurrences kept in FP32 stall or diverge
+// on the pinned graph Laplacians of the PPE, while an FP32 matrix with FP64
+// vectors converges like the FP64 solver).
+template <typename TM>
 struct SolverBufs {
   int m = 0;
   const int* sptr = nullptr;       // slice offsets, ceil(m/64) + 1 entries
   const int* col = nullptr;        // padded columns
-  const T* val = nullptr;          // A S in precision T
+  const TM* val = nullptr;         // A S in precision TM
   const double* val64 = nullptr;   // A in FP64 (true residuals), same layout
   const double* b = nullptr;
   const double* scale = nullptr;   // S (nullptr = identity)
   double* y = nullptr;             // FP64 iterate of A S y = b
-  T* r = nullptr;
-  T* r0 = nullptr;
-  T* p = nullptr;
-  T* v = nullptr;
-  T* s = nullptr;
-  T* t = nullptr;
+  double* r = nullptr;
+  double* r0 = nullptr;
+  double* p = nullptr;
+  double* v = nullptr;
+  double* s = nullptr;
+  double* t = nullptr;
   double* partials = nullptr;      // >= 3 * SOLVER_MAX_GRID doubles
 };
 
@@ -60,12 +63,12 @@ template <typename T>
 void launch_sell_scale(const int* col, const double* val64, const double* scale, int64_t nnz, T* val,
                        hipStream_t st);
 void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st);
-template <typename T>
-void launch_solver_init(const SolverBufs<T>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,
+template <typename TM>
+void launch_solver_init(const SolverBufs<TM>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,
                         int flags, double delta, hipStream_t st);
-template <typename T>
-void launch_solver_iteration(const SolverBufs<T>& B, bool with_replace, SolverScalars* sc, hipStream_t st);
-template <typename T>
-void launch_solver_final(const SolverBufs<T>& B, const double* xe, double* x, SolverScalars* sc, hipStream_t st);
+template <typename TM>
+void launch_solver_iteration(const SolverBufs<TM>& B, bool with_replace, SolverScalars* sc, hipStream_t st);
+template <typename TM>
+void launch_solver_final(const SolverBufs<TM>& B, const double* xe, double* x, SolverScalars* sc, hipStream_t st);
 
 }  // namespace lsmps_hip
