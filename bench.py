@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--particles", type=int, default=3_000_000)
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--solvers", default="fp64_jacobi,mixed,learned_mixed",
+                    help="comma list of the PPE solves to time (fp64_jacobi, mixed, learned_mixed)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-max-iter", type=int, default=40000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -242,9 +244,10 @@ def main():
                         "algorithmic_GBs": round(asm_bytes / (pr["ms"] * 1e-3) / 1e9, 1)}
         log(f"assembly {pr}")
         solves = {}
-        methods = [("fp64_jacobi", L.SOLVE_JACOBI), ("mixed", L.SOLVE_MIXED)]
+        wanted = set(args.solvers.split(","))
+        methods = [(k, m) for k, m in (("fp64_jacobi", L.SOLVE_JACOBI), ("mixed", L.SOLVE_MIXED)) if k in wanted]
         weights = os.path.join(ROOT, "ls-mps_amd", "weights", "mlp_bulk_lattice.npy")
-        if os.path.exists(weights):
+        if os.path.exists(weights) and "learned_mixed" in wanted:
             from lsmps_amd import learn
 
             p.set_params(learn.load_params(weights))
@@ -258,7 +261,7 @@ def main():
                             "fallback": st["fallback"], "initial_residual": st["initial_residual"],
                             "first_call_ms": round(st0["ms"], 3)}
         for name in ("mixed", "learned_mixed"):
-            if name in solves and solves[name]["ms"] > 0:
+            if name in solves and "fp64_jacobi" in solves and solves[name]["ms"] > 0:
                 solves[f"speedup_{name}_vs_fp64"] = round(solves["fp64_jacobi"]["ms"] / solves[name]["ms"], 3)
         extra["ppe_time_to_1e-8"] = solves
     p.close()

@@ -154,9 +154,30 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--report", default=None, help="also write the evaluation as JSON here")
+    ap.add_argument("--eval-only", action="store_true",
+                    help="evaluate the weights at --out on the held-out seeds (no training); needs --report")
     args = ap.parse_args(argv)
     t0 = time.time()
     pipe = Pipeline(0)
+    if args.eval_only:
+        if not args.report:
+            ap.error("--eval-only needs --report")
+        params = load_params(args.out)
+        pipe.set_params(params)
+        eval_seeds = list(range(1001, 1001 + args.eval_seeds))
+        rows = evaluate(pipe, eval_seeds, args.eval_particles)
+        summary = summarize(rows)
+        meta_path = os.path.splitext(args.out)[0] + ".json"
+        meta = {}
+        if os.path.exists(meta_path):
+            with open(meta_path) as f:
+                meta = json.load(f)
+        meta["eval"] = {"seeds": eval_seeds, "particles": args.eval_particles, "tol": 1e-8, "summary": summary}
+        with open(args.report, "w") as f:
+            json.dump({"meta": meta, "rows": rows}, f, indent=1)
+        log(f"evaluated {args.out}; summary {json.dumps(summary)}; {time.time() - t0:.0f} s")
+        pipe.close()
+        return
     pipe.set_params(init_params(0))
     train_seeds = list(range(1, args.train_seeds + 1))
     n_samples = collect(pipe, train_seeds, stride=args.stride)

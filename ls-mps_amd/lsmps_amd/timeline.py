@@ -30,11 +30,12 @@ every step and the neighbor lists rebuilt on a cadence:
             motion over one cadence below skin / 2;
   per step  WLS (mixed precision) of a manufactured quadratic, PPE assembly
             over all fluid unknowns with a seeded pressure, and the solve to
-            a true relative residual of 1e-8 (learned start and diagonal in
-            the mixed solver, with the plain mixed method as the fallback,
-            when weights are given; else plain mixed).
+            a true relative residual of 1e-8: by default the learned start
+            and diagonal in the mixed solver, with the plain mixed method as
+            the fallback (--method mixed / jacobi for the plain mixed or the
+            FP64 Jacobi solver).
 
-    python -m lsmps_amd.timeline --particles 1500000 --steps 200
+    python -m lsmps_amd.timeline --particles 1500000 --steps 200 [--method jacobi]
 
 prints one JSON summary: p50 / p99 / max of the step wall time and of each
 stage, particle counts and solver statistics.
@@ -48,7 +49,8 @@ import time
 
 import numpy as np
 
-from . import C1_FIELD, INTERIOR, SOLVE_LEARNED, SOLVE_MIXED, WALL, WLS_MIXED, XSTAR_RANDOM, Pipeline
+from . import (C1_FIELD, INTERIOR, SOLVE_JACOBI, SOLVE_LEARNED, SOLVE_MIXED, WALL, WLS_MIXED, XSTAR_RANDOM,
+               Pipeline)
 
 
 def log(*a):
@@ -122,7 +124,7 @@ class DropletSites:
 
 
 def run(particles=1_500_000, steps=200, inject=2000, rebuild=10, l0=0.004, speed=1.0, seed=5, maxn=128,
-        weights=None, tol=1e-8, max_iter=20000, pipe=None, verbose=True):
+        weights=None, tol=1e-8, max_iter=20000, pipe=None, verbose=True, method=None):
     re = 2.1 * l0
     skin = 0.2 * re
     # The largest relative speed of two particles within re + skin is U (the
@@ -138,10 +140,12 @@ def run(particles=1_500_000, steps=200, inject=2000, rebuild=10, l0=0.004, speed
     p.upload(xyz, types)
     p.set_velocities(u)
     del xyz, types, layer, u
-    method = SOLVE_MIXED
-    if weights is not None:
+    if method is None:
+        method = SOLVE_LEARNED if weights is not None else SOLVE_MIXED
+    if method == SOLVE_LEARNED:
+        if weights is None:
+            raise ValueError("the learned method needs weights")
         p.set_params(weights)
-        method = SOLVE_LEARNED
     sites = DropletSites(lx, ly, l0, speed, seed)
     du = np.array([LAYER_SPEED[DROPLET_LAYER] * speed, 0.0, 0.0])
     box_lo = np.array([-1e30, -1e30, -1e30])
@@ -183,7 +187,7 @@ def run(particles=1_500_000, steps=200, inject=2000, rebuild=10, l0=0.004, speed
         "scene": f"gap between walls 4 l0 apart, film on 2 layers, droplet layer open; l0={l0}, re=2.1 l0, "
                  f"skin=0.2 re (lists to re + skin, stages filter r < re)",
         "steps": steps, "inject_per_step": inject, "rebuild_every": rebuild, "dt": dt, "speed": speed,
-        "method": "learned+mixed" if method == SOLVE_LEARNED else "mixed",
+        "method": {SOLVE_LEARNED: "learned+mixed", SOLVE_MIXED: "mixed", SOLVE_JACOBI: "fp64 jacobi"}[method],
         "particles_start": n0, "particles_end": int(p.n), "injected": injected, "removed": removed,
         "all_converged": bool(all(r["converged"] for r in rows)),
         "fallbacks": int(sum(r["fallback"] for r in rows)),
@@ -207,15 +211,17 @@ def main(argv=None):
     ap.add_argument("--rebuild", type=int, default=10)
     ap.add_argument("--weights", default=None,
                     help="learned-preconditioner weights (.npy); default: the weights shipped with the package")
-    ap.add_argument("--mixed", action="store_true", help="plain mixed solver, no learned start")
+    ap.add_argument("--method", choices=("learned", "mixed", "jacobi"), default="learned",
+                    help="learned start + mixed solver (default), plain mixed, or FP64 Jacobi BiCGStab")
     ap.add_argument("--out", default=None, help="write the per-step rows as JSON here")
     args = ap.parse_args(argv)
     w = None
-    if not args.mixed:
+    if args.method == "learned":
         from .learn import DEFAULT_WEIGHTS, load_params
 
         w = load_params(args.weights or DEFAULT_WEIGHTS)
-    summary, rows = run(args.particles, args.steps, args.inject, args.rebuild, weights=w)
+    method = {"learned": SOLVE_LEARNED, "mixed": SOLVE_MIXED, "jacobi": SOLVE_JACOBI}[args.method]
+    summary, rows = run(args.particles, args.steps, args.inject, args.rebuild, weights=w, method=method)
     if args.out:
         with open(args.out, "w") as f:
             json.dump({"summary": summary, "rows": rows}, f, indent=1)
