@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LSMPS_HIP_ABI_VERSION 3
+#define LSMPS_HIP_ABI_VERSION 4
 
 enum {
   LSMPS_OK = 0,
@@ -145,14 +145,30 @@ typedef struct {
   float ms;
   float kernel_ms;     /* GPU time of the fill kernel alone */
   uint64_t padded_nnz; /* stored entries of the sliced-ELL matrix (nnz + padding) */
+  uint64_t graph_rows; /* lsmps_assemble_ppe_wls: rows that took the graph-Laplacian row */
 } lsmps_ppe_result;
 int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_result* out);
+/* WLS-Laplacian rows in place of the graph Laplacian: the reference's
+ * stated next PPE (README.md:34), built as its 2-D model builds it
+ * (python/rain_plate_lsmps_cpu.py:204-253), in 3-D with the reference's
+ * weight, basis and Cholesky ladder (src/lsmps.cpp:22-25, l0 and reg as in
+ * wls_test). Same unknowns, layout, x* and b = A x* as lsmps_assemble_ppe.
+ * Row i is minus the WLS Laplacian (the graph Laplacian's sign): -c_ij for
+ * each unknown neighbor in list order, c_ij = (z . q_ij) w_ij / r_ij / ws /
+ * l0 with z = M^-1 (0,0,0,1,1,1,0,0,0), then the diagonal d = sum_j c_ij
+ * over all weighted neighbors (non-unknown neighbors are p = 0). A row
+ * whose WLS fails, or with d <= 0 or d < 0.25 sum |c_ij| (a one-sided
+ * stencil), takes the graph-Laplacian row of lsmps_assemble_ppe instead
+ * (graph_rows); row 0 is not pinned. */
+int lsmps_assemble_ppe_wls(lsmps_ctx* ctx, const lsmps_ppe_params* p, double l0, double reg,
+                           lsmps_ppe_result* out);
 
 /* ---- BiCGStab: replaces solve_ppe (src/lsmps.cpp:65-67) ---------------- */
 enum {
   LSMPS_SOLVE_PLAIN = 0,   /* FP64, unpreconditioned, the reference's stopping rule */
   LSMPS_SOLVE_JACOBI = 1,  /* FP64, right Jacobi, true-residual check */
-  LSMPS_SOLVE_MIXED = 2,   /* FP32 Krylov + FP64 reliable updates, right Jacobi */
+  LSMPS_SOLVE_MIXED = 2,   /* FP32 matrix, FP64 Krylov vectors, FP64 reliable
+                              updates and restarts, right Jacobi */
   LSMPS_SOLVE_LEARNED = 3, /* MIXED with the MLP diagonal and initial guess
                               (lsmps_mlp_set_params first); plain MIXED when
                               the guess raises the residual */

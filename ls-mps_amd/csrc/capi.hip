@@ -878,12 +878,20 @@ int lsmps_wls_sampled_stats(lsmps_ctx* ctx, int32_t samples, lsmps_wls_stats* ou
   GUARD_END
 }
 
-int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_result* out) {
-  if (!ctx || !p) return LSMPS_E_ARG;
+}  // extern "C"
+
+namespace {
+
+// Shared by both row kinds: unknown numbering, slice widths, then the fill
+// (graph Laplacian, or WLS Laplacian when wls).
+int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0, double reg,
+                 lsmps_ppe_result* out) {
   if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "PPE assembly needs neighbors");
   if (p->xmode != LSMPS_XSTAR_QUADRATIC && p->xmode != LSMPS_XSTAR_RANDOM) return fail(ctx, LSMPS_E_ARG, "bad xmode");
   if (!(p->re > 0) || !std::isfinite(p->re)) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
-  GUARD_BEGIN
+  if (p->re > ctx->re) return fail(ctx, LSMPS_E_ARG, "re exceeds the radius of the neighbor lists");
+  if (wls && (!(l0 > 0) || !std::isfinite(l0) || !(reg >= 0) || !std::isfinite(reg)))
+    return fail(ctx, LSMPS_E_ARG, "WLS rows need l0 > 0 and reg >= 0, finite");
   HIPCHK(hipSetDevice(ctx->device));
   const int n = (int)ctx->n_cnl;
   hipStream_t st = ctx->st;
@@ -918,7 +926,8 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
   HIPCHK(ctx->sell_ptr.ensure((size_t)ns + 1));
   launch_unknown_maps(ctx->perm.p, ctx->fs.p, ctx->upos.p, ctx->pos.p, n, ctx->uid.p, ctx->uslot.p, ctx->urr.p,
                       ctx->rru.p, st);
-  launch_ppe_count(ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, m, ctx->sell_w.p, st);
+  launch_ppe_count(ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, m, !wls, ctx->sell_w.p,
+                   st);
   HIPCHK(ctx->scan_scr.ensure(std::max(scan_scratch_elems((int64_t)n + 1), scan_scratch_elems((int64_t)ns + 1))));
   launch_exclusive_scan(ctx->sell_w.p, ns, ctx->sell_ptr.p, ctx->scan_scr.p, st);
   if (int rc = check_launch(ctx, "ppe_count")) return rc;
@@ -933,12 +942,19 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
   HIPCHK(ctx->sell_v64.ensure(ctx->sell_nnz));
   HIPCHK(ctx->bpart.ensure(3 * (size_t)ns + 1));
   zero_counters(ctx, 16);
-  launch_ppe_fill(ctx->spos.p, ctx->type_s.p, ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn,
-                  m, p->re, p->shift, p->xmode, p->xseed, ctx->sell_ptr.p, ctx->sell_col.p, ctx->sell_v64.p,
-                  ctx->rlen.p, ctx->diag.p, ctx->xe.p, ctx->b.p, ctx->counters.p, ctx->bpart.p, st, ctx->e2, ctx->e3);
+  if (wls)
+    launch_ppe_wls_fill(ctx->spos.p, ctx->type_s.p, ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p,
+                        ctx->maxn, m, p->re, l0, reg, p->shift, p->xmode, p->xseed, ctx->sell_ptr.p, ctx->sell_col.p,
+                        ctx->sell_v64.p, ctx->rlen.p, ctx->diag.p, ctx->xe.p, ctx->b.p, ctx->counters.p, ctx->bpart.p,
+                        st, ctx->e2, ctx->e3);
+  else
+    launch_ppe_fill(ctx->spos.p, ctx->type_s.p, ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p,
+                    ctx->maxn, m, p->re, p->shift, p->xmode, p->xseed, ctx->sell_ptr.p, ctx->sell_col.p,
+                    ctx->sell_v64.p, ctx->rlen.p, ctx->diag.p, ctx->xe.p, ctx->b.p, ctx->counters.p, ctx->bpart.p, st,
+                    ctx->e2, ctx->e3);
   if (int rc = check_launch(ctx, "ppe_fill")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
-  unsigned long long c[3];
+  unsigned long long c[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   ctx->m = (uint64_t)m;
@@ -956,8 +972,26 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
     out->ms = elapsed(ctx);
     out->kernel_ms = m > 0 ? kernel_elapsed(ctx) : 0.0f;
     out->padded_nnz = ctx->sell_nnz;
+    out->graph_rows = wls ? c[3] : 0;
   }
   return LSMPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_result* out) {
+  if (!ctx || !p) return LSMPS_E_ARG;
+  GUARD_BEGIN
+  return assemble_ppe(ctx, p, false, 0.0, 0.0, out);
+  GUARD_END
+}
+
+int lsmps_assemble_ppe_wls(lsmps_ctx* ctx, const lsmps_ppe_params* p, double l0, double reg, lsmps_ppe_result* out) {
+  if (!ctx || !p) return LSMPS_E_ARG;
+  GUARD_BEGIN
+  return assemble_ppe(ctx, p, true, l0, reg, out);
   GUARD_END
 }
 

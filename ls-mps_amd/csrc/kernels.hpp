@@ -90,13 +90,22 @@ void launch_unknown_flags(const uint8_t* type_s, const int* iperm, int n, int* f
 void launch_unknown_slots(const int* perm, const int* flag, const int* pos, int n, int cap, int* fs, hipStream_t st);
 void launch_unknown_maps(const int* perm, const int* fs, const int* upos, const int* pos, int n, int* uid, int* uslot,
                          int* urr, int* rru, hipStream_t st);
+// pin0: row 0 is the identity (graph-Laplacian rows), so its width is 1.
 void launch_ppe_count(const int* uslot, const int* urr, const int* uid, const int* nbr, const int* ncnt, int maxn,
-                      int m, int* wsz, hipStream_t st);
+                      int m, bool pin0, int* wsz, hipStream_t st);
 void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr, const int* uid,
                      const int* nbr, const int* ncnt, int maxn, int m, double re, double shift, int xmode,
                      uint32_t xseed, const int* sptr, int* col, double* val, int* rlen, double* diag, double* xe,
                      double* b, unsigned long long* counters, unsigned* bpart, hipStream_t st,
                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+// WLS-Laplacian rows into the same sliced-ELL layout; counters as
+// launch_ppe_fill, plus [3] rows that took the graph-Laplacian row. bpart:
+// 4 * ceil(ceil(m / 64) / 2) words.
+void launch_ppe_wls_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr,
+                         const int* uid, const int* nbr, const int* ncnt, int maxn, int m, double re, double l0,
+                         double reg, double shift, int xmode, uint32_t xseed, const int* sptr, int* col, double* val,
+                         int* rlen, double* diag, double* xe, double* b, unsigned long long* counters,
+                         unsigned* bpart, hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 void launch_rr_lengths(const int* rru, const int* rlen, int m, int* cnt_rr, hipStream_t st);
 void launch_sell_to_csr(const int* rru, const int* urr, const int* sptr, const int* scol, const double* sval,
                         const int* rlen, const int* prow, int m, int* pcol, double* pval, hipStream_t st);

@@ -29,7 +29,31 @@
 // broadcasting each kept entry to the row's lanes in order, so they are
 // bit-identical to the reference (this file is built with
 // -ffp-contract=off).
+//
+// WLS-Laplacian rows (k_ppe_wls_fill). The reference's C++ has only the
+// graph Laplacian; its stated next step is PPE rows from the WLS operator
+// (README.md:34), which its 2-D model builds (python/rain_plate_lsmps_cpu.py:
+// 204-253). In 3-D with the reference's weight, basis and Cholesky ladder
+// (src/lsmps.cpp:22-25,28): the WLS Laplacian of a field f at particle i is
+// (x3 + x4 + x5) / l0 with x = M^-1 sum_j w_j q_j (f_j - f_i) / (r_j ws), so
+// it is sum_j c_ij (f_j - f_i) with c_ij = (z . q_j) w_j / r_j / ws / l0 and
+// z = M^-1 (0,0,0,1,1,1,0,0,0). Row i is minus that operator, the sign of
+// the reference's graph Laplacian: -c_ij for each unknown neighbor (list
+// order) and the diagonal d = sum_j c_ij over every neighbor with w > 0;
+// neighbors that are not unknowns (FreeSurface, Splash, Wall) are p = 0
+// (Dirichlet).
+//
+// A one-sided neighborhood (a surface particle the sector rule typed
+// Interior, a plate edge) gives a stencil that is no Laplacian: d <= 0 or
+// off-diagonals that outweigh it, which makes the matrix indefinite (on a
+// wall film, Jacobi-scaled eigenvalues down to -32; BiCGStab and GMRES
+// both fail). Such a row -- WLS failed, d not finite, d <= 0 or
+// d < WLS_ROW_DOMINANCE * sum |c_ij| over the unknown neighbors -- takes the
+// reference's graph-Laplacian row instead (-w/r^2 per unknown neighbor,
+// their sum on the diagonal, 1 when that is not positive). One thread per
+// row, FP64, no contraction: bit-identical to the oracle's restatement.
 #include "kernels.hpp"
+#include "wls_core.hpp"
 
 namespace lsmps_hip {
 
@@ -118,14 +142,14 @@ __device__ __forceinline__ PpeRow ppe_row(int m) {
 // and the slice widths wsz[slice] = 64 * longest row.
 __global__ __launch_bounds__(PPE_THREADS) void k_ppe_count(const int* __restrict__ uslot, const int* __restrict__ urr,
                                                            const int* __restrict__ uid, const int* __restrict__ nbr,
-                                                           const int* __restrict__ ncnt, int maxn, int m,
+                                                           const int* __restrict__ ncnt, int maxn, int m, int pin0,
                                                            int* __restrict__ wsz) {
   __shared__ int wmax;
   if (threadIdx.x == 0) wmax = 0;
   __syncthreads();
   const PpeRow R = ppe_row(m);
   int c = 0;
-  if (R.live && urr[R.u] != 0) {
+  if (R.live && (!pin0 || urr[R.u] != 0)) {
     const int s = uslot[R.u];
     const int* row = nbr + (size_t)s * maxn;
     const int k = ncnt[s];
@@ -240,6 +264,174 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
   block_count_partials<3>(v, bpart, lds);
 }
 
+// z = M^-1 (0,0,0,1,1,1,0,0,0) for the particle at pi with the reference's
+// moments (list order) and Cholesky ladder; false when the weight sum is not
+// positive or every rung fails.
+__device__ __noinline__ bool wls_lap_weights(const double4 pi, const int* __restrict__ row, int k,
+                                             const double4* __restrict__ spos, double re, double l0, double reg,
+                                             double (&z)[9], double* ws_out) {
+  const double h2 = 2 * l0;
+  for (int attempt = 0; attempt < 5; ++attempt) {
+    double M[45], ws = 0.0;
+#pragma unroll
+    for (int e = 0; e < 45; ++e) M[e] = 0.0;
+    for (int q = 0; q < k; ++q) {
+      const double4 pj = spos[row[q]];
+      const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      if (r <= 1e-12 || r >= re) continue;
+      const double tq = re / r - 1.0;
+      const double w = tq * tq;
+      if (w == 0.0) continue;
+      const double inv = 1.0 / r, h = h2 * r, g = l0 * r;
+      const double qv[9] = {dx * inv,    dy * inv,    dz * inv,    dx * dx / h, dy * dy / h,
+                            dz * dz / h, dx * dy / g, dx * dz / g, dy * dz / g};
+      ws += w;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+        const double wq = w * qv[a];
+#pragma unroll
+        for (int c = 0; c <= a; ++c) M[tri(a, c)] += wq * qv[c];
+      }
+    }
+    if (ws <= 0) return false;
+#pragma unroll
+    for (int e = 0; e < 45; ++e) M[e] /= ws;
+    const double add = attempt == 0 ? 0.0 : attempt == 1 ? reg * 1.0 : attempt == 2 ? reg * 10.0
+                                                            : attempt == 3 ? reg * 100.0 : reg * 1000.0;
+    if (attempt > 0 && add == 0.0) return false;
+    if (add != 0.0) {
+#pragma unroll
+      for (int a = 0; a < 9; ++a) M[tri(a, a)] += add;
+    }
+    const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!chol9<double>(M, 1e-14, 0.0, zero9)) continue;
+    const double e3[9] = {0, 0, 0, 1, 1, 1, 0, 0, 0};
+    chol9_solve<double>(M, e3, z);
+    *ws_out = ws;
+    return true;
+  }
+  return false;
+}
+
+constexpr int PPE_WLS_THREADS = 128;
+constexpr double WLS_ROW_DOMINANCE = 0.25;
+// One thread per stored row u < mpad (all rows of all slices). Counts per
+// block into bpart: [0] rows whose diagonal was set to 1, [1] NearWall
+// rows, [2] nonzeros, [3] rows that took the graph-Laplacian row.
+__global__ __launch_bounds__(PPE_WLS_THREADS) void k_ppe_wls_fill(
+    const double4* __restrict__ spos, const uint8_t* __restrict__ type_s, const int* __restrict__ uslot,
+    const int* __restrict__ urr, const int* __restrict__ uid, const int* __restrict__ nbr,
+    const int* __restrict__ ncnt, int maxn, int m, int mpad, double re, double l0, double reg, double shift,
+    int xmode, uint32_t xseed, const int* __restrict__ sptr, int* __restrict__ col, double* __restrict__ val,
+    int* __restrict__ rlen, double* __restrict__ diag, double* __restrict__ xe, double* __restrict__ b,
+    unsigned* __restrict__ bpart) {
+  __shared__ unsigned lds[4 * PPE_WLS_THREADS / WAVE];
+  const int u = (int)xcd_block(blockIdx.x, gridDim.x) * PPE_WLS_THREADS + threadIdx.x;
+  int failed = 0, neumann = 0, len = 0, graph = 0;
+  if (u < mpad) {
+    const int sl = u / PPE_SLICE;
+    const int sb = sptr[sl];
+    const int width = (sptr[sl + 1] - sb) / PPE_SLICE;
+    const int base = sb + (u - sl * PPE_SLICE);
+    if (u < m) {
+      const int s = uslot[u];
+      const int rr = urr[u];
+      const double4 pi = spos[s];
+      const double xu = xmode == 0 ? pi.x * pi.x + pi.y * pi.y : xstar_random(xseed, rr);
+      const int* row = nbr + (size_t)s * maxn;
+      const int k = ncnt[s];
+      double z[9], ws = 0.0;
+      bool ok = wls_lap_weights(pi, row, k, spos, re, l0, reg, z, &ws);
+      double d = 0.0, bs = 0.0, offabs = 0.0;
+      int kept = 0, below_u = 0;
+      if (ok) {
+        const double h2 = 2 * l0;
+        for (int q = 0; q < k; ++q) {
+          const int s2 = row[q];
+          const double4 pj = spos[s2];
+          const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+          const double r = sqrt(dx * dx + dy * dy + dz * dz);
+          if (r <= 1e-12 || r >= re) continue;
+          const double tq = re / r - 1.0;
+          const double w = tq * tq;
+          if (w == 0.0) continue;
+          const double inv = 1.0 / r, h = h2 * r, g = l0 * r;
+          const double qv[9] = {dx * inv,    dy * inv,    dz * inv,    dx * dx / h, dy * dy / h,
+                                dz * dz / h, dx * dy / g, dx * dz / g, dy * dz / g};
+          double dot = 0.0;
+#pragma unroll
+          for (int a = 0; a < 9; ++a) dot += z[a] * qv[a];
+          const double c = dot * w / r / ws / l0;
+          d += c;
+          const int cj = uid[s2];
+          if (cj >= 0) {
+            // Columns above u sit one slot later: the diagonal takes slot below_u.
+            const int slot = kept + (cj > u ? 1 : 0);
+            col[base + slot * PPE_SLICE] = cj;
+            val[base + slot * PPE_SLICE] = -c;
+            const double xj = xmode == 0 ? pj.x * pj.x + pj.y * pj.y : xstar_random(xseed, urr[cj]);
+            bs += -c * xj;
+            offabs += fabs(c);
+            ++kept;
+            below_u += cj < u ? 1 : 0;
+          }
+        }
+        ok = isfinite(d) && d > 0.0 && d >= WLS_ROW_DOMINANCE * offabs;
+      }
+      if (!ok) {
+        // The reference's graph-Laplacian row (src/lsmps.cpp:41-51).
+        graph = 1;
+        d = 0.0;
+        bs = 0.0;
+        kept = 0;
+        below_u = 0;
+        for (int q = 0; q < k; ++q) {
+          const int s2 = row[q];
+          const int cj = uid[s2];
+          if (cj < 0) continue;
+          const double4 pj = spos[s2];
+          double a;
+          if (!pair_coef(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, re, &a)) continue;
+          const int slot = kept + (cj > u ? 1 : 0);
+          col[base + slot * PPE_SLICE] = cj;
+          val[base + slot * PPE_SLICE] = -a;
+          const double xj = xmode == 0 ? pj.x * pj.x + pj.y * pj.y : xstar_random(xseed, urr[cj]);
+          d += a;
+          bs += -a * xj;
+          ++kept;
+          below_u += cj < u ? 1 : 0;
+        }
+        if (d <= 0 || !isfinite(d)) {
+          failed = 1;
+          d = 1.0;
+        }
+      }
+      if (shift != 0.0) d += shift * d;
+      neumann = type_s[s] == T_NEARWALL ? 1 : 0;
+      bs += d * xu;
+      len = kept + 1;
+      col[base + below_u * PPE_SLICE] = u;
+      val[base + below_u * PPE_SLICE] = d;
+      rlen[u] = len;
+      diag[u] = d;
+      xe[u] = xu;
+      b[u] = bs;
+      for (int k2 = len; k2 < width; ++k2) {
+        col[base + k2 * PPE_SLICE] = u;
+        val[base + k2 * PPE_SLICE] = 0.0;
+      }
+    } else {
+      for (int k2 = 0; k2 < width; ++k2) {
+        col[base + k2 * PPE_SLICE] = 0;
+        val[base + k2 * PPE_SLICE] = 0.0;
+      }
+    }
+  }
+  unsigned v[4] = {(unsigned)failed, (unsigned)neumann, (unsigned)len, (unsigned)graph};
+  block_count_partials<4>(v, bpart, lds);
+}
+
 // ---- reference-order views, for downloads and uploads -----------------
 
 // cnt_rr[rr] = rlen[rru[rr]] (CSR row lengths in reference order).
@@ -315,10 +507,10 @@ void launch_unknown_maps(const int* perm, const int* fs, const int* upos, const 
                        rru);
 }
 void launch_ppe_count(const int* uslot, const int* urr, const int* uid, const int* nbr, const int* ncnt, int maxn,
-                      int m, int* wsz, hipStream_t st) {
+                      int m, bool pin0, int* wsz, hipStream_t st) {
   if (m > 0)
     hipLaunchKernelGGL(k_ppe_count, dim3(ppe_slices(m)), dim3(PPE_THREADS), 0, st, uslot, urr, uid, nbr, ncnt, maxn, m,
-                       wsz);
+                       pin0 ? 1 : 0, wsz);
 }
 void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr, const int* uid,
                      const int* nbr, const int* ncnt, int maxn, int m, double re, double shift, int xmode,
@@ -332,6 +524,20 @@ void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslo
                      re, shift, xmode, xseed, sptr, col, val, rlen, diag, xe, b, bpart);
   if (ev_end) (void)hipEventRecord(ev_end, st);
   launch_sum_counts(bpart, g, CountSlots{3, {0, 1, 2, 0, 0, 0, 0, 0}}, counters, st);
+}
+void launch_ppe_wls_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr,
+                         const int* uid, const int* nbr, const int* ncnt, int maxn, int m, double re, double l0,
+                         double reg, double shift, int xmode, uint32_t xseed, const int* sptr, int* col, double* val,
+                         int* rlen, double* diag, double* xe, double* b, unsigned long long* counters,
+                         unsigned* bpart, hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
+  if (m <= 0) return;
+  const int mpad = ppe_slices(m) * PPE_SLICE;
+  const int g = (int)blocks(mpad, PPE_WLS_THREADS);
+  if (ev_begin) (void)hipEventRecord(ev_begin, st);
+  hipLaunchKernelGGL(k_ppe_wls_fill, dim3(g), dim3(PPE_WLS_THREADS), 0, st, spos, type_s, uslot, urr, uid, nbr, ncnt,
+                     maxn, m, mpad, re, l0, reg, shift, xmode, xseed, sptr, col, val, rlen, diag, xe, b, bpart);
+  if (ev_end) (void)hipEventRecord(ev_end, st);
+  launch_sum_counts(bpart, g, CountSlots{4, {0, 1, 2, 3, 0, 0, 0, 0}}, counters, st);
 }
 void launch_rr_lengths(const int* rru, const int* rlen, int m, int* cnt_rr, hipStream_t st) {
   if (m > 0) hipLaunchKernelGGL(k_rr_lengths, dim3(blocks(m, 256)), dim3(256), 0, st, rru, rlen, m, cnt_rr);

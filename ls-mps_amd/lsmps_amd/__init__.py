@@ -66,7 +66,7 @@ class PpeParams(C.Structure):
 
 class PpeResult(C.Structure):
     _fields_ = [("unknowns", _u64), ("nnz", _u64), ("dirichlet", _u64), ("neumann", _u64), ("failed_rows", _u64),
-                ("ms", _f), ("kernel_ms", _f), ("padded_nnz", _u64)]
+                ("ms", _f), ("kernel_ms", _f), ("padded_nnz", _u64), ("graph_rows", _u64)]
 
 
 class SolveParams(C.Structure):
@@ -115,6 +115,7 @@ ABI = {
     "lsmps_wls": (C.c_int, [_p, C.POINTER(WlsParams), C.POINTER(WlsResult)]),
     "lsmps_wls_sampled_stats": (C.c_int, [_p, _i32, C.POINTER(WlsStats)]),
     "lsmps_assemble_ppe": (C.c_int, [_p, C.POINTER(PpeParams), C.POINTER(PpeResult)]),
+    "lsmps_assemble_ppe_wls": (C.c_int, [_p, C.POINTER(PpeParams), C.c_double, C.c_double, C.POINTER(PpeResult)]),
     "lsmps_solve_ppe": (C.c_int, [_p, C.POINTER(SolveParams), C.POINTER(SolveResult)]),
     "lsmps_upload_x0": (C.c_int, [_p, _p, _u64]),
     "lsmps_download_types": (C.c_int, [_p, _p]),
@@ -244,6 +245,14 @@ class Pipeline:
         p = PpeParams(re, max_unknowns, xmode, xseed, shift)
         s = PpeResult()
         self._check(self.L.lsmps_assemble_ppe(self.h, C.byref(p), C.byref(s)), "assemble_ppe")
+        self.m, self.nnz = s.unknowns, s.nnz
+        return as_dict(s)
+
+    def assemble_wls(self, re, l0, max_unknowns, xmode=XSTAR_QUADRATIC, xseed=0, shift=0.0, reg=1e-8):
+        """WLS-Laplacian PPE rows (lsmps_assemble_ppe_wls)."""
+        p = PpeParams(re, max_unknowns, xmode, xseed, shift)
+        s = PpeResult()
+        self._check(self.L.lsmps_assemble_ppe_wls(self.h, C.byref(p), l0, reg, C.byref(s)), "assemble_ppe_wls")
         self.m, self.nnz = s.unknowns, s.nnz
         return as_dict(s)
 

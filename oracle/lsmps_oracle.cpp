@@ -449,6 +449,130 @@ void assemble(orc_state& s, double re, int max_unknowns, int xmode, uint32_t xse
   st.assembly_ms = ms_since(t0);
 }
 
+// WLS-Laplacian PPE rows. No C++ reference: the reference plans them
+// (README.md:34) and its 2-D model builds them (python/rain_plate_lsmps_cpu.py:
+// 204-253: row = w * (lap_row . basis(off)) per neighbor, diagonal minus their
+// sum, free-surface/splash neighbors at p = 0, invalid rows the identity).
+// Here in 3-D with the reference's weight, basis and ladder (src/lsmps.cpp:
+// 22-25) and wls_test's normalisation (:28): the WLS Laplacian at i is
+// (x3 + x4 + x5) / l0 with x = M^-1 sum_j w q_j (f_j - f_i) / (r_j ws), so
+// c_ij = (z . q_j) w_j / r_j / ws / l0 with z = M^-1 (0,0,0,1,1,1,0,0,0).
+// Row i is minus the operator (the graph Laplacian's sign): -c_ij per unknown
+// neighbor, d = sum_j c_ij over all weighted neighbors on the diagonal. A row
+// whose WLS fails, or with d <= 0 or d < 0.25 sum |c_ij| over the unknown
+// neighbors (a one-sided stencil), takes the graph-Laplacian row of
+// assemble() instead (src/lsmps.cpp:41-51). Unknowns, x* and b = A x* as in
+// assemble(); row 0 is not pinned.
+void assemble_wls(orc_state& s, double re, double l0, double reg, int max_unknowns, int xmode, uint32_t xseed,
+                  double shift, orc_ppe_stats& st) {
+  const auto t0 = Clock::now();
+  const size_t n = s.r.size();
+  const size_t cap = std::min(n, (size_t)std::max(0, max_unknowns));
+  std::vector<int> map(n, -1);
+  s.rev.clear();
+  for (size_t i = 0; i < n && s.rev.size() < cap; ++i)
+    if (s.type[i] == T_INTERIOR || s.type[i] == T_NEARWALL) {
+      map[i] = (int)s.rev.size();
+      s.rev.push_back((int)i);
+    }
+  const size_t m = s.rev.size();
+  Csr& A = s.A;
+  A = Csr{};
+  A.row.push_back(0);
+  A.b.assign(m, 0.0);
+  A.xe.resize(m);
+  for (size_t k = 0; k < m; ++k) {
+    const P3& p = s.r[s.rev[k]];
+    A.xe[k] = xmode == 0 ? p.x * p.x + p.y * p.y : hashed_uniform(xseed, k);
+  }
+  const double e3[NB] = {0, 0, 0, 1, 1, 1, 0, 0, 0};
+  for (size_t k = 0; k < m; ++k) {
+    const int i = s.rev[k];
+    const size_t start = A.col.size();
+    // Moments with the reference's normalisation (the right-hand side is not needed).
+    std::array<double, NB * NB> M{};
+    double wsum = 0.0;
+    for (int j : s.nbr[i]) {
+      const P3 d = sub(s.r[j], s.r[i]);
+      const double r = std::sqrt(len2(d));
+      const double w = kernel_weight(r, re);
+      if (w == 0.0) continue;
+      double q[NB];
+      monomials(d, r, l0, q);
+      wsum += w;
+      for (int a = 0; a < NB; ++a) {
+        const double wq = w * q[a];
+        for (int c = 0; c < NB; ++c) M[a * NB + c] += wq * q[c];
+      }
+    }
+    bool ok = wsum > 0;
+    double diag = 0.0, offabs = 0.0;
+    if (ok) {
+      for (double& v : M) v /= wsum;
+      bool shifted = false;
+      ok = chol_ladder(M, reg, shifted);
+    }
+    if (ok) {
+      const std::array<double, NB> z = chol_solve(M, e3);
+      for (int j : s.nbr[i]) {
+        const P3 d = sub(s.r[j], s.r[i]);
+        const double r = std::sqrt(len2(d));
+        const double w = kernel_weight(r, re);
+        if (w == 0.0) continue;
+        double q[NB];
+        monomials(d, r, l0, q);
+        double dot = 0.0;
+        for (int a = 0; a < NB; ++a) dot += z[a] * q[a];
+        const double c = dot * w / r / wsum / l0;
+        diag += c;
+        if (map[j] >= 0) {
+          A.col.push_back(map[j]);
+          A.val.push_back(-c);
+          offabs += std::fabs(c);
+        }
+      }
+      ok = std::isfinite(diag) && diag > 0.0 && diag >= 0.25 * offabs;
+    }
+    if (!ok) {
+      // The graph-Laplacian row, as assemble() builds it.
+      A.col.resize(start);
+      A.val.resize(start);
+      st.graph_rows++;
+      diag = 0.0;
+      for (int j : s.nbr[i]) {
+        const int cj = map[j];
+        if (cj < 0) continue;
+        const P3 d = sub(s.r[j], s.r[i]);
+        const double r = std::sqrt(len2(d));
+        const double w = kernel_weight(r, re);
+        if (w <= 0) continue;
+        const double a = w / (r * r);
+        diag += a;
+        A.col.push_back(cj);
+        A.val.push_back(-a);
+      }
+      if (diag <= 0 || !std::isfinite(diag)) {
+        st.failed_rows++;
+        diag = 1.0;
+      }
+    }
+    if (shift != 0.0) diag += shift * diag;
+    A.col.push_back((int)k);
+    A.val.push_back(diag);
+    A.row.push_back((int)A.col.size());
+    if (s.type[i] == T_NEARWALL) st.neumann++;
+  }
+  for (size_t k = 0; k < m; ++k) {
+    double acc = 0.0;
+    for (int e = A.row[k]; e < A.row[k + 1]; ++e) acc += A.val[e] * A.xe[A.col[e]];
+    A.b[k] = acc;
+  }
+  st.unknowns = m;
+  st.nnz = A.val.size();
+  st.dirichlet = n - m;
+  st.assembly_ms = ms_since(t0);
+}
+
 // y = A x, row by row in stored order (src/lsmps.cpp:65).
 void csr_mul(const Csr& A, const std::vector<double>& x, std::vector<double>& y) {
   const size_t m = A.b.size();
@@ -686,6 +810,13 @@ void orc_assemble_ppe(orc_state* s, double re, int max_unknowns, int xmode, uint
                       orc_ppe_stats* out) {
   orc_ppe_stats st{};
   assemble(*s, re, max_unknowns, xmode, xseed, shift, st);
+  *out = st;
+}
+
+void orc_assemble_ppe_wls(orc_state* s, double re, double l0, double reg, int max_unknowns, int xmode,
+                          uint32_t xseed, double shift, orc_ppe_stats* out) {
+  orc_ppe_stats st{};
+  assemble_wls(*s, re, l0, reg, max_unknowns, xmode, xseed, shift, st);
   *out = st;
 }
 

@@ -50,6 +50,7 @@ typedef struct {
   int32_t iterations, converged;
   double initial_residual, final_residual, relative_l2_error, max_error;
   double assembly_ms, solve_ms;
+  uint64_t graph_rows; /* WLS rows: rows that took the graph-Laplacian row */
 } orc_ppe_stats;
 
 typedef struct {
@@ -101,6 +102,16 @@ void orc_wls_test(const orc_state* s, double re, double l0, double reg, int wls_
  * diagonal after assembly (0 = reference). */
 void orc_assemble_ppe(orc_state* s, double re, int max_unknowns, int xmode, uint32_t xseed,
                       double shift, orc_ppe_stats* out);
+/* WLS-Laplacian rows (no C++ reference; README.md:34, 2-D model
+ * python/rain_plate_lsmps_cpu.py:204-253): same unknowns, x* and b = A x*;
+ * row i holds -c_ij, c_ij = (z . q_ij) w_ij / r_ij / ws / l0, per unknown
+ * neighbor in list order and the diagonal d = sum c_ij over all weighted
+ * neighbors, with z = M^-1 (0,0,0,1,1,1,0,0,0) from the reference's moments
+ * and ladder. A row whose WLS fails or with d <= 0 or d < 0.25 sum |c_ij|
+ * takes the graph-Laplacian row of orc_assemble_ppe (graph_rows). Read back
+ * with orc_get_ppe. */
+void orc_assemble_ppe_wls(orc_state* s, double re, double l0, double reg, int max_unknowns, int xmode,
+                          uint32_t xseed, double shift, orc_ppe_stats* out);
 void orc_get_ppe(const orc_state* s, int32_t* row, int32_t* col, double* val, double* b,
                  double* x_exact, int32_t* rev);
 /* BiCGStab (src/lsmps.cpp:65-67). precond 0 = none (reference), 1 = right

@@ -41,7 +41,7 @@ class PpeStats(C.Structure):
     _fields_ = [("unknowns", _u64), ("nnz", _u64), ("dirichlet", _u64), ("neumann", _u64),
                 ("failed_rows", _u64), ("iterations", _i32), ("converged", _i32),
                 ("initial_residual", _d), ("final_residual", _d), ("relative_l2_error", _d),
-                ("max_error", _d), ("assembly_ms", _d), ("solve_ms", _d)]
+                ("max_error", _d), ("assembly_ms", _d), ("solve_ms", _d), ("graph_rows", _u64)]
 
 
 class RunStats(C.Structure):
@@ -84,6 +84,7 @@ def lib():
         L.orc_wls_moments.argtypes = [_p, _p, _d, _d, _u64, _p, _p]
         L.orc_wls_test.argtypes = [_p, _d, _d, _d, C.c_int, C.POINTER(WlsStats)]
         L.orc_assemble_ppe.argtypes = [_p, _d, C.c_int, C.c_int, C.c_uint32, _d, C.POINTER(PpeStats)]
+        L.orc_assemble_ppe_wls.argtypes = [_p, _d, _d, _d, C.c_int, C.c_int, C.c_uint32, _d, C.POINTER(PpeStats)]
         L.orc_get_ppe.argtypes = [_p, _p, _p, _p, _p, _p, _p]
         L.orc_solve_ppe.argtypes = [_p, C.c_int, _d, C.c_int, _p, C.POINTER(PpeStats), _p]
         L.orc_run.argtypes = [_p, _d, _d, C.c_int, C.c_int, C.c_int, C.c_int, _d, C.c_int,
@@ -176,6 +177,12 @@ class Oracle:
     def assemble(self, re, max_unknowns, xmode=0, xseed=0, shift=0.0):
         st = PpeStats()
         self.L.orc_assemble_ppe(self.h, re, max_unknowns, xmode, xseed, shift, C.byref(st))
+        self._ppe = _struct_dict(st)
+        return self._ppe
+
+    def assemble_wls(self, re, l0, max_unknowns, xmode=0, xseed=0, shift=0.0, reg=1e-8):
+        st = PpeStats()
+        self.L.orc_assemble_ppe_wls(self.h, re, l0, reg, max_unknowns, xmode, xseed, shift, C.byref(st))
         self._ppe = _struct_dict(st)
         return self._ppe
 

@@ -292,3 +292,149 @@ def test_feature_invariants():
     assert abs(sb - np.sqrt(np.mean(q * q))) <= 1e-12 * sb
     assert abs(np.mean(f[:, 12].astype(np.float64) ** 2) - 1.0) < 1e-5
     assert f[0, 15] == 1.0 and f[0, 13] == 0.0
+
+
+def _wls_ppe_case(n=3000, seed=5, l0=0.01):
+    xyz, t = O.generate_case("bulk_uniform", n, seed, l0)
+    o = O.Oracle(xyz, t)
+    re = 2.1 * l0
+    o.neighbors(re, 128)
+    o.classify(l0)
+    return o, xyz, re, l0
+
+
+def _all_unknown_rows(o, A, re, xyz):
+    """Reference rows whose weighted neighbors are all unknowns."""
+    row, col = o.get_neighbors()
+    unknown = np.zeros(len(xyz), bool)
+    unknown[A["rev"]] = True
+    out = []
+    for k, i in enumerate(A["rev"]):
+        nb = col[row[i]:row[i + 1]]
+        r = np.linalg.norm(xyz[nb] - xyz[i], axis=1)
+        nb = nb[(r > 1e-12) & (r < re)]
+        if len(nb) and unknown[nb].all():
+            out.append(k)
+    return np.array(out)
+
+
+def _graph_rows(o, re, A, xmode, xseed):
+    """Rows of the WLS matrix that equal the graph-Laplacian row (src/lsmps.cpp:
+    41-51, without the row-0 pin): -w/r^2 per unknown neighbor in list
+    order, their sum on the diagonal (1 when not positive)."""
+    row, col = o.get_neighbors()
+    xyz = o.xyz
+    pos_of = {int(i): k for k, i in enumerate(A["rev"])}
+    out = np.zeros(len(A["b"]), bool)
+    for k, i in enumerate(A["rev"]):
+        nb = col[row[i]:row[i + 1]]
+        unk = np.array([j in pos_of for j in nb.tolist()], bool)
+        nb = nb[unk] if len(nb) else nb
+        r = np.sqrt(((xyz[nb] - xyz[i]) ** 2).sum(1))
+        keep = (r > 1e-12) & (r < re)
+        nb, r = nb[keep], r[keep]
+        a = (re / r - 1.0) ** 2 / (r * r)
+        dg = a.sum() if a.sum() > 0 else 1.0
+        s, e = A["row"][k], A["row"][k + 1]
+        cols = [pos_of[j] for j in nb.tolist()]
+        if A["col"][s:e - 1].tolist() != cols or A["col"][e - 1] != k:
+            continue
+        out[k] = np.allclose(A["val"][s:e - 1], -a, rtol=1e-12, atol=0) and abs(A["val"][e - 1] - dg) <= 1e-12 * dg
+    return out
+
+
+def test_wls_ppe_reproduces_quadratic_laplacian():
+    """Closed form: where every neighbor is an unknown and the row is a WLS
+    row, row k of A applied to x* = x^2 + y^2 is minus the WLS Laplacian of
+    x*, exact (-4) for a quadratic; and for any quadratic field f, A f equals
+    minus the WLS operator's Laplacian of f (orc_wls_all), row by row."""
+    o, xyz, re, l0 = _wls_ppe_case()
+    st = o.assemble_wls(re, l0, 1 << 30, xmode=0)
+    A = o.get_ppe()
+    m = len(A["b"])
+    assert st["unknowns"] == m > 1000 and st["failed_rows"] == 0
+    graph = _graph_rows(o, re, A, 0, 0)
+    assert graph.sum() == st["graph_rows"] < 0.2 * m
+    inner = _all_unknown_rows(o, A, re, xyz)
+    inner = inner[~graph[inner]]
+    assert len(inner) > 200
+    np.testing.assert_allclose(A["b"][inner], -4.0, rtol=0, atol=1e-6)
+    f = test_field(xyz)
+    _, lap, wst = o.wls_all(f, re, l0)
+    fu = f[A["rev"]]
+    af = np.zeros(m)
+    for k in inner:
+        s, e = A["row"][k], A["row"][k + 1]
+        af[k] = A["val"][s:e] @ fu[A["col"][s:e]]
+    np.testing.assert_allclose(af[inner], -lap[A["rev"][inner]], rtol=1e-9, atol=1e-7)
+    np.testing.assert_allclose(lap[A["rev"][inner]], 1.8, atol=1e-6)
+
+
+def test_wls_ppe_rows_against_library_solve():
+    """Each row from numpy: M from the basis and weight, z = solve(M, e3)
+    by LU, c_j = (z . q_j) w_j / r_j / ws / l0. A WLS row holds -c_j for the
+    unknown neighbors in list order and d = sum c_j on the diagonal; a row
+    with d <= 0 or d < 0.25 sum |c_j| (unknown neighbors) is the graph row."""
+    o, xyz, re, l0 = _wls_ppe_case(1200, 7)
+    st = o.assemble_wls(re, l0, 1 << 30, xmode=1, xseed=4)
+    A = o.get_ppe()
+    graph = _graph_rows(o, re, A, 1, 4)
+    row, col = o.get_neighbors()
+    pos_of = {int(i): k for k, i in enumerate(A["rev"])}
+    checked = [0, 0]
+    for k in range(len(A["b"])):
+        i = int(A["rev"][k])
+        nb = col[row[i]:row[i + 1]]
+        d = xyz[nb] - xyz[i]
+        r = np.linalg.norm(d, axis=1)
+        keep = (r > 1e-12) & (r < re)
+        nb, d, r = nb[keep], d[keep], r[keep]
+        w = (re / r - 1.0) ** 2
+        q = np.column_stack([d[:, 0] / r, d[:, 1] / r, d[:, 2] / r, d[:, 0] ** 2 / (2 * l0 * r),
+                             d[:, 1] ** 2 / (2 * l0 * r), d[:, 2] ** 2 / (2 * l0 * r), d[:, 0] * d[:, 1] / (l0 * r),
+                             d[:, 0] * d[:, 2] / (l0 * r), d[:, 1] * d[:, 2] / (l0 * r)])
+        M = (q * w[:, None]).T @ q / w.sum()
+        z = np.linalg.solve(M, np.array([0, 0, 0, 1, 1, 1, 0, 0, 0.0]))
+        c = (q @ z) * w / r / w.sum() / l0
+        unk = np.array([j in pos_of for j in nb.tolist()], bool)
+        dsum, offabs = c.sum(), np.abs(c[unk]).sum()
+        if abs(dsum - 0.25 * offabs) < 1e-6 * offabs or abs(dsum) < 1e-6 * offabs:
+            continue  # too close to the rule's edge to decide in another arithmetic
+        valid = dsum > 0 and dsum >= 0.25 * offabs
+        assert valid != graph[k], k
+        if not valid:
+            checked[1] += 1
+            continue
+        s, e = A["row"][k], A["row"][k + 1]
+        assert A["col"][s:e - 1].tolist() == [pos_of[j] for j in nb[unk].tolist()]
+        assert A["col"][e - 1] == k
+        np.testing.assert_allclose(A["val"][s:e - 1], -c[unk], rtol=1e-8, atol=1e-8 * np.abs(c).max())
+        assert abs(A["val"][e - 1] - dsum) <= 1e-8 * np.abs(c).sum()
+        checked[0] += 1
+    assert checked[0] > 300 and checked[1] > 10 and checked[1] == st["graph_rows"]
+    # b = A x* left to right.
+    b = np.array([A["val"][A["row"][k]:A["row"][k + 1]] @ A["x_exact"][A["col"][A["row"][k]:A["row"][k + 1]]]
+                  for k in range(len(A["b"]))])
+    np.testing.assert_allclose(A["b"], b, rtol=1e-12, atol=1e-9)
+
+
+def test_wls_ppe_fallback_rows():
+    """On a flat sheet every moment matrix lacks the z directions: with no
+    regularisation (reg = 0) the Cholesky fails and every row is the graph
+    row; a particle with no neighbor has no weight and a graph row with a
+    zero diagonal, which becomes the identity (failed_rows). With the
+    default ladder the shifted solve succeeds for the sheet instead."""
+    l0 = 0.01
+    g = np.arange(7) * l0
+    xyz = np.array([(x, y, 0.05) for x in g for y in g] + [(1.0, 1.0, 1.0)])
+    t = np.zeros(len(xyz), np.uint8)
+    o = O.Oracle(xyz, t)
+    re = 2.1 * l0
+    o.neighbors(re, 128)
+    st = o.assemble_wls(re, l0, 1 << 30, xmode=1, xseed=2, reg=0.0)
+    A = o.get_ppe()
+    assert st["graph_rows"] == len(xyz) == len(A["b"]) and st["failed_rows"] == 1
+    assert _graph_rows(o, re, A, 1, 2).all()
+    assert A["row"][-1] - A["row"][-2] == 1 and A["val"][-1] == 1.0
+    st = o.assemble_wls(re, l0, 1 << 30, xmode=1, xseed=2)
+    assert st["graph_rows"] < len(xyz) and st["failed_rows"] == 1
