@@ -49,8 +49,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--particles", type=int, default=3_000_000)
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--solvers", default="fp64_jacobi,mixed,learned_mixed",
-                    help="comma list of the PPE solves to time (fp64_jacobi, mixed, learned_mixed)")
+    ap.add_argument("--solvers", default="fp64_jacobi,mixed,learned_mixed,wls_rows",
+                    help="comma list of the PPE solves to time (fp64_jacobi, mixed, learned_mixed; wls_rows: "
+                         "the WLS-Laplacian rows, assembled and solved with FP64 Jacobi and mixed)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-max-iter", type=int, default=40000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -264,6 +265,21 @@ def main():
             if name in solves and "fp64_jacobi" in solves and solves[name]["ms"] > 0:
                 solves[f"speedup_{name}_vs_fp64"] = round(solves["fp64_jacobi"]["ms"] / solves[name]["ms"], 3)
         extra["ppe_time_to_1e-8"] = solves
+        if "wls_rows" in wanted:
+            # The WLS-Laplacian rows on the same particles (steady state).
+            p.assemble_wls(re, l0, 1 << 30, L.XSTAR_RANDOM, 11)
+            pw = p.assemble_wls(re, l0, 1 << 30, L.XSTAR_RANDOM, 11)
+            log(f"wls rows {pw}")
+            wls_rows = {"unknowns": pw["unknowns"], "nnz": pw["nnz"], "graph_rows": pw["graph_rows"],
+                        "failed_rows": pw["failed_rows"], "assembly_ms": round(pw["ms"], 3),
+                        "fill_kernel_ms": round(pw["kernel_ms"], 4)}
+            for name, method in (("fp64_jacobi", L.SOLVE_JACOBI), ("mixed", L.SOLVE_MIXED)):
+                p.solve(method, args.solve_max_iter, 1e-8, chunk=32)
+                st = p.solve(method, args.solve_max_iter, 1e-8, chunk=32)
+                log(f"wls rows solve {name}: {st}")
+                wls_rows[name] = {"ms": round(st["ms"], 3), "iterations": st["iterations"],
+                                  "converged": st["converged"]}
+            extra["ppe_wls_rows"] = wls_rows
     p.close()
 
     out = {

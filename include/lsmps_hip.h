@@ -266,6 +266,47 @@ int lsmps_download_ppe(lsmps_ctx* ctx, int32_t* row, int32_t* col, double* val, 
                        int32_t* rev);
 int lsmps_download_solution(lsmps_ctx* ctx, double* x);
 
+/* ---- synthetic CSR replay (python/workload_calibration.py:236-264) ------
+ * The calibration tool's customer-profile proxy: a CSR neighbor graph
+ * (csr_row_ptr int64[n+1], csr_col_idx int32[nnz]), particle_type u8[n] in
+ * the tool's numbering (0 interior, 1 near wall, 2 free surface, 3 splash,
+ * 4 wall/other; mapped to ParticleType) and wls_ill_conditioned bool[n]
+ * (config/customer_car_rain_profile.example.json:5-18 describes the
+ * profile). The graph has no positions: every edge gets a synthetic offset
+ * of length [0.35, 0.95) re from a seeded hash, and flagged rows get
+ * near-planar offsets, so their WLS takes the FP64 redo and the
+ * regularisation ladder. Rows are sorted ascending on load; self-edges are
+ * kept in the table and skipped by the stages. No counterpart in the
+ * reference's C++: the tool writes these files for GPU kernel replay. */
+typedef struct {
+  uint64_t rows, edges;
+  int32_t max_row;     /* longest row: the width of the neighbor table (<= 256) */
+  int32_t pad;
+  uint64_t self_edges;
+  uint64_t types[5];   /* rows per ParticleType after the mapping */
+  uint64_t flagged;    /* rows flagged wls_ill_conditioned */
+  float ms;
+} lsmps_replay_info;
+/* Validates the arrays on the host (row_ptr[0] = 0, non-decreasing, nnz and
+ * rows within int32, rows <= 256, columns in range, types <= 4) and replaces
+ * the context's particles and neighbor table by the graph. */
+int lsmps_replay_load(lsmps_ctx* ctx, const int64_t* csr_row_ptr, const int32_t* csr_col_idx,
+                      const uint8_t* particle_type, const uint8_t* wls_ill_conditioned, uint64_t n,
+                      uint32_t seed, lsmps_replay_info* out);
+/* The WLS stage on the replay graph: FP32 moments and Cholesky per row on
+ * the synthetic offsets with the reference's manufactured quadratic, FP64
+ * redo with the ladder of rows whose FP32 pivots are not trusted. Results
+ * download with lsmps_download_wls (gradient (2, -3, 0.5), Laplacian 1.8
+ * where the fit is exact). */
+int lsmps_replay_wls(lsmps_ctx* ctx, double re, double l0, double reg, lsmps_wls_result* out);
+/* The graph-Laplacian PPE (src/lsmps.cpp:29-64 rules: unknowns Interior and
+ * NearWall, row 0 pinned, diagonal last) on the replay graph with the
+ * synthetic symmetric weight (0.5 + u) / l0^2 per edge, u in [0, 1) from
+ * the splitmix64 finaliser of (lo << 32 | hi) ^ (seed * 0x9E3779B97F4A7C15);
+ * x* ~ U(-1, 1) as LSMPS_XSTAR_RANDOM. Solve with lsmps_solve_ppe; tol = 0
+ * runs exactly max_iter iterations (a fixed per-step count). */
+int lsmps_replay_assemble(lsmps_ctx* ctx, int32_t max_unknowns, uint32_t xseed, double l0, lsmps_ppe_result* out);
+
 #ifdef __cplusplus
 }
 #endif

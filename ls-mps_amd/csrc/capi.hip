@@ -86,6 +86,8 @@ struct lsmps_ctx {
   // cell list: particles in cell-major ("sorted") order and the ELL
   // neighbor table in sorted-slot indices; CSR only for downloads
   bool have_cnl = false;
+  bool replay = false;  // the neighbor table holds a replay graph (no positions)
+  uint32_t replay_seed = 0;
   double re = 0.0;
   int maxn = 0, gx = 0, gy = 0, gz = 0;
   uint64_t pairs = 0;
@@ -139,6 +141,11 @@ struct lsmps_ctx {
   DBuf<int> bidx;
   int64_t dcount = 0;
   int adam_step = 0;
+
+  // synthetic CSR replay: staged CSR and the ill-conditioned flags
+  DBuf<int64_t> rrow;
+  DBuf<int> rcol;
+  DBuf<uint8_t> rflag, rtype;
   void reset_graphs() {
     graph[0].reset();
     graph[1].reset();
@@ -360,6 +367,10 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->fbits.release();
   ctx->bidx.release();
   ctx->scan_scr.release();
+  ctx->rrow.release();
+  ctx->rcol.release();
+  ctx->rflag.release();
+  ctx->rtype.release();
   ctx->counters.release();
   ctx->bpart.release();
   ctx->sell_v32.release();
@@ -411,6 +422,7 @@ int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->n = n;
   ctx->n_cnl = 0;
+  ctx->replay = false;
   ctx->have_u = false;
   ctx->types_stale = false;
   ctx->have_features = false;
@@ -566,6 +578,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   cs.ms = elapsed(ctx);
   cs.kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
   ctx->re = rl;
+  ctx->replay = false;
   ctx->n_cnl = (uint64_t)n;
   ctx->maxn = maxn;
   ctx->types_stale = classify;
@@ -770,6 +783,7 @@ int lsmps_set_field_values(lsmps_ctx* ctx, const double* f, uint64_t n) {
 int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) {
   if (!ctx || !p) return LSMPS_E_ARG;
   if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "WLS needs neighbors");
+  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph: use lsmps_replay_wls");
   if (!ctx->have_field) return fail(ctx, LSMPS_E_STATE, "WLS needs a field (lsmps_set_field_*)");
   if (p->precision != LSMPS_WLS_FP64 && p->precision != LSMPS_WLS_MIXED)
     return fail(ctx, LSMPS_E_ARG, "unknown WLS precision");
@@ -885,11 +899,16 @@ namespace {
 // Shared by both row kinds: unknown numbering, slice widths, then the fill
 // (graph Laplacian, or WLS Laplacian when wls).
 int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0, double reg,
-                 lsmps_ppe_result* out) {
+                 lsmps_ppe_result* out, bool replay = false, uint32_t wseed = 0) {
   if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "PPE assembly needs neighbors");
+  if (ctx->replay != replay)
+    return fail(ctx, LSMPS_E_STATE, replay ? "replay assembly needs lsmps_replay_load"
+                                           : "the context holds a replay graph: use lsmps_replay_assemble");
   if (p->xmode != LSMPS_XSTAR_QUADRATIC && p->xmode != LSMPS_XSTAR_RANDOM) return fail(ctx, LSMPS_E_ARG, "bad xmode");
-  if (!(p->re > 0) || !std::isfinite(p->re)) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
-  if (p->re > ctx->re) return fail(ctx, LSMPS_E_ARG, "re exceeds the radius of the neighbor lists");
+  if (replay && p->xmode != LSMPS_XSTAR_RANDOM) return fail(ctx, LSMPS_E_ARG, "a replay graph has no positions: x* must be random");
+  if (!replay && (!(p->re > 0) || !std::isfinite(p->re))) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
+  if (!replay && p->re > ctx->re) return fail(ctx, LSMPS_E_ARG, "re exceeds the radius of the neighbor lists");
+  if (replay && (!(l0 > 0) || !std::isfinite(l0))) return fail(ctx, LSMPS_E_ARG, "replay weights need l0 > 0");
   if (wls && (!(l0 > 0) || !std::isfinite(l0) || !(reg >= 0) || !std::isfinite(reg)))
     return fail(ctx, LSMPS_E_ARG, "WLS rows need l0 > 0 and reg >= 0, finite");
   HIPCHK(hipSetDevice(ctx->device));
@@ -951,7 +970,7 @@ int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0,
     launch_ppe_fill(ctx->spos.p, ctx->type_s.p, ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p,
                     ctx->maxn, m, p->re, p->shift, p->xmode, p->xseed, ctx->sell_ptr.p, ctx->sell_col.p,
                     ctx->sell_v64.p, ctx->rlen.p, ctx->diag.p, ctx->xe.p, ctx->b.p, ctx->counters.p, ctx->bpart.p, st,
-                    ctx->e2, ctx->e3);
+                    ctx->e2, ctx->e3, replay, wseed, l0);
   if (int rc = check_launch(ctx, "ppe_fill")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
   unsigned long long c[4] = {0, 0, 0, 0};
@@ -992,6 +1011,147 @@ int lsmps_assemble_ppe_wls(lsmps_ctx* ctx, const lsmps_ppe_params* p, double l0,
   if (!ctx || !p) return LSMPS_E_ARG;
   GUARD_BEGIN
   return assemble_ppe(ctx, p, true, l0, reg, out);
+  GUARD_END
+}
+
+int lsmps_replay_load(lsmps_ctx* ctx, const int64_t* row_ptr, const int32_t* col, const uint8_t* particle_type,
+                      const uint8_t* wls_ill_conditioned, uint64_t n, uint32_t seed, lsmps_replay_info* out) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (!row_ptr || (n && (!particle_type || !wls_ill_conditioned))) return fail(ctx, LSMPS_E_ARG, "null replay arrays");
+  if (n >= (uint64_t)INT_MAX / 4) return fail(ctx, LSMPS_E_LIMIT, "row count beyond int32 indexing");
+  // Validation on the host, before anything reaches the device.
+  if (row_ptr[0] != 0) return fail(ctx, LSMPS_E_ARG, "csr_row_ptr[0] must be 0");
+  int64_t longest = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const int64_t len = row_ptr[i + 1] - row_ptr[i];
+    if (len < 0) return fail(ctx, LSMPS_E_ARG, "csr_row_ptr must be non-decreasing");
+    longest = std::max(longest, len);
+  }
+  const int64_t nnz = row_ptr[n];
+  if (nnz > (int64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "edges beyond int32 indexing");
+  if (longest > 256) return fail(ctx, LSMPS_E_LIMIT, "rows longer than 256 neighbors");
+  if (nnz && !col) return fail(ctx, LSMPS_E_ARG, "null csr_col_idx");
+  uint64_t self_edges = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+      if (col[e] < 0 || (uint64_t)col[e] >= n) return fail(ctx, LSMPS_E_ARG, "csr_col_idx out of range");
+      self_edges += (uint64_t)col[e] == i ? 1u : 0u;
+    }
+  uint64_t counts[5] = {0, 0, 0, 0, 0}, flagged = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (particle_type[i] > 4) return fail(ctx, LSMPS_E_ARG, "particle_type outside 0..4");
+    counts[particle_type[i]]++;
+    flagged += wls_ill_conditioned[i] ? 1u : 0u;
+  }
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->st;
+  const int ni = (int)n, maxn = std::max<int>(1, (int)longest);
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  HIPCHK(ctx->rrow.ensure(n + 1));
+  HIPCHK(ctx->rcol.ensure(std::max<int64_t>(nnz, 1)));
+  HIPCHK(ctx->rflag.ensure(std::max<uint64_t>(n, 1)));
+  HIPCHK(ctx->rtype.ensure(std::max<uint64_t>(n, 1)));
+  DBuf<int>* per_particle[] = {&ctx->perm, &ctx->iperm, &ctx->cnt, &ctx->cnt_orig};
+  for (auto* b : per_particle) HIPCHK(b->ensure(std::max<uint64_t>(n, 1)));
+  HIPCHK(ctx->type.ensure(std::max<uint64_t>(n, 1)));
+  HIPCHK(ctx->type_s.ensure(std::max<uint64_t>(n, 1)));
+  HIPCHK(ctx->spos.ensure(std::max<uint64_t>(n, 1)));
+  HIPCHK(ctx->nbr.ensure((size_t)std::max<uint64_t>(n, 1) * (size_t)maxn));
+  HIPCHK(hipMemcpyAsync(ctx->rrow.p, row_ptr, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  if (nnz) HIPCHK(hipMemcpyAsync(ctx->rcol.p, col, (size_t)nnz * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  if (n) {
+    HIPCHK(hipMemcpyAsync(ctx->rflag.p, wls_ill_conditioned, n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->rtype.p, particle_type, n, hipMemcpyHostToDevice, st));
+  }
+  zero_counters(ctx, 16);
+  launch_replay_load(ctx->rrow.p, ctx->rcol.p, ctx->rtype.p, ni, maxn, seed, ctx->nbr.p, ctx->cnt.p, ctx->type_s.p,
+                     ctx->spos.p, reinterpret_cast<unsigned*>(ctx->counters.p), st);
+  launch_iota(ni, ctx->perm.p, st);
+  launch_iota(ni, ctx->iperm.p, st);
+  if (n) HIPCHK(hipMemcpyAsync(ctx->type.p, ctx->type_s.p, n, hipMemcpyDeviceToDevice, st));
+  if (int rc = check_launch(ctx, "replay_load")) return rc;
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->n = ctx->n_cnl = n;
+  ctx->replay = true;
+  ctx->re = 0.0;
+  ctx->maxn = maxn;
+  ctx->pairs = (uint64_t)nnz;
+  ctx->replay_seed = seed;
+  ctx->types_stale = false;
+  ctx->have_u = false;
+  ctx->have_cnl = true;
+  ctx->have_field = ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_x0 = ctx->have_features = false;
+  ctx->reset_graphs();
+  if (out) {
+    std::memset(out, 0, sizeof(*out));
+    out->rows = n;
+    out->edges = (uint64_t)nnz;
+    out->max_row = (int32_t)longest;
+    out->self_edges = self_edges;
+    // Types mapped to the ParticleType numbering (include/lsmps.hpp:8).
+    out->types[0] = counts[0];
+    out->types[3] = counts[1];
+    out->types[1] = counts[2];
+    out->types[2] = counts[3];
+    out->types[4] = counts[4];
+    out->flagged = flagged;
+    out->ms = elapsed(ctx);
+  }
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_replay_wls(lsmps_ctx* ctx, double re, double l0, double reg, lsmps_wls_result* out) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (!ctx->replay) return fail(ctx, LSMPS_E_STATE, "replay WLS needs lsmps_replay_load");
+  if (!(re > 0) || !(l0 > 0) || !(reg >= 0) || !std::isfinite(re) || !std::isfinite(l0) || !std::isfinite(reg))
+    return fail(ctx, LSMPS_E_ARG, "re, l0 > 0 and reg >= 0, finite");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = (int)ctx->n_cnl;
+  hipStream_t st = ctx->st;
+  HIPCHK(ctx->grad_s.ensure(3 * (size_t)std::max(n, 1)));
+  HIPCHK(ctx->lap_s.ensure(std::max(n, 1)));
+  HIPCHK(ctx->wstatus_s.ensure(std::max(n, 1)));
+  HIPCHK(ctx->redo.ensure(std::max(n, 1)));
+  HIPCHK(ctx->bpart.ensure(3 * ((size_t)n / 128 + 1)));
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  zero_counters(ctx, 16);
+  launch_fill_int(ctx->nredo.p, 1, 0, st);
+  launch_replay_wls(ctx->nbr.p, ctx->cnt.p, ctx->maxn, ctx->type_s.p, ctx->rflag.p, ctx->spos.p, n, re, l0, reg,
+                    ctx->replay_seed, ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p,
+                    ctx->counters.p, ctx->bpart.p, st, ctx->e2, ctx->e3);
+  if (int rc = check_launch(ctx, "replay_wls")) return rc;
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  unsigned long long c[6];
+  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->have_wls = true;
+  if (out) {
+    out->ok = c[WLS_OK];
+    out->regularized = c[WLS_REGULARIZED];
+    out->failed = c[WLS_FAILED] + c[WLS_NO_WEIGHT];
+    out->skipped = c[WLS_SKIPPED];
+    out->fp64_redo = c[WLS_REDO_SLOT];
+    out->ms = elapsed(ctx);
+    out->kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
+  }
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_replay_assemble(lsmps_ctx* ctx, int32_t max_unknowns, uint32_t xseed, double l0, lsmps_ppe_result* out) {
+  if (!ctx) return LSMPS_E_ARG;
+  lsmps_ppe_params p{};
+  p.re = 0.0;
+  p.max_unknowns = max_unknowns;
+  p.xmode = LSMPS_XSTAR_RANDOM;
+  p.xseed = xseed;
+  p.shift = 0.0;
+  GUARD_BEGIN
+  return assemble_ppe(ctx, &p, false, l0, 0.0, out, true, ctx->replay_seed);
   GUARD_END
 }
 
@@ -1180,7 +1340,8 @@ int solve_mixed(lsmps_ctx* ctx, const double* scale, const double* x0, bool prec
     if (!(s0.init_res <= s0.bnorm * (1.0 + 1e-6))) return 1;
   }
   int rc = run_bicgstab<float>(ctx, B, x0, max_iter, tol, flags, chunk, out);
-  if (rc == LSMPS_OK && !out->converged) {
+  // tol = 0 asks for exactly max_iter iterations: nothing to fall back to.
+  if (rc == LSMPS_OK && !out->converged && tol > 0) {
     // Fallback and judge: FP64 right-Jacobi BiCGStab, continuing from the
     // mixed iterate when it lowered the residual, else from zero.
     const lsmps_solve_result first = *out;

@@ -185,4 +185,14 @@ __device__ __forceinline__ void finalize_sum(const double* partials, int nblk, d
   for (int k = 0; k < NV; ++k) tot[k] = bc[k];
 }
 
+// splitmix64 finaliser (Steele, Lea, Flood 2014) and the top 53 bits of a
+// hash as a double in [0, 1): the counter-based generators of the
+// synthetic inputs.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double unit53(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
+
 }  // namespace lsmps_hip

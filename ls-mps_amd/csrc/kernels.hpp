@@ -93,11 +93,13 @@ void launch_unknown_maps(const int* perm, const int* fs, const int* upos, const 
 // pin0: row 0 is the identity (graph-Laplacian rows), so its width is 1.
 void launch_ppe_count(const int* uslot, const int* urr, const int* uid, const int* nbr, const int* ncnt, int maxn,
                       int m, bool pin0, int* wsz, hipStream_t st);
+// replay: edges of a replay graph with synthetic weights (replay.hip).
 void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr, const int* uid,
                      const int* nbr, const int* ncnt, int maxn, int m, double re, double shift, int xmode,
                      uint32_t xseed, const int* sptr, int* col, double* val, int* rlen, double* diag, double* xe,
                      double* b, unsigned long long* counters, unsigned* bpart, hipStream_t st,
-                     hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                     hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, bool replay = false,
+                     uint32_t wseed = 0, double l0 = 0.0);
 // WLS-Laplacian rows into the same sliced-ELL layout; counters as
 // launch_ppe_fill, plus [3] rows that took the graph-Laplacian row. bpart:
 // 4 * ceil(ceil(m / 64) / 2) words.
@@ -112,5 +114,16 @@ void launch_sell_to_csr(const int* rru, const int* urr, const int* sptr, const i
 void launch_to_rr(const int* rru, const double* src, int m, double* dst, hipStream_t st);
 void launch_rr_particle(const int* rru, const int* uslot, const int* perm, int m, int* rev, hipStream_t st);
 void launch_from_rr(const int* urr, const double* src, int m, double* dst, hipStream_t st);
+
+// replay.hip: the synthetic CSR replay (calibration tool's .npy graph).
+// type_in uses the tool's numbering; unknown values count in *bad_types.
+void launch_replay_load(const int64_t* row_ptr, const int* col, const uint8_t* type_in, int n, int maxn,
+                        uint32_t seed, int* nbr, int* cnt, uint8_t* type_s, double4* rec, unsigned* bad_types,
+                        hipStream_t st);
+// bpart: 3 * ceil(n / 128) words; counts[] as the WLS stage.
+void launch_replay_wls(const int* nbr, const int* cnt, int maxn, const uint8_t* type, const uint8_t* bad,
+                       const double4* rec, int n, double re, double l0, double reg, uint32_t seed, double* grad,
+                       double* lap, uint8_t* status, int* redo, int* nredo, unsigned long long* counts,
+                       unsigned* bpart, hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 }  // namespace lsmps_hip

@@ -122,6 +122,14 @@ __device__ __forceinline__ double xstar_random(uint32_t seed, int k) {
   return -1.0 + 2.0 * ((double)(z >> 11) * 0x1.0p-53);
 }
 
+// Synthetic weight of the replay edge {lo, hi} (lo <= hi, particle indices):
+// (0.5 + u) / l0^2 with u in [0, 1) from the splitmix64 finaliser of
+// (lo * 2^32 + hi) xor (seed * 2^64/phi); symmetric in the edge.
+__device__ __forceinline__ double replay_weight(uint32_t seed, int lo, int hi, double l0) {
+  const uint64_t key = (((uint64_t)(unsigned)lo << 32) | (uint64_t)(unsigned)hi) ^ ((uint64_t)seed * 0x9E3779B97F4A7C15ull);
+  return (0.5 + unit53(mix64(key))) / (l0 * l0);
+}
+
 // Slice-local row of this thread's group and the slice index.
 struct PpeRow {
   int slice, u, g, gbase;
@@ -165,12 +173,17 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_count(const int* __restrict
 // numbering) and FP64 values, the row lengths, the diagonal, x* and b.
 // Counts per block into bpart (then counters): [0] failed rows, [1] NearWall
 // rows except row 0, [2] nonzeros.
+// REPLAY: the edges of a replay graph (slots are particle indices) with
+// replay_weight(wseed, ...) in place of w/r^2 from positions (re unused);
+// self-edges are skipped as the reference skips r <= 1e-12.
+template <bool REPLAY>
 __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
     const double4* __restrict__ spos, const uint8_t* __restrict__ type_s, const int* __restrict__ uslot,
     const int* __restrict__ urr, const int* __restrict__ uid, const int* __restrict__ nbr,
     const int* __restrict__ ncnt, int maxn, int m, double re, double shift, int xmode, uint32_t xseed,
-    const int* __restrict__ sptr, int* __restrict__ col, double* __restrict__ val, int* __restrict__ rlen,
-    double* __restrict__ diag, double* __restrict__ xe, double* __restrict__ b, unsigned* __restrict__ bpart) {
+    uint32_t wseed, double l0, const int* __restrict__ sptr, int* __restrict__ col, double* __restrict__ val,
+    int* __restrict__ rlen, double* __restrict__ diag, double* __restrict__ xe, double* __restrict__ b,
+    unsigned* __restrict__ bpart) {
   __shared__ unsigned lds[3 * PPE_THREADS / WAVE];
   const PpeRow R = ppe_row(m);
   const unsigned below = (1u << R.g) - 1u;
@@ -199,9 +212,17 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
           const int s2 = row[q];
           cj = uid[s2];
           if (cj >= 0) {
-            const double4 pj = spos[s2];
-            keep = pair_coef(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, re, &a);
-            if (keep) xj = xmode == 0 ? pj.x * pj.x + pj.y * pj.y : xstar_random(xseed, urr[cj]);
+            if (REPLAY) {
+              keep = s2 != s;
+              if (keep) {
+                a = replay_weight(wseed, s < s2 ? s : s2, s < s2 ? s2 : s, l0);
+                xj = xstar_random(xseed, urr[cj]);
+              }
+            } else {
+              const double4 pj = spos[s2];
+              keep = pair_coef(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, re, &a);
+              if (keep) xj = xmode == 0 ? pj.x * pj.x + pj.y * pj.y : xstar_random(xseed, urr[cj]);
+            }
           }
         }
         const unsigned gm = group8_ballot(keep, R.gbase);
@@ -516,12 +537,16 @@ void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslo
                      const int* nbr, const int* ncnt, int maxn, int m, double re, double shift, int xmode,
                      uint32_t xseed, const int* sptr, int* col, double* val, int* rlen, double* diag, double* xe,
                      double* b, unsigned long long* counters, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin,
-                     hipEvent_t ev_end) {
+                     hipEvent_t ev_end, bool replay, uint32_t wseed, double l0) {
   if (m <= 0) return;
   const int g = ppe_slices(m);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
-  hipLaunchKernelGGL(k_ppe_fill, dim3(g), dim3(PPE_THREADS), 0, st, spos, type_s, uslot, urr, uid, nbr, ncnt, maxn, m,
-                     re, shift, xmode, xseed, sptr, col, val, rlen, diag, xe, b, bpart);
+  if (replay)
+    hipLaunchKernelGGL(k_ppe_fill<true>, dim3(g), dim3(PPE_THREADS), 0, st, spos, type_s, uslot, urr, uid, nbr, ncnt,
+                       maxn, m, re, shift, xmode, xseed, wseed, l0, sptr, col, val, rlen, diag, xe, b, bpart);
+  else
+    hipLaunchKernelGGL(k_ppe_fill<false>, dim3(g), dim3(PPE_THREADS), 0, st, spos, type_s, uslot, urr, uid, nbr, ncnt,
+                       maxn, m, re, shift, xmode, xseed, 0u, 0.0, sptr, col, val, rlen, diag, xe, b, bpart);
   if (ev_end) (void)hipEventRecord(ev_end, st);
   launch_sum_counts(bpart, g, CountSlots{3, {0, 1, 2, 0, 0, 0, 0, 0}}, counters, st);
 }

@@ -69,6 +69,11 @@ class PpeResult(C.Structure):
                 ("ms", _f), ("kernel_ms", _f), ("padded_nnz", _u64), ("graph_rows", _u64)]
 
 
+class ReplayInfo(C.Structure):
+    _fields_ = [("rows", _u64), ("edges", _u64), ("max_row", _i32), ("pad", _i32), ("self_edges", _u64),
+                ("types", _u64 * 5), ("flagged", _u64), ("ms", _f)]
+
+
 class SolveParams(C.Structure):
     _fields_ = [("method", _i32), ("max_iter", _i32), ("tol", _d), ("use_x0", _i32), ("chunk", _i32)]
 
@@ -116,6 +121,9 @@ ABI = {
     "lsmps_wls_sampled_stats": (C.c_int, [_p, _i32, C.POINTER(WlsStats)]),
     "lsmps_assemble_ppe": (C.c_int, [_p, C.POINTER(PpeParams), C.POINTER(PpeResult)]),
     "lsmps_assemble_ppe_wls": (C.c_int, [_p, C.POINTER(PpeParams), C.c_double, C.c_double, C.POINTER(PpeResult)]),
+    "lsmps_replay_load": (C.c_int, [_p, _p, _p, _p, _p, _u64, C.c_uint32, C.POINTER(ReplayInfo)]),
+    "lsmps_replay_wls": (C.c_int, [_p, C.c_double, C.c_double, C.c_double, C.POINTER(WlsResult)]),
+    "lsmps_replay_assemble": (C.c_int, [_p, C.c_int32, C.c_uint32, C.c_double, C.POINTER(PpeResult)]),
     "lsmps_solve_ppe": (C.c_int, [_p, C.POINTER(SolveParams), C.POINTER(SolveResult)]),
     "lsmps_upload_x0": (C.c_int, [_p, _p, _u64]),
     "lsmps_download_types": (C.c_int, [_p, _p]),
@@ -245,6 +253,34 @@ class Pipeline:
         p = PpeParams(re, max_unknowns, xmode, xseed, shift)
         s = PpeResult()
         self._check(self.L.lsmps_assemble_ppe(self.h, C.byref(p), C.byref(s)), "assemble_ppe")
+        self.m, self.nnz = s.unknowns, s.nnz
+        return as_dict(s)
+
+    # -- synthetic CSR replay ------------------------------------------------------
+    def replay_load(self, row_ptr, col, particle_type, wls_ill_conditioned, seed=0):
+        """Loads a replay graph (arrays in the calibration tool's dtypes)."""
+        row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+        n = row_ptr.shape[0] - 1
+        col = np.ascontiguousarray(col, np.int32)
+        t = np.ascontiguousarray(particle_type, np.uint8)
+        f = np.ascontiguousarray(wls_ill_conditioned, np.uint8)
+        if t.shape != (n,) or f.shape != (n,):
+            raise ValueError("particle_type and wls_ill_conditioned need one entry per row")
+        s = ReplayInfo()
+        self._check(self.L.lsmps_replay_load(self.h, _ptr(row_ptr), _ptr(col), _ptr(t), _ptr(f), n, seed,
+                                             C.byref(s)), "replay_load")
+        self.n = n
+        self.pairs = s.edges
+        return as_dict(s)
+
+    def replay_wls(self, re, l0, reg=1e-8):
+        s = WlsResult()
+        self._check(self.L.lsmps_replay_wls(self.h, re, l0, reg, C.byref(s)), "replay_wls")
+        return as_dict(s)
+
+    def replay_assemble(self, max_unknowns, xseed, l0):
+        s = PpeResult()
+        self._check(self.L.lsmps_replay_assemble(self.h, max_unknowns, xseed, l0, C.byref(s)), "replay_assemble")
         self.m, self.nnz = s.unknowns, s.nnz
         return as_dict(s)
 
