@@ -31,6 +31,7 @@ Injection and deletion counts are reported, not applied: the proxy graph
 carries no geometry to place new particles in.
 
     python -m lsmps_amd.replay <synthetic_workload dir> [--steps K]
+    python -m lsmps_amd.replay <dir> --generate <profile.json>   (write, then replay)
 """
 from __future__ import annotations
 
@@ -135,6 +136,74 @@ def write_contract(path: str, row_len, particle_type, wls_bad, virtual=None, col
                 w.writerow(r)
 
 
+def profile_row_lengths(n: int, mean: float, p90: float, p99: float, rng) -> np.ndarray:
+    """Row lengths with a profile's mean, P90 and P99, built the way the
+    calibration tool builds them (python/workload_calibration.py:175-209):
+    a four-band mixture (50% low, 40% middle, 9% high, 1% tail band) and a
+    final +-1 adjustment of random rows to the requested mean."""
+    u = rng.random(n)
+    low = max(1.0, 0.55 * mean)
+    mid = max(low, min(p90, 1.06 * mean))
+    high = max(mid, p90)
+    tail = max(high, p99)
+    bands = ((u < 0.50, 0.75 * low, 1.05 * low), ((u >= 0.50) & (u < 0.90), 0.90 * mid, 1.02 * high),
+             ((u >= 0.90) & (u < 0.99), high, max(high + 1.0, 0.98 * tail)), (u >= 0.99, tail, 1.08 * tail))
+    x = np.empty(n)
+    for sel, a, b in bands:
+        x[sel] = rng.uniform(a, b, int(sel.sum()))
+    x = np.maximum(1, np.rint(x)).astype(np.int64)
+    delta = int(round(mean * n)) - int(x.sum())
+    if delta:
+        order = rng.permutation(n)
+        step = 1 if delta > 0 else -1
+        k = 0
+        while delta and k < 100 * n:
+            i = order[k % n]
+            if step > 0 or x[i] > 1:
+                x[i] += step
+                delta -= step
+            k += 1
+    return x.astype(np.int32)
+
+
+def make_workload(metrics: dict, path: str, seed: int = 43, materialize: bool = False) -> dict:
+    """A synthetic workload directory in the tool's layout from a profile's
+    metrics (config/customer_car_rain_profile.example.json:5-18): row
+    lengths as profile_row_lengths, types in the profile's ratios (near
+    wall, free surface, splash; the rest interior) shuffled, flags with the
+    ill-conditioned ratio, and a timeline of Poisson injection/deletion
+    counts, the rebuild cadence and the PPE iteration count."""
+    rng = np.random.default_rng(seed)
+    n = int(round(metrics["particles"]))
+    row_len = profile_row_lengths(n, metrics["neighbor_mean"], metrics["neighbor_p90"], metrics["neighbor_p99"], rng)
+    ratios = [min(max(metrics.get(k, 0.0), 0.0), 1.0) for k in ("near_wall_ratio", "free_surface_ratio",
+                                                               "splash_ratio")]
+    if sum(ratios) > 0.95:
+        ratios = [r * 0.95 / sum(ratios) for r in ratios]
+    counts = [int(round(n * r)) for r in ratios]
+    t = np.concatenate([np.zeros(n - sum(counts), np.uint8)] +
+                       [np.full(c, k + 1, np.uint8) for k, c in enumerate(counts)])
+    rng.shuffle(t)
+    bad = rng.random(n) < metrics.get("wls_ill_conditioned_ratio", 0.0)
+    virtual = rng.binomial(np.maximum(row_len, 1), min(metrics.get("virtual_particle_ratio", 0.0), 0.8))
+    steps = int(metrics.get("timeline_steps", 200))
+    rebuild = max(1, int(round(metrics.get("cnl_rebuild_interval", 10))))
+    inj, dele = metrics.get("injected_per_step", 0), metrics.get("deleted_per_step", 0)
+    it = metrics.get("ppe_iterations", 1)
+    tl = [{"step": s, "injected": int(rng.poisson(inj)) if inj else 0, "deleted": int(rng.poisson(dele)) if dele else 0,
+           "cnl_rebuild": int(s % rebuild == 0),
+           "ppe_iterations": max(1, int(round(rng.normal(it, max(1.0, 0.05 * it)))))} for s in range(steps)]
+    col = synthesize_columns(row_len, seed + 1) if materialize else None
+    write_contract(path, row_len, t, bad, virtual=virtual.astype(np.int16), col=col, timeline=tl)
+    meta = {"particles": n, "neighbor_mean": float(row_len.mean()),
+            "neighbor_p90": float(np.quantile(row_len, 0.9, method="higher")),
+            "neighbor_p99": float(np.quantile(row_len, 0.99, method="higher")),
+            "wls_ill_conditioned_ratio": float(bad.mean()), "csr_nnz": int(row_len.sum())}
+    with open(os.path.join(path, "metadata.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    return meta
+
+
 def percentiles(v):
     v = np.asarray(v, np.float64)
     if v.size == 0:
@@ -199,12 +268,18 @@ def run(path: str, steps: int | None = None, l0: float = 0.004, re_ratio: float 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("path", help="synthetic_workload directory written by the calibration tool")
+    ap.add_argument("--generate", default=None, metavar="PROFILE_JSON",
+                    help="first write a workload into PATH from a profile's metrics")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--l0", type=float, default=0.004)
     ap.add_argument("--method", choices=("mixed", "jacobi"), default="mixed")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--out", default=None, help="write the per-step rows as JSON here")
     args = ap.parse_args(argv)
+    if args.generate:
+        doc = json.load(open(args.generate))
+        meta = make_workload(doc.get("metrics", doc), args.path)
+        log(f"wrote {args.path}: {meta}")
     summary, rows = run(args.path, args.steps, args.l0, method=args.method, seed=args.seed)
     if args.out:
         with open(args.out, "w") as f:
