@@ -157,8 +157,9 @@ int lsmps_assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, lsmps_ppe_resu
  * each unknown neighbor in list order, c_ij = (z . q_ij) w_ij / r_ij / ws /
  * l0 with z = M^-1 (0,0,0,1,1,1,0,0,0), then the diagonal d = sum_j c_ij
  * over all weighted neighbors (non-unknown neighbors are p = 0). A row
- * whose WLS fails, or with d <= 0 or d < 0.25 sum |c_ij| (a one-sided
- * stencil), takes the graph-Laplacian row of lsmps_assemble_ppe instead
+ * whose WLS fails, or with d <= 0 or d < sum |c_ij| over the unknown
+ * neighbors (not weakly diagonally dominant, as one-sided stencils are),
+ * takes the graph-Laplacian row of lsmps_assemble_ppe instead
  * (graph_rows); row 0 is not pinned. */
 int lsmps_assemble_ppe_wls(lsmps_ctx* ctx, const lsmps_ppe_params* p, double l0, double reg,
                            lsmps_ppe_result* out);

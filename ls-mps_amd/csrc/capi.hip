@@ -513,6 +513,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
                     lsmps_type_stats* tout) {
   if (!(re > 0) || !std::isfinite(re)) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
   if (!(skin >= 0) || !std::isfinite(skin)) return fail(ctx, LSMPS_E_ARG, "skin must be finite and >= 0");
+  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
   const double rl = re + skin;  // list radius and cell size
   // Lists above 256 entries would take a 64-thread variant of the neighbor
   // pass that has not been validated on the GPU; it stays switched off.
@@ -622,6 +623,7 @@ int lsmps_build_neighbors_skin(lsmps_ctx* ctx, double re, double skin, int32_t m
 
 int lsmps_set_velocities(lsmps_ctx* ctx, const double* uvw, uint64_t n) {
   if (!ctx || (n && !uvw)) return LSMPS_E_ARG;
+  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
   if (n != ctx->n) return fail(ctx, LSMPS_E_ARG, "velocity count differs from the particle count");
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
@@ -635,6 +637,7 @@ int lsmps_set_velocities(lsmps_ctx* ctx, const double* uvw, uint64_t n) {
 int lsmps_append_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* type, const double* uvw,
                            uint64_t count) {
   if (!ctx || (count && (!xyz || !type))) return LSMPS_E_ARG;
+  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
   if (ctx->n + count >= (uint64_t)INT_MAX / 4) return fail(ctx, LSMPS_E_LIMIT, "particle count beyond int32 indexing");
   if (uvw && !ctx->have_u && ctx->n) return fail(ctx, LSMPS_E_STATE, "velocities of the existing particles unset");
   GUARD_BEGIN
@@ -664,6 +667,7 @@ int lsmps_append_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
 
 int lsmps_advance(lsmps_ctx* ctx, double dt) {
   if (!ctx) return LSMPS_E_ARG;
+  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
   if (!ctx->have_u) return fail(ctx, LSMPS_E_STATE, "advance needs velocities (lsmps_set_velocities)");
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
@@ -682,6 +686,7 @@ int lsmps_advance(lsmps_ctx* ctx, double dt) {
 
 int lsmps_remove_outside(lsmps_ctx* ctx, const double* lo, const double* hi, uint64_t* removed) {
   if (!ctx || !lo || !hi) return LSMPS_E_ARG;
+  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->st;
@@ -732,6 +737,7 @@ int lsmps_remove_outside(lsmps_ctx* ctx, const double* lo, const double* hi, uin
 int lsmps_classify(lsmps_ctx* ctx, double l0, lsmps_type_stats* out) {
   if (!ctx) return LSMPS_E_ARG;
   if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "classify needs neighbors");
+  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->st;
@@ -1437,6 +1443,7 @@ int lsmps_download_types(lsmps_ctx* ctx, uint8_t* type) {
 
 int lsmps_download_positions(lsmps_ctx* ctx, double* xyz) {
   if (!ctx || (ctx->n && !xyz)) return LSMPS_E_ARG;
+  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   const int64_t n = (int64_t)ctx->n;

@@ -48,7 +48,7 @@
 // off-diagonals that outweigh it, which makes the matrix indefinite (on a
 // wall film, Jacobi-scaled eigenvalues down to -32; BiCGStab and GMRES
 // both fail). Such a row -- WLS failed, d not finite, d <= 0 or
-// d < WLS_ROW_DOMINANCE * sum |c_ij| over the unknown neighbors -- takes the
+// d < sum |c_ij| over the unknown neighbors -- takes the
 // reference's graph-Laplacian row instead (-w/r^2 per unknown neighbor,
 // their sum on the diagonal, 1 when that is not positive). One thread per
 // row, FP64, no contraction: bit-identical to the oracle's restatement.
@@ -336,7 +336,11 @@ __device__ __noinline__ bool wls_lap_weights(const double4 pi, const int* __rest
 }
 
 constexpr int PPE_WLS_THREADS = 128;
-constexpr double WLS_ROW_DOMINANCE = 0.25;
+// A kept WLS row is weakly diagonally dominant with a positive diagonal, so
+// every Gershgorin disc of D^-1 A lies in Re >= 0 (|z - 1| <= 1). A weaker
+// bound (0.25) solved a 30k wall film but left the 3M industrial scene with
+// eigenvalues in the left half-plane: BiCGStab broke down there.
+constexpr double WLS_ROW_DOMINANCE = 1.0;
 // One thread per stored row u < mpad (all rows of all slices). Counts per
 // block into bpart: [0] rows whose diagonal was set to 1, [1] NearWall
 // rows, [2] nonzeros, [3] rows that took the graph-Laplacian row.

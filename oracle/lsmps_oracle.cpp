@@ -459,7 +459,7 @@ void assemble(orc_state& s, double re, int max_unknowns, int xmode, uint32_t xse
 // c_ij = (z . q_j) w_j / r_j / ws / l0 with z = M^-1 (0,0,0,1,1,1,0,0,0).
 // Row i is minus the operator (the graph Laplacian's sign): -c_ij per unknown
 // neighbor, d = sum_j c_ij over all weighted neighbors on the diagonal. A row
-// whose WLS fails, or with d <= 0 or d < 0.25 sum |c_ij| over the unknown
+// whose WLS fails, or with d <= 0 or d < sum |c_ij| over the unknown
 // neighbors (a one-sided stencil), takes the graph-Laplacian row of
 // assemble() instead (src/lsmps.cpp:41-51). Unknowns, x* and b = A x* as in
 // assemble(); row 0 is not pinned.
@@ -531,7 +531,7 @@ void assemble_wls(orc_state& s, double re, double l0, double reg, int max_unknow
           offabs += std::fabs(c);
         }
       }
-      ok = std::isfinite(diag) && diag > 0.0 && diag >= 0.25 * offabs;
+      ok = std::isfinite(diag) && diag > 0.0 && diag >= offabs;
     }
     if (!ok) {
       // The graph-Laplacian row, as assemble() builds it.

@@ -107,7 +107,7 @@ void orc_assemble_ppe(orc_state* s, double re, int max_unknowns, int xmode, uint
  * row i holds -c_ij, c_ij = (z . q_ij) w_ij / r_ij / ws / l0, per unknown
  * neighbor in list order and the diagonal d = sum c_ij over all weighted
  * neighbors, with z = M^-1 (0,0,0,1,1,1,0,0,0) from the reference's moments
- * and ladder. A row whose WLS fails or with d <= 0 or d < 0.25 sum |c_ij|
+ * and ladder. A row whose WLS fails or with d <= 0 or d < sum |c_ij|
  * takes the graph-Laplacian row of orc_assemble_ppe (graph_rows). Read back
  * with orc_get_ppe. */
 void orc_assemble_ppe_wls(orc_state* s, double re, double l0, double reg, int max_unknowns, int xmode,

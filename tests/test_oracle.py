@@ -354,7 +354,8 @@ def test_wls_ppe_reproduces_quadratic_laplacian():
     m = len(A["b"])
     assert st["unknowns"] == m > 1000 and st["failed_rows"] == 0
     graph = _graph_rows(o, re, A, 0, 0)
-    assert graph.sum() == st["graph_rows"] < 0.2 * m
+    # A 14^3 cube is mostly boundary: a third of its rows are one-sided.
+    assert graph.sum() == st["graph_rows"] < 0.4 * m
     inner = _all_unknown_rows(o, A, re, xyz)
     inner = inner[~graph[inner]]
     assert len(inner) > 200
@@ -374,7 +375,7 @@ def test_wls_ppe_rows_against_library_solve():
     """Each row from numpy: M from the basis and weight, z = solve(M, e3)
     by LU, c_j = (z . q_j) w_j / r_j / ws / l0. A WLS row holds -c_j for the
     unknown neighbors in list order and d = sum c_j on the diagonal; a row
-    with d <= 0 or d < 0.25 sum |c_j| (unknown neighbors) is the graph row."""
+    with d <= 0 or d < sum |c_j| (unknown neighbors) is the graph row."""
     o, xyz, re, l0 = _wls_ppe_case(1200, 7)
     st = o.assemble_wls(re, l0, 1 << 30, xmode=1, xseed=4)
     A = o.get_ppe()
@@ -398,9 +399,9 @@ def test_wls_ppe_rows_against_library_solve():
         c = (q @ z) * w / r / w.sum() / l0
         unk = np.array([j in pos_of for j in nb.tolist()], bool)
         dsum, offabs = c.sum(), np.abs(c[unk]).sum()
-        if abs(dsum - 0.25 * offabs) < 1e-6 * offabs or abs(dsum) < 1e-6 * offabs:
+        if abs(dsum - offabs) < 1e-6 * offabs or abs(dsum) < 1e-6 * offabs:
             continue  # too close to the rule's edge to decide in another arithmetic
-        valid = dsum > 0 and dsum >= 0.25 * offabs
+        valid = dsum > 0 and dsum >= offabs
         assert valid != graph[k], k
         if not valid:
             checked[1] += 1
