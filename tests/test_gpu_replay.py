@@ -112,9 +112,13 @@ def test_replay_fixed_iteration_solves(pipe):
         say("fixed", method, st)
         assert st["iterations"] == 37 and st["converged"] == 0 and st["fallback"] == 0
         assert st["state"] == 4  # max_iter
-    # The position-based stages refuse the replay graph.
+    # The position-based calls refuse the replay graph (it has no positions).
     with pytest.raises(L.LsmpsError):
         pipe.assemble(0.0084, 1 << 30)
+    with pytest.raises(L.LsmpsError):
+        pipe.neighbors(0.0084, 64)
+    with pytest.raises(L.LsmpsError):
+        pipe.advance(1e-3)
 
 
 def test_replay_timeline_driver(tmp_path):
