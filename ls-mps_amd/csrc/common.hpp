@@ -59,6 +59,15 @@ __device__ __forceinline__ float group8_sum(float v) {
   v += __int_as_float(dpp_i<DPP_HALF_MIRROR>(__float_as_int(v)));
   return v;
 }
+// Sum over aligned groups of G lanes (G = 2, 4 or 8), by the same DPP steps.
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+  static_assert(G == 2 || G == 4 || G == 8, "group of 2, 4 or 8 lanes");
+  v += __int_as_float(dpp_i<DPP_QUAD_SWAP1>(__float_as_int(v)));
+  if (G >= 4) v += __int_as_float(dpp_i<DPP_QUAD_SWAP2>(__float_as_int(v)));
+  if (G >= 8) v += __int_as_float(dpp_i<DPP_HALF_MIRROR>(__float_as_int(v)));
+  return v;
+}
 __device__ __forceinline__ int group8_sum_int(int v) {
   v += dpp_i<DPP_QUAD_SWAP1>(v);
   v += dpp_i<DPP_QUAD_SWAP2>(v);

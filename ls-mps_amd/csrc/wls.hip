@@ -158,16 +158,17 @@ __global__ __launch_bounds__(128) void k_wls_exact_list(const double4* __restric
 // dimensionless and df/r = (df/re)/(r/re).
 //
 // One wavefront per 64 consecutive sorted particles, in two halves of 32:
-//  phase 1: 8 lanes per particle (8 particles at a time) stride through the
-//           particle's neighbor row, so row reads are coalesced and the 8
-//           lanes gather neighboring particles; each lane accumulates the 55
-//           sums (45 lower-triangle moments, 9 right-hand sides, weight
-//           sum), the 8 lanes combine them with xor shuffles and write them
-//           to LDS;
+//  phase 1: 2 lanes per particle (32 particles at a time) stride through the
+//           particle's neighbor row, four records in flight per lane; each
+//           lane accumulates the 55 sums (45 lower-triangle moments, 9
+//           right-hand sides, weight sum), the pair combines them with one
+//           DPP step and writes them to LDS. (8 lanes per particle spent 165
+//           DPP adds per particle on the combine and ran 0.81 ms at 3M;
+//           4 lanes 0.62 ms, 2 lanes 0.58 ms, measured on MI355X.)
 //  phase 2: one lane per particle factorises its 9x9 from LDS (FP32
 //           Cholesky with reciprocal square roots) and solves. Rows whose
 //           pivots lose more than three digits go to the FP64 work list.
-constexpr int WLS_GROUP = 8;   // lanes per particle in phase 1
+constexpr int WLS_GROUP = 2;   // lanes per particle in phase 1
 constexpr int WLS_NACC = 55;   // 45 moments + 9 RHS + weight sum
 constexpr int WLS_STRIDE = 57; // LDS floats per particle (odd: no bank conflicts)
 constexpr int WLS_HALF = 32;   // particles per phase-2 batch
@@ -298,11 +299,12 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
         }
       }
 #pragma unroll
-      for (int k = 0; k < WLS_NACC; ++k) a[k] = group8_sum(a[k]);
-      // Lane g of the group stores sums g*7 .. g*7+6.
+      for (int k = 0; k < WLS_NACC; ++k) a[k] = group_sum<WLS_GROUP>(a[k]);
+      // Lane g of the group stores its share of the sums.
+      constexpr int share = (WLS_NACC + WLS_GROUP - 1) / WLS_GROUP;
 #pragma unroll
       for (int k = 0; k < WLS_NACC; ++k)
-        if (k / 7 == g) sums[local * WLS_STRIDE + k] = a[k];
+        if (k / share == g) sums[local * WLS_STRIDE + k] = a[k];
     }
     __syncthreads();
     // phase 2: one lane per particle of this half.
