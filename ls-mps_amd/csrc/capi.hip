@@ -91,7 +91,7 @@ struct lsmps_ctx {
   double re = 0.0;
   int maxn = 0, gx = 0, gy = 0, gz = 0;
   uint64_t pairs = 0;
-  DBuf<int> cx, cy, cz, box, cell, slot, ccount, cstart, perm, iperm, scell, nbr, cnt, cnt_orig, row_ptr, col, hist;
+  DBuf<int> cx, cy, cz, box, cell, slot, ccount, cstart, perm, iperm, sflag, nbr, cnt, cnt_orig, row_ptr, col, hist;
   DBuf<double4> spos;
   DBuf<float4> sposf;
   double ox = 0.0, oy = 0.0, oz = 0.0;  // grid origin of the FP32 offsets
@@ -340,7 +340,7 @@ void lsmps_destroy(lsmps_ctx* ctx) {
                        &ctx->vec64,  &ctx->partials, &ctx->sell_v64, &ctx->sell_vs64};
   for (auto* p : d) p->release();
   DBuf<int>* i[] = {&ctx->cx,    &ctx->cy,     &ctx->cz,    &ctx->box,      &ctx->cell,     &ctx->slot,
-                    &ctx->ccount, &ctx->cstart, &ctx->perm,  &ctx->iperm,    &ctx->scell,    &ctx->nbr,
+                    &ctx->ccount, &ctx->cstart, &ctx->perm,  &ctx->iperm,    &ctx->sflag,    &ctx->nbr,
                     &ctx->cnt,   &ctx->cnt_orig, &ctx->row_ptr, &ctx->col,   &ctx->hist,     &ctx->redo,
                     &ctx->nredo, &ctx->flag,   &ctx->pos,   &ctx->fs,       &ctx->upos,     &ctx->uid,
                     &ctx->uslot, &ctx->urr,    &ctx->rru,   &ctx->rlen,     &ctx->rcnt,     &ctx->prow,
@@ -488,7 +488,7 @@ CnlBufs cnl_bufs(lsmps_ctx* ctx) {
   B.scan_scratch = ctx->scan_scr.p;
   B.perm = ctx->perm.p;
   B.iperm = ctx->iperm.p;
-  B.scell = ctx->scell.p;
+  B.sflag = ctx->sflag.p;
   B.spos = ctx->spos.p;
   B.sposf = ctx->sposf.p;
   B.ox = ctx->ox;
@@ -525,10 +525,10 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
     return fail(ctx, LSMPS_E_LIMIT, "neighbor table too large");
   sync_types(ctx);  // with the previous order, before it is replaced
   DBuf<int>* per_particle[] = {&ctx->cx,   &ctx->cy,    &ctx->cz,    &ctx->cell, &ctx->slot,
-                               &ctx->perm, &ctx->iperm, &ctx->scell, &ctx->cnt,  &ctx->cnt_orig};
+                               &ctx->perm, &ctx->iperm, &ctx->sflag, &ctx->cnt,  &ctx->cnt_orig};
   for (auto* b : per_particle) HIPCHK(b->ensure(n));
   HIPCHK(ctx->spos.ensure(n));
-  HIPCHK(ctx->sposf.ensure(n));
+  HIPCHK(ctx->sposf.ensure((size_t)n + CNL_PAD));
   HIPCHK(ctx->type_s.ensure(n));
   HIPCHK(ctx->nbr.ensure((size_t)n * (size_t)std::max(1, maxn)));
   HIPCHK(ctx->hist.ensure((size_t)maxn + 1));

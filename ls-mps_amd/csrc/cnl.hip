@@ -118,21 +118,25 @@ __global__ __launch_bounds__(256) void k_cell_sort(const int* __restrict__ start
 }
 
 // Sorted-slot copies of positions ({x, y, z, field} records, the field
-// filled in later), FP32 offsets from the grid origin (for the candidate
-// prefilter), cell ids, and the inverse permutation.
+// filled in later), {FP32 offsets from the grid origin, cell id} records
+// (for the candidate prefilter), the cell's grid-boundary flags, and the
+// inverse permutation.
 __global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ perm, const int* __restrict__ cell,
-                                                       const double* __restrict__ x, const double* __restrict__ y,
-                                                       const double* __restrict__ z, int n, double ox, double oy,
-                                                       double oz, double4* __restrict__ spos,
-                                                       float4* __restrict__ sposf, int* __restrict__ scell,
-                                                       int* __restrict__ iperm) {
+                                                       int gx, int gy, int gz, const double* __restrict__ x,
+                                                       const double* __restrict__ y, const double* __restrict__ z,
+                                                       int n, double ox, double oy, double oz,
+                                                       double4* __restrict__ spos, float4* __restrict__ sposf,
+                                                       int* __restrict__ sflag, int* __restrict__ iperm) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
   const double a = x[i], b = y[i], c = z[i];
   spos[s] = make_double4(a, b, c, 0.0);
-  sposf[s] = make_float4((float)(a - ox), (float)(b - oy), (float)(c - oz), 0.0f);
-  scell[s] = cell[i];
+  const int cl = cell[i];
+  sposf[s] = make_float4((float)(a - ox), (float)(b - oy), (float)(c - oz), __int_as_float(cl));
+  const int u = cl % gx, t = cl / gx, v = t % gy, w = t / gy;
+  sflag[s] = (u > 0 ? CNL_XLO : 0) | (u < gx - 1 ? CNL_XHI : 0) | (v > 0 ? CNL_YLO : 0) | (v < gy - 1 ? CNL_YHI : 0) |
+             (w > 0 ? CNL_ZLO : 0) | (w < gz - 1 ? CNL_ZHI : 0);
   iperm[i] = s;
 }
 
@@ -148,6 +152,22 @@ __device__ __forceinline__ unsigned sector_bits(double dx, double dy, double dz)
   return (1u << axis) | (1u << oct);
 }
 
+// The same sectors from FP32 components each within eta of the exact
+// offset, without branches. *sure is false when a sign (|d| <= es = 1.5 eta)
+// or a comparison of two magnitudes (difference <= ec = 3 eta) is too close
+// to call at that error; then the FP64 offset decides.
+__device__ __forceinline__ unsigned sector_bits_f32(float dx, float dy, float dz, float es, float ec, bool* sure) {
+  const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+  // 0/1 flags in integer arithmetic (a chain of selects would become
+  // branches): negative signs, and which axis dominates (x, else y, else z).
+  const unsigned nx = dx >= 0.f ? 0u : 1u, ny = dy >= 0.f ? 0u : 1u, nz = dz >= 0.f ? 0u : 1u;
+  const unsigned isx = (ax >= ay) & (ax >= az), isy = (1u - isx) & (ay >= az), isz = 1u - isx - isy;
+  const unsigned axis = isx * nx + isy * (2u + ny) + isz * (4u + nz);
+  const unsigned oct = 6u + (1u - nx) + 2u * (1u - ny) + 4u * (1u - nz);
+  *sure = fminf(fminf(ax, ay), az) > es && fminf(fminf(fabsf(ax - ay), fabsf(ax - az)), fabsf(ay - az)) > ec;
+  return (1u << axis) | (1u << oct);
+}
+
 __device__ __forceinline__ uint8_t classify_rule(uint8_t t0, int kept, double z, double zlim, unsigned mask) {
   if (t0 == T_WALL) return T_WALL;
   if (kept < 10) return T_SPLASH;
@@ -158,8 +178,8 @@ __device__ __forceinline__ uint8_t classify_rule(uint8_t t0, int kept, double z,
 
 // Eight lanes per particle, eight consecutive sorted particles (nearly
 // always from the same few cells) per wavefront. The lanes step through
-// each candidate run sixteen at a time (two loads in flight per lane), so
-// position loads are coalesced; a ballot gives every hit its rank among
+// each candidate run thirty-two at a time (four loads in flight per lane),
+// so position loads are coalesced; a ballot gives every hit its rank among
 // the group's hits, which keeps the reference's visiting order (and the
 // cut at max_neighbors). Hits are staged in LDS; afterwards the group
 // writes the row out with coalesced stores and, when CLASSIFY, takes the
@@ -170,99 +190,179 @@ __device__ __forceinline__ uint8_t classify_rule(uint8_t t0, int kept, double z,
 // decides every pair outside [lo, hi], a band around re^2 wider than the
 // FP32 error bound (see cnl_prefilter_band); pairs inside it take the
 // reference's FP64 test, so the result is exactly the FP64 predicate.
-constexpr int CNL_GROUP = 8, CNL_CHUNKS = 4;
-__device__ __forceinline__ bool cnl_near(const float4 q, const float4 pf, int s2, const double4 pi,
-                                         const double4* __restrict__ spos, double re2, float lo, float hi) {
+constexpr int CNL_GROUP = 8, CNL_EPI = 4;
+static_assert(WAVE <= CNL_PAD + 1, "sposf padding covers one staged block");
+
+// One chunk of candidates, one per lane: slot sc with FP32 offset record q,
+// `take` false for slots outside the group's run and for the particle
+// itself. The FP32 decision comes first, as wavefront masks; the rare
+// pairs inside the band take the FP64 test in a branch that is uniform
+// over the wavefront, so the common path has no per-candidate exec-mask
+// branches. A ballot ranks every hit among its group's hits, which keeps
+// the visiting order; hits go to the group's LDS row up to maxn.
+__device__ __forceinline__ void cnl_chunk(const float4 q, int sc, bool take, const float4 pf, const double4 pi,
+                                          const double4* __restrict__ spos, double re2, float lo, float mid, float hw,
+                                          int lane, int gshift, unsigned below, int maxn, int* buf, int& found) {
   const float dx = q.x - pf.x, dy = q.y - pf.y, dz = q.z - pf.z;
-  const float d2 = dx * dx + dy * dy + dz * dz;
-  if (d2 < lo) return true;
-  if (d2 > hi) return false;
-  const double4 o = spos[s2];
-  const double ddx = o.x - pi.x, ddy = o.y - pi.y, ddz = o.z - pi.z;
-  return ddx * ddx + ddy * ddy + ddz * ddz < re2;
+  const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  const float dv = take ? d2 : __builtin_huge_valf();
+  uint64_t hm = __builtin_amdgcn_ballot_w64(dv < lo);
+  const uint64_t bm = __builtin_amdgcn_ballot_w64(fabsf(dv - mid) <= hw);
+  if (bm) {
+    bool f = false;
+    if ((bm >> lane) & 1ull) {
+      const double4 o = spos[sc];
+      const double ddx = o.x - pi.x, ddy = o.y - pi.y, ddz = o.z - pi.z;
+      f = ddx * ddx + ddy * ddy + ddz * ddz < re2;
+    }
+    hm = (hm & ~bm) | __builtin_amdgcn_ballot_w64(f);
+  }
+  const unsigned mk = (unsigned)(hm >> gshift) & 0xFFu;
+  if (__builtin_amdgcn_inverse_ballot_w64(hm)) {
+    const int pos = found + __popc(mk & below);
+    if (pos < maxn) buf[pos] = sc;
+  }
+  found += __popc(mk);
 }
 template <bool CLASSIFY>
 __global__ __launch_bounds__(256) void k_cnl_pass(const double4* __restrict__ spos, const float4* __restrict__ sposf,
-                                                  const int* __restrict__ start, const int* __restrict__ scell, int n,
-                                                  int gx, int gy, int gz, double re2, double re2c, float lo, float hi,
+                                                  const int* __restrict__ start, const int* __restrict__ sflag, int n,
+                                                  int gx, int gy, int gz, double re2, double re2c, float lo,
+                                                  float mid, float hw, float eta, float loc, float midc, float hwc,
                                                   int maxn,
                                                   int* __restrict__ nbr, int* __restrict__ cnt,
                                                   uint8_t* __restrict__ type_s, double zlim,
                                                   unsigned long long* __restrict__ trunc_total) {
-  extern __shared__ int rowbuf[];  // (blockDim.x / CNL_GROUP) rows of maxn
+  // LDS: (blockDim.x / CNL_GROUP) rows of maxn hits, then WAVE staged
+  // candidate records per wavefront and WAVE records of tail padding.
+  extern __shared__ int rowbuf[];
   const int lane = threadIdx.x & (WAVE - 1), g = lane & (CNL_GROUP - 1), sub = threadIdx.x / CNL_GROUP;
   const int per_block = blockDim.x / CNL_GROUP;
   const int s = (int)xcd_block(blockIdx.x, gridDim.x) * per_block + sub;
   const bool live = s < n;
   int* buf = rowbuf + sub * maxn;
+  float4* stage = reinterpret_cast<float4*>(rowbuf + per_block * maxn) + (threadIdx.x & ~(WAVE - 1));
   int found = 0;
   double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+  float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+  int fl = 0;
   if (live) {
     pi = spos[s];
-    const float4 pf = sposf[s];
-    const int c = scell[s];
-    const int cxl = c % gx, t = c / gx, cyl = t % gy, czl = t / gy;
-    const int x0 = cxl > 0 ? cxl - 1 : 0, x1 = cxl + 1 < gx ? cxl + 1 : gx - 1;
-    const unsigned below = (1u << g) - 1u;
-    const int gshift = lane & ~(CNL_GROUP - 1);
-    // All nine run bounds first (independent loads, one latency), then the
-    // runs in visiting order.
-    int rb[9], re_[9];
+    pf = sposf[s];
+    fl = sflag[s];
+  }
+  // Cells are x-fastest, so run r (rows dz = r / 3 - 1, dy = r % 3 - 1)
+  // spans cells [c - 1, c + 1] + dz gx gy + dy gx, clipped at the grid's
+  // x faces; rows outside the grid are empty. A dead lane (s >= n) has
+  // nine empty runs.
+  const unsigned c = (unsigned)__float_as_int(pf.w);
+  const unsigned clo = c - ((fl & CNL_XLO) ? 1u : 0u), chi = c + ((fl & CNL_XHI) ? 2u : 1u);
+  const unsigned below = (1u << g) - 1u;
+  const int gshift = lane & ~(CNL_GROUP - 1);
+  int rb[9], re_[9];
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const int zz = czl + r / 3 - 1, yy = cyl + r % 3 - 1;
-      const bool ok = zz >= 0 && zz < gz && yy >= 0 && yy < gy;
-      const int rowc = ok ? (zz * gy + yy) * gx : 0;
-      rb[r] = ok ? start[rowc + x0] : 0;
-      re_[r] = ok ? start[rowc + x1 + 1] : 0;
-    }
+  for (int r = 0; r < 9; ++r) {
+    const int dz = r / 3 - 1, dy = r % 3 - 1;
+    const int need = (dz < 0 ? CNL_ZLO : dz > 0 ? CNL_ZHI : 0) | (dy < 0 ? CNL_YLO : dy > 0 ? CNL_YHI : 0);
+    const bool ok = live && (fl & need) == need;
+    const int* sr = start + ((ptrdiff_t)dz * gy + dy) * gx;
+    rb[r] = ok ? sr[clo] : 0;
+    re_[r] = ok ? sr[chi] : 0;
+  }
+  // The wavefront's eight particles are consecutive in cell order, so for
+  // each run index their non-empty runs start and end in lane order and
+  // their union is one slot range [u0, u1), from the first and the last of
+  // them. When it holds at most WAVE slots (nearly always), the wavefront
+  // stages it through LDS with one coalesced load, and each group steps
+  // through its own run eight slots at a time. Otherwise (particles far
+  // apart, in a sparse region) each group loads its own run.
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      {
-        const int b = rb[r], e = re_[r];
-        // Four chunks of eight per iteration: a run (three cells, ~28
-        // candidates at the usual density) mostly takes one iteration
-        // with all four loads in flight.
-        for (int base = b; base < e; base += CNL_CHUNKS * CNL_GROUP) {
-          float4 q[CNL_CHUNKS];
+  for (int r = 0; r < 9; ++r) {
+    const int b = rb[r], e = re_[r];
+    const uint64_t ne = __builtin_amdgcn_ballot_w64(b < e);
+    if (ne == 0) continue;
+    const int u0 = __builtin_amdgcn_readlane(b, __builtin_ctzll(ne));
+    const int u1 = __builtin_amdgcn_readlane(e, 63 - __builtin_clzll(ne));
+    if (u1 - u0 <= WAVE) {
+      // sposf holds CNL_PAD records past n, so this load needs no mask.
+      stage[lane] = sposf[u0 + lane];
+      wave_sync();
+      // Lane g reads record (b - u0) + g + 8 j: below 2 WAVE for every
+      // chunk the wavefront takes (b - u0 < WAVE, 8 j < WAVE), so reads
+      // past this run land in the next wavefront's records or the padding.
+      // An empty run takes no slot (rem <= 0), wherever it lies.
+      const int bq = b < e ? b : u0;
+      const float4* sq = stage + (bq - u0) + g;
+      const int rem = (b < e ? e - b : 0) - g;
 #pragma unroll
-          for (int u = 0; u < CNL_CHUNKS; ++u) {
-            const int sc = base + u * CNL_GROUP + g;
-            q[u] = sc < e ? sposf[sc] : make_float4(0.f, 0.f, 0.f, 0.f);
-          }
-#pragma unroll
-          for (int u = 0; u < CNL_CHUNKS; ++u) {
-            const int sc = base + u * CNL_GROUP + g;
-            const bool h = sc < e && sc != s && cnl_near(q[u], pf, sc, pi, spos, re2, lo, hi);
-            const unsigned mk = (unsigned)(__ballot(h) >> gshift) & 0xFFu;
-            if (h) {
-              const int pos = found + __popc(mk & below);
-              if (pos < maxn) buf[pos] = sc;
-            }
-            found += __popc(mk);
-          }
-        }
+      for (int j = 0; j < WAVE / CNL_GROUP; ++j) {
+        const bool inrun = j * CNL_GROUP < rem;
+        if (__builtin_amdgcn_ballot_w64(inrun) == 0) break;
+        const int sc = bq + j * CNL_GROUP + g;
+        cnl_chunk(sq[j * CNL_GROUP], sc, inrun && (r != 4 || sc != s), pf, pi, spos, re2, lo, mid, hw, lane, gshift,
+                  below, maxn, buf, found);
+      }
+      wave_sync();  // this run's reads are done before the next run's writes
+    } else {
+      for (int base = b; base < e; base += CNL_GROUP) {
+        const int sc = base + g;
+        cnl_chunk(sposf[sc], sc, sc < e && (r != 4 || sc != s), pf, pi, spos, re2, lo, mid, hw, lane, gshift, below,
+                  maxn, buf, found);
       }
     }
   }
-  __syncthreads();
+  // Each row was written by its own group, in this wavefront.
+  wave_sync();
   const int kept = found < maxn ? found : maxn;
   // Classification counts and takes sectors of the kept neighbors within
   // the classification radius (the list radius unless the lists carry a
   // skin, in which case it drops the skin-only entries).
+  // Both come from the FP32 offset records where the FP32 error bound eta
+  // decides them (every sign and magnitude comparison, and the radius test
+  // outside its band); the rest take the FP64 positions, so the result is
+  // the FP64 one. Each lane takes four neighbors at a time, loads first.
   unsigned mask = 0;
   int kin = 0;
   if (live) {
     int* out = nbr + (size_t)s * maxn;
-    for (int k = g; k < kept; k += CNL_GROUP) {
-      const int s2 = buf[k];
-      out[k] = s2;
+    const bool skin = re2c < re2;
+    const float es = 1.5f * eta, ec = 3.0f * eta;
+    for (int k0 = 0; k0 < kept; k0 += CNL_EPI * CNL_GROUP) {
+      int s2[CNL_EPI];
+      bool v[CNL_EPI];
+#pragma unroll
+      for (int u = 0; u < CNL_EPI; ++u) {
+        const int k = k0 + u * CNL_GROUP + g;
+        v[u] = k < kept;
+        s2[u] = buf[v[u] ? k : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < CNL_EPI; ++u)
+        if (v[u]) out[k0 + u * CNL_GROUP + g] = s2[u];
       if (CLASSIFY) {
-        const double4 q = spos[s2];
-        const double dx = q.x - pi.x, dy = q.y - pi.y, dz = q.z - pi.z;
-        if (dx * dx + dy * dy + dz * dz < re2c) {
-          mask |= sector_bits(dx, dy, dz);
-          ++kin;
+        float4 q[CNL_EPI];
+#pragma unroll
+        for (int u = 0; u < CNL_EPI; ++u) q[u] = sposf[v[u] ? s2[u] : s];
+#pragma unroll
+        for (int u = 0; u < CNL_EPI; ++u) {
+          const float dx = q[u].x - pf.x, dy = q[u].y - pf.y, dz = q[u].z - pf.z;
+          bool sure;
+          unsigned bits = sector_bits_f32(dx, dy, dz, es, ec, &sure);
+          bool inside = true;
+          if (skin) {
+            const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            inside = d2 < loc;
+            sure = sure && !(fabsf(d2 - midc) <= hwc);
+          }
+          if (v[u] && !sure) {
+            const double4 o = spos[s2[u]];
+            const double ex = o.x - pi.x, ey = o.y - pi.y, ez = o.z - pi.z;
+            bits = sector_bits(ex, ey, ez);
+            inside = ex * ex + ey * ey + ez * ez < re2c;
+          }
+          const bool take = v[u] && inside;
+          mask |= take ? bits : 0u;
+          kin += take ? 1 : 0;
         }
       }
     }
@@ -382,8 +482,8 @@ void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t 
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart, n, B.perm);
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks(ncells, 256)), dim3(256), 0, st, B.cstart, (int)ncells, B.perm);
-    hipLaunchKernelGGL(k_gather_sorted, dim3(blocks(n, 256)), dim3(256), 0, st, B.perm, B.cell, B.x, B.y, B.z, n,
-                       B.ox, B.oy, B.oz, B.spos, B.sposf, B.scell, B.iperm);
+    hipLaunchKernelGGL(k_gather_sorted, dim3(blocks(n, 256)), dim3(256), 0, st, B.perm, B.cell, B.gx, B.gy, B.gz, B.x,
+                       B.y, B.z, n, B.ox, B.oy, B.oz, B.spos, B.sposf, B.sflag, B.iperm);
   }
 }
 
@@ -392,15 +492,30 @@ void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t 
 // rounded once from FP64 (error <= E 2^-24 for offsets up to E, the grid
 // extent), a difference adds |d| 2^-24 with |d| <= 2 re near the boundary,
 // so each FP32 difference is within eta = 2 E 2^-24 + 2 re 2^-24 of the
-// exact one; the squared norm then errs by at most 2 sqrt(3) eta |d| +
-// 3 eta^2 plus four roundings of ~re^2 2^-24. Twice that, plus a relative
-// 1e-9 for the FP64 side, is the half-width.
+// exact one; the squared norm (a product and two FMAs, three roundings)
+// then errs by at most 2 sqrt(3) eta |d| + 3 eta^2 plus three roundings of
+// ~re^2 2^-24 (the bound allows four). Twice that, plus a relative 1e-9 for
+// the FP64 side, is the half-width.
+static double cnl_eta(double re, double extent) { return 2.0 * extent * 0x1.0p-24 + 2.0 * re * 0x1.0p-24; }
 static void cnl_prefilter_band(double re2, double extent, float* lo, float* hi) {
   const double u = 0x1.0p-24, re = std::sqrt(re2);
-  const double eta = 2.0 * extent * u + 2.0 * re * u;
+  const double eta = cnl_eta(re, extent);
   const double m = 2.0 * (2.0 * std::sqrt(3.0) * eta * (1.01 * re) + 3.0 * eta * eta + 4.0 * 1.1 * re2 * u) + 1e-9 * re2;
   *lo = (float)(re2 - m);
   *hi = (float)(re2 + m);
+}
+
+// Band test for the kernels: |d2 - mid| <= hw holds for every d2 in the
+// band [lo, hi]: d2 - mid is exact there (Sterbenz), and hw covers half the
+// width plus the rounding of mid.
+static void cnl_band_test(double r2, double extent, float* lo, float* mid, float* hw) {
+  float hi;
+  cnl_prefilter_band(r2, extent, lo, &hi);
+  const double mid_d = 0.5 * ((double)*lo + (double)hi);
+  *mid = (float)mid_d;
+  const double hw_d = 0.5 * ((double)hi - (double)*lo) + std::fabs((double)*mid - mid_d);
+  *hw = (float)hw_d;
+  if ((double)*hw < hw_d) *hw = std::nextafter(*hw, INFINITY);
 }
 
 void launch_cnl_pass(const CnlBufs& B, int n, double re2, double re2c, int maxn, bool classify, double zlim,
@@ -409,18 +524,25 @@ void launch_cnl_pass(const CnlBufs& B, int n, double re2, double re2c, int maxn,
   if (n <= 0) return;
   // 256 threads (32 particles) per block while the LDS rows stay <= 32 KB.
   const int threads = maxn <= 256 ? 256 : WAVE;
-  const size_t lds = (size_t)(threads / CNL_GROUP) * (size_t)(maxn > 0 ? maxn : 1) * sizeof(int);
+  const size_t lds = (size_t)(threads / CNL_GROUP) * (size_t)(maxn > 0 ? maxn : 1) * sizeof(int) +
+                     (size_t)(threads + WAVE) * sizeof(float4);
   const unsigned grid = blocks(n, threads / CNL_GROUP);
   const double extent = std::sqrt(re2) * (double)(std::max(B.gx, std::max(B.gy, B.gz)) + 1);
-  float lo, hi;
-  cnl_prefilter_band(re2, extent, &lo, &hi);
+  float lo, mid, hw, loc, midc, hwc;
+  cnl_band_test(re2, extent, &lo, &mid, &hw);
+  cnl_band_test(std::min(re2c, re2), extent, &loc, &midc, &hwc);
+  // Bound on the error of an FP32 offset component (for the sectors),
+  // rounded up.
+  const double eta_d = cnl_eta(std::sqrt(re2), extent);
+  float eta = (float)eta_d;
+  if ((double)eta < eta_d) eta = std::nextafter(eta, INFINITY);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
   if (classify)
-    hipLaunchKernelGGL(k_cnl_pass<true>, dim3(grid), dim3(threads), lds, st, B.spos, B.sposf, B.cstart, B.scell, n,
-                       B.gx, B.gy, B.gz, re2, re2c, lo, hi, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
+    hipLaunchKernelGGL(k_cnl_pass<true>, dim3(grid), dim3(threads), lds, st, B.spos, B.sposf, B.cstart, B.sflag, n,
+                       B.gx, B.gy, B.gz, re2, re2c, lo, mid, hw, eta, loc, midc, hwc, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
   else
-    hipLaunchKernelGGL(k_cnl_pass<false>, dim3(grid), dim3(threads), lds, st, B.spos, B.sposf, B.cstart, B.scell, n,
-                       B.gx, B.gy, B.gz, re2, re2c, lo, hi, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
+    hipLaunchKernelGGL(k_cnl_pass<false>, dim3(grid), dim3(threads), lds, st, B.spos, B.sposf, B.cstart, B.sflag, n,
+                       B.gx, B.gy, B.gz, re2, re2c, lo, mid, hw, eta, loc, midc, hwc, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
   if (ev_end) (void)hipEventRecord(ev_end, st);
   const unsigned gs = blocks(n, 256) < 1024u ? blocks(n, 256) : 1024u;
   hipLaunchKernelGGL(k_cnl_stats, dim3(gs), dim3(256), (size_t)(maxn + 6) * sizeof(int), st, B.cnt, type_s, n, maxn,

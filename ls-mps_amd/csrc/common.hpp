@@ -59,11 +59,11 @@ __device__ __forceinline__ float group8_sum(float v) {
   v += __int_as_float(dpp_i<DPP_HALF_MIRROR>(__float_as_int(v)));
   return v;
 }
-// Sum over aligned groups of G lanes (G = 2, 4 or 8), by the same DPP steps.
+// Sum over aligned groups of G lanes (G = 1, 2, 4 or 8), by the same DPP steps.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
-  static_assert(G == 2 || G == 4 || G == 8, "group of 2, 4 or 8 lanes");
-  v += __int_as_float(dpp_i<DPP_QUAD_SWAP1>(__float_as_int(v)));
+  static_assert(G == 1 || G == 2 || G == 4 || G == 8, "group of 1, 2, 4 or 8 lanes");
+  if (G >= 2) v += __int_as_float(dpp_i<DPP_QUAD_SWAP1>(__float_as_int(v)));
   if (G >= 4) v += __int_as_float(dpp_i<DPP_QUAD_SWAP2>(__float_as_int(v)));
   if (G >= 8) v += __int_as_float(dpp_i<DPP_HALF_MIRROR>(__float_as_int(v)));
   return v;
@@ -84,6 +84,14 @@ __device__ __forceinline__ unsigned group8_or(unsigned v) {
   v |= (unsigned)dpp_i<DPP_QUAD_SWAP2>((int)v);
   v |= (unsigned)dpp_i<DPP_HALF_MIRROR>((int)v);
   return v;
+}
+
+// Orders LDS accesses between the lanes of one wavefront (which execute in
+// lockstep): everything before it is visible to every lane after it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Inclusive prefix sum over the wavefront.

@@ -33,15 +33,22 @@ void launch_sum_counts(const unsigned* partials, int nblk, CountSlots slots, uns
                        hipStream_t st);
 
 // cnl.hip: buffers of the cell list (device pointers).
+// Records allocated past n in sposf: the neighbor pass loads a whole block
+// of candidates without masking the ones past the end of a run.
+constexpr int CNL_PAD = 64;
+// Grid-boundary flags of a particle's cell: a neighbor cell exists below /
+// above it along x, y, z.
+constexpr int CNL_XLO = 1, CNL_XHI = 2, CNL_YLO = 4, CNL_YHI = 8, CNL_ZLO = 16, CNL_ZHI = 32;
 struct CnlBufs {
   int gx = 0, gy = 0, gz = 0;
   const double *x = nullptr, *y = nullptr, *z = nullptr;  // original order
   int *cx = nullptr, *cy = nullptr, *cz = nullptr, *cell = nullptr, *slot = nullptr;
   int *ccount = nullptr, *cstart = nullptr;               // per cell
   int64_t* scan_scratch = nullptr;
-  int *perm = nullptr, *iperm = nullptr, *scell = nullptr;  // sorted slot <-> particle
+  int *perm = nullptr, *iperm = nullptr;                    // sorted slot <-> particle
+  int* sflag = nullptr;                                     // sorted CNL_* grid-boundary flags of the cell
   double4* spos = nullptr;                                  // sorted {x, y, z, field}
-  float4* sposf = nullptr;                                  // sorted FP32 offsets from (ox, oy, oz)
+  float4* sposf = nullptr;  // sorted {FP32 offsets from (ox, oy, oz), cell id as int bits}, n + CNL_PAD
   double ox = 0.0, oy = 0.0, oz = 0.0;                      // grid origin
   int *nbr = nullptr, *cnt = nullptr, *cnt_orig = nullptr;  // ELL table
 };
