@@ -66,6 +66,10 @@ struct GraphCache {
 
 }  // namespace
 
+// Pinned readback area: 16 64-bit counters, then up to 512 histogram bins
+// (max_neighbors <= 256).
+constexpr size_t HOST_RB_BYTES = 16 * sizeof(unsigned long long) + 512 * sizeof(int);
+
 struct lsmps_ctx {
   int device = 0;
   hipStream_t st = nullptr;
@@ -91,7 +95,7 @@ struct lsmps_ctx {
   double re = 0.0;
   int maxn = 0, gx = 0, gy = 0, gz = 0;
   uint64_t pairs = 0;
-  DBuf<int> cx, cy, cz, box, cell, slot, ccount, cstart, perm, iperm, sflag, nbr, cnt, cnt_orig, row_ptr, col, hist;
+  DBuf<int> cx, cy, cz, box, cell, slot, skey, ccount, cstart, perm, iperm, sflag, nbr, cnt, cnt_orig, row_ptr, col, hist;
   DBuf<double4> spos;
   DBuf<float4> sposf;
   double ox = 0.0, oy = 0.0, oz = 0.0;  // grid origin of the FP32 offsets
@@ -128,6 +132,7 @@ struct lsmps_ctx {
   DBuf<SolverScalars> sc;
   SolverScalars* host_sc = nullptr;  // pinned
   int* host_small = nullptr;         // pinned scratch for small readbacks (64 ints)
+  unsigned long long* host_rb = nullptr;  // pinned: 16 counters, then up to 512 histogram bins
   bool have_x0 = false, have_solution = false;
   GraphCache graph[2];  // captured iterations, one per solve precision (FP64, FP32)
 
@@ -295,6 +300,16 @@ void zero_counters(lsmps_ctx* ctx, int k) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->st, ctx->counters.p, k);
 }
 
+// Counters (and the neighbor histogram) come back through pinned memory: a
+// copy into pageable memory takes the runtime's staged path, tens of
+// microseconds per call. Waits for the stream.
+hipError_t read_counters(lsmps_ctx* ctx, unsigned long long* c, size_t k) {
+  hipError_t e = hipMemcpyAsync(ctx->host_rb, ctx->counters.p, k * sizeof(*c), hipMemcpyDeviceToHost, ctx->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+  if (e == hipSuccess) std::memcpy(c, ctx->host_rb, k * sizeof(*c));
+  return e;
+}
+
 }  // namespace
 
 extern "C" {
@@ -317,6 +332,7 @@ int lsmps_create(int device, lsmps_ctx** out) {
       hipEventCreate(&ctx->e2) != hipSuccess || hipEventCreate(&ctx->e3) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->host_sc), sizeof(SolverScalars)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->host_small), 64 * sizeof(int64_t)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&ctx->host_rb), HOST_RB_BYTES) != hipSuccess ||
       ctx->counters.ensure(16) != hipSuccess || ctx->sc.ensure(1) != hipSuccess ||
       ctx->partials.ensure(4 * SOLVER_MAX_GRID) != hipSuccess || ctx->box.ensure(8) != hipSuccess ||
       ctx->nredo.ensure(1) != hipSuccess) {
@@ -339,8 +355,8 @@ void lsmps_destroy(lsmps_ctx* ctx) {
                        &ctx->b,      &ctx->xe,   &ctx->diag, &ctx->scale, &ctx->yv,    &ctx->xsol,     &ctx->x0,
                        &ctx->vec64,  &ctx->partials, &ctx->sell_v64, &ctx->sell_vs64};
   for (auto* p : d) p->release();
-  DBuf<int>* i[] = {&ctx->cx,    &ctx->cy,     &ctx->cz,    &ctx->box,      &ctx->cell,     &ctx->slot,
-                    &ctx->ccount, &ctx->cstart, &ctx->perm,  &ctx->iperm,    &ctx->sflag,    &ctx->nbr,
+  DBuf<int>* i[] = {&ctx->cx,    &ctx->cy,     &ctx->cz,    &ctx->box,      &ctx->cell,     &ctx->skey,
+                    &ctx->ccount, &ctx->cstart, &ctx->slot, &ctx->perm,  &ctx->iperm,    &ctx->sflag,    &ctx->nbr,
                     &ctx->cnt,   &ctx->cnt_orig, &ctx->row_ptr, &ctx->col,   &ctx->hist,     &ctx->redo,
                     &ctx->nredo, &ctx->flag,   &ctx->pos,   &ctx->fs,       &ctx->upos,     &ctx->uid,
                     &ctx->uslot, &ctx->urr,    &ctx->rru,   &ctx->rlen,     &ctx->rcnt,     &ctx->prow,
@@ -377,6 +393,7 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->sc.release();
   if (ctx->host_sc) (void)hipHostFree(ctx->host_sc);
   if (ctx->host_small) (void)hipHostFree(ctx->host_small);
+  if (ctx->host_rb) (void)hipHostFree(ctx->host_rb);
   if (ctx->e0) (void)hipEventDestroy(ctx->e0);
   if (ctx->e1) (void)hipEventDestroy(ctx->e1);
   if (ctx->e2) (void)hipEventDestroy(ctx->e2);
@@ -482,6 +499,7 @@ CnlBufs cnl_bufs(lsmps_ctx* ctx) {
   B.cy = ctx->cy.p;
   B.cz = ctx->cz.p;
   B.cell = ctx->cell.p;
+  B.skey = ctx->skey.p;
   B.slot = ctx->slot.p;
   B.ccount = ctx->ccount.p;
   B.cstart = ctx->cstart.p;
@@ -524,14 +542,14 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   if ((uint64_t)n * (uint64_t)std::max(1, maxn) > ((uint64_t)1 << 40))
     return fail(ctx, LSMPS_E_LIMIT, "neighbor table too large");
   sync_types(ctx);  // with the previous order, before it is replaced
-  DBuf<int>* per_particle[] = {&ctx->cx,   &ctx->cy,    &ctx->cz,    &ctx->cell, &ctx->slot,
+  DBuf<int>* per_particle[] = {&ctx->cx,   &ctx->cy,    &ctx->cz,    &ctx->cell, &ctx->slot, &ctx->skey,
                                &ctx->perm, &ctx->iperm, &ctx->sflag, &ctx->cnt,  &ctx->cnt_orig};
   for (auto* b : per_particle) HIPCHK(b->ensure(n));
   HIPCHK(ctx->spos.ensure(n));
   HIPCHK(ctx->sposf.ensure((size_t)n + CNL_PAD));
   HIPCHK(ctx->type_s.ensure(n));
   HIPCHK(ctx->nbr.ensure((size_t)n * (size_t)std::max(1, maxn)));
-  HIPCHK(ctx->hist.ensure((size_t)maxn + 1));
+  HIPCHK(ctx->hist.ensure((size_t)maxn + 1 + (size_t)(maxn + 6) * CNL_STATS_BLOCKS));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
   launch_cell_coords(ctx->x.p, ctx->y.p, ctx->z.p, n, rl, ctx->cx.p, ctx->cy.p, ctx->cz.p, ctx->box.p, st);
@@ -563,16 +581,17 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   launch_gather_types(ctx->perm.p, ctx->type.p, n, ctx->type_s.p, st);
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
   zero_counters(ctx, 16);
-  launch_fill_int(ctx->hist.p, maxn + 1, 0, st);
+  if (n == 0) launch_fill_int(ctx->hist.p, maxn + 1, 0, st);  // the pass's reduction writes every bin
   launch_cnl_pass(B, n, rl * rl, re * re, maxn, classify, 1.5 * l0, ctx->type_s.p, ctx->counters.p, ctx->hist.p,
                   ctx->counters.p + 8, st, ctx->e2, ctx->e3);
   if (int rc = check_launch(ctx, "cnl_pass")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
   std::vector<int> hist((size_t)maxn + 1);
   unsigned long long c[16];
-  HIPCHK(hipMemcpyAsync(hist.data(), ctx->hist.p, hist.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  int* hist_rb = reinterpret_cast<int*>(ctx->host_rb + 16);
+  HIPCHK(hipMemcpyAsync(hist_rb, ctx->hist.p, hist.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(read_counters(ctx, c, 16));
+  std::memcpy(hist.data(), hist_rb, hist.size() * sizeof(int));
   lsmps_cnl_stats cs;
   fill_cnl_stats(ctx, hist, c[0], &cs);
   if (cs.pairs > (uint64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "neighbor pairs beyond int32 indexing");
@@ -748,8 +767,7 @@ int lsmps_classify(lsmps_ctx* ctx, double l0, lsmps_type_stats* out) {
   ctx->types_stale = true;
   HIPCHK(hipEventRecord(ctx->e1, st));
   unsigned long long c[5];
-  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(read_counters(ctx, c, sizeof(c) / sizeof(c[0])));
   ctx->have_wls = ctx->have_ppe = ctx->have_solution = false;
   if (out) {
     for (int k = 0; k < 5; ++k) out->count[k] = c[k];
@@ -825,8 +843,7 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   if (int rc = check_launch(ctx, "wls")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
   unsigned long long c[6];
-  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(read_counters(ctx, c, sizeof(c) / sizeof(c[0])));
   ctx->have_wls = true;
   if (out) {
     out->ok = c[WLS_OK];
@@ -980,8 +997,7 @@ int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0,
   if (int rc = check_launch(ctx, "ppe_fill")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
   unsigned long long c[4] = {0, 0, 0, 0};
-  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(read_counters(ctx, c, sizeof(c) / sizeof(c[0])));
   ctx->m = (uint64_t)m;
   ctx->ppe_re = p->re;
   ctx->nnz = m > 0 ? (uint64_t)c[2] : 0;
@@ -1132,8 +1148,7 @@ int lsmps_replay_wls(lsmps_ctx* ctx, double re, double l0, double reg, lsmps_wls
   if (int rc = check_launch(ctx, "replay_wls")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
   unsigned long long c[6];
-  HIPCHK(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(read_counters(ctx, c, sizeof(c) / sizeof(c[0])));
   ctx->have_wls = true;
   if (out) {
     out->ok = c[WLS_OK];

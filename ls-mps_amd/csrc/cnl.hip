@@ -82,46 +82,72 @@ __global__ void k_fill_int(int* __restrict__ p, int64_t n, int v) {
   if (i < n) p[i] = v;
 }
 
-// Linear cell id (x fastest) and an arrival slot inside the cell.
+// Counting sort by cell, stable by particle index, in four launches:
+//  k_cell_key    linear cell id (x fastest) and a slot inside the cell from
+//                an atomic on the cell's count. Inputs often come in
+//                lattice order, so consecutive particles share a cell: a run
+//                of equal cells among consecutive lanes takes its slots with
+//                one atomic from its first lane (about half the atomics).
+//  (scan)        exclusive scan of the counts: the cells' first slots.
+//  k_scatter     particle and cell id to start + slot (arrival order).
+//  k_cell_rank   each slot's rank among its cell's particle indices, so
+//                perm holds every cell's particles in ascending index order.
 __global__ __launch_bounds__(256) void k_cell_key(const int* __restrict__ cx, const int* __restrict__ cy,
                                                   const int* __restrict__ cz, int n, int bx, int by, int bz,
                                                   int gx, int gy, int* __restrict__ cell, int* __restrict__ slot,
                                                   int* __restrict__ count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int c = ((cz[i] - bz) * gy + (cy[i] - by)) * gx + (cx[i] - bx);
-  cell[i] = c;
-  slot[i] = atomicAdd(&count[c], 1);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const bool valid = i < n;
+  const int c = valid ? ((cz[i] - bz) * gy + (cy[i] - by)) * gx + (cx[i] - bx) : -1;
+  const int prev = __shfl_up(c, 1);
+  const bool head = valid && (lane == 0 || c != prev);
+  const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
+  const int nvalid = __popcll(__builtin_amdgcn_ballot_w64(valid));  // lanes 0 .. nvalid - 1
+  // This lane's run starts at the last head at or below it and ends at the
+  // next head (or the first invalid lane).
+  const uint64_t upto = heads & (((uint64_t)2 << lane) - 1u);
+  const int rs = valid ? 63 - __builtin_clzll(upto) : lane;
+  const uint64_t later = heads & ~(((uint64_t)2 << rs) - 1u);
+  const int rend = later ? __builtin_ctzll(later) : nvalid;
+  int base = 0;
+  if (head) base = atomicAdd(&count[c], rend - rs);
+  base = __shfl(base, rs);
+  if (valid) {
+    cell[i] = c;
+    slot[i] = base + (lane - rs);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ cell, const int* __restrict__ slot,
-                                                 const int* __restrict__ start, int n, int* __restrict__ perm) {
+                                                 const int* __restrict__ start, int n, int* __restrict__ arrival,
+                                                 int* __restrict__ skey) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) perm[start[cell[i]] + slot[i]] = i;
+  if (i < n) {
+    const int c = cell[i], s = start[c] + slot[i];
+    arrival[s] = i;
+    skey[s] = c;
+  }
 }
 
-// Restores ascending particle order inside each cell (the atomic slots are
-// arrival order); cells hold ~10 particles, insertion sort per cell.
-__global__ __launch_bounds__(256) void k_cell_sort(const int* __restrict__ start, int ncells, int* __restrict__ perm) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  const int b = start[c], e = start[c + 1];
-  for (int k = b + 1; k < e; ++k) {
-    const int v = perm[k];
-    int m = k - 1;
-    while (m >= b && perm[m] > v) {
-      perm[m + 1] = perm[m];
-      --m;
-    }
-    perm[m + 1] = v;
-  }
+// Slot s of cell c holds arrival[s]; its place in ascending index order is
+// start[c] + (number of the cell's particles with a smaller index). The
+// cell's indices are read by every lane of the cell (contiguous, cached).
+__global__ __launch_bounds__(256) void k_cell_rank(const int* __restrict__ arrival, const int* __restrict__ skey,
+                                                   const int* __restrict__ start, int n, int* __restrict__ perm) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int c = skey[s], b = start[c], e = start[c + 1], v = arrival[s];
+  int r = 0;
+  for (int j = b; j < e; ++j) r += arrival[j] < v ? 1 : 0;
+  perm[b + r] = v;
 }
 
 // Sorted-slot copies of positions ({x, y, z, field} records, the field
 // filled in later), {FP32 offsets from the grid origin, cell id} records
 // (for the candidate prefilter), the cell's grid-boundary flags, and the
 // inverse permutation.
-__global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ perm, const int* __restrict__ cell,
+__global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ perm, const int* __restrict__ skey,
                                                        int gx, int gy, int gz, const double* __restrict__ x,
                                                        const double* __restrict__ y, const double* __restrict__ z,
                                                        int n, double ox, double oy, double oz,
@@ -132,7 +158,7 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ p
   const int i = perm[s];
   const double a = x[i], b = y[i], c = z[i];
   spos[s] = make_double4(a, b, c, 0.0);
-  const int cl = cell[i];
+  const int cl = skey[s];
   sposf[s] = make_float4((float)(a - ox), (float)(b - oy), (float)(c - oz), __int_as_float(cl));
   const int u = cl % gx, t = cl / gx, v = t % gy, w = t / gy;
   sflag[s] = (u > 0 ? CNL_XLO : 0) | (u < gx - 1 ? CNL_XHI : 0) | (v > 0 ? CNL_YLO : 0) | (v < gy - 1 ? CNL_YHI : 0) |
@@ -382,22 +408,51 @@ __global__ __launch_bounds__(256) void k_cnl_pass(const double4* __restrict__ sp
 }
 
 // Histogram of the kept counts and (optionally) the type counts, from the
-// sorted-slot arrays; LDS bins per block, one flush per block.
+// sorted-slot arrays: LDS bins per block for the counts; the types (most
+// particles share one) are counted with ballots per wavefront instead of
+// same-address LDS atomics. Each block's bins go to its column of `part`
+// (bin k of block b at part[k * gridDim.x + b]), then one block per bin adds
+// its row. (Flushing with global atomics put ~1000 same-address atomics on
+// each busy bin, serialised across the XCDs.)
 __global__ __launch_bounds__(256) void k_cnl_stats(const int* __restrict__ cnt, const uint8_t* __restrict__ type_s,
-                                                   int n, int maxn, int* __restrict__ hist,
-                                                   unsigned long long* __restrict__ tcount) {
+                                                   int n, int maxn, bool types, int* __restrict__ part) {
   extern __shared__ int bins[];  // maxn + 1 count bins, then 5 type bins
-  for (int k = threadIdx.x; k <= maxn + 5; k += blockDim.x) bins[k] = 0;
+  const int nb = maxn + 6;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) bins[k] = 0;
   __syncthreads();
+  int tc[5] = {0, 0, 0, 0, 0};
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     atomicAdd(&bins[cnt[s]], 1);
-    if (tcount) atomicAdd(&bins[maxn + 1 + type_s[s]], 1);
+    if (types) {
+      const int t = type_s[s];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) tc[k] += __popcll(__builtin_amdgcn_ballot_w64(t == k));
+    }
   }
+  if (types && (threadIdx.x & (WAVE - 1)) == 0)
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (tc[k]) atomicAdd(&bins[maxn + 1 + k], tc[k]);
   __syncthreads();
-  for (int k = threadIdx.x; k <= maxn; k += blockDim.x)
-    if (bins[k]) atomicAdd(&hist[k], bins[k]);
-  if (tcount && threadIdx.x < 5 && bins[maxn + 1 + threadIdx.x])
-    atomicAdd(&tcount[threadIdx.x], (unsigned long long)bins[maxn + 1 + threadIdx.x]);
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) part[(size_t)k * gridDim.x + blockIdx.x] = bins[k];
+}
+
+__global__ __launch_bounds__(256) void k_cnl_stats_reduce(const int* __restrict__ part, int nblk, int maxn,
+                                                          int* __restrict__ hist,
+                                                          unsigned long long* __restrict__ tcount) {
+  __shared__ unsigned long long lds[256 / WAVE];
+  const int k = blockIdx.x;
+  unsigned long long s = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += (unsigned)part[(size_t)k * nblk + b];
+  s = wave_sum(s);
+  if ((threadIdx.x & (WAVE - 1)) == 0) lds[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < 256 / WAVE; ++w) t += lds[w];
+    if (k <= maxn) hist[k] = (int)t;
+    else tcount[k - maxn - 1] = t;
+  }
 }
 
 // Stand-alone classification over the ELL table (src/lsmps.cpp:27,73),
@@ -480,9 +535,11 @@ void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t 
                        box_host[1], box_host[2], B.gx, B.gy, B.cell, B.slot, B.ccount);
   launch_exclusive_scan(B.ccount, ncells, B.cstart, B.scan_scratch, st);
   if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart, n, B.perm);
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(ncells, 256)), dim3(256), 0, st, B.cstart, (int)ncells, B.perm);
-    hipLaunchKernelGGL(k_gather_sorted, dim3(blocks(n, 256)), dim3(256), 0, st, B.perm, B.cell, B.gx, B.gy, B.gz, B.x,
+    // iperm holds the arrival order until k_gather_sorted writes it.
+    hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart, n, B.iperm,
+                       B.skey);
+    hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n, 256)), dim3(256), 0, st, B.iperm, B.skey, B.cstart, n, B.perm);
+    hipLaunchKernelGGL(k_gather_sorted, dim3(blocks(n, 256)), dim3(256), 0, st, B.perm, B.skey, B.gx, B.gy, B.gz, B.x,
                        B.y, B.z, n, B.ox, B.oy, B.oz, B.spos, B.sposf, B.sflag, B.iperm);
   }
 }
@@ -544,9 +601,13 @@ void launch_cnl_pass(const CnlBufs& B, int n, double re2, double re2c, int maxn,
     hipLaunchKernelGGL(k_cnl_pass<false>, dim3(grid), dim3(threads), lds, st, B.spos, B.sposf, B.cstart, B.sflag, n,
                        B.gx, B.gy, B.gz, re2, re2c, lo, mid, hw, eta, loc, midc, hwc, maxn, B.nbr, B.cnt, type_s, zlim, trunc_total);
   if (ev_end) (void)hipEventRecord(ev_end, st);
-  const unsigned gs = blocks(n, 256) < 1024u ? blocks(n, 256) : 1024u;
+  // hist holds maxn + 1 bins followed by (maxn + 6) * CNL_STATS_BLOCKS partials.
+  const unsigned gs = std::min(blocks(n, 256), (unsigned)CNL_STATS_BLOCKS);
+  int* part = hist + maxn + 1;
   hipLaunchKernelGGL(k_cnl_stats, dim3(gs), dim3(256), (size_t)(maxn + 6) * sizeof(int), st, B.cnt, type_s, n, maxn,
-                     hist, classify ? tcount : nullptr);
+                     classify, part);
+  hipLaunchKernelGGL(k_cnl_stats_reduce, dim3(classify ? maxn + 6 : maxn + 1), dim3(256), 0, st, part, (int)gs, maxn,
+                     hist, tcount);
 }
 
 void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* type_s, unsigned long long* tcount,

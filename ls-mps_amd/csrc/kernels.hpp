@@ -36,6 +36,10 @@ void launch_sum_counts(const unsigned* partials, int nblk, CountSlots slots, uns
 // Records allocated past n in sposf: the neighbor pass loads a whole block
 // of candidates without masking the ones past the end of a run.
 constexpr int CNL_PAD = 64;
+// Blocks of the neighbor-count histogram; the hist buffer passed to
+// launch_cnl_pass holds maxn + 1 bins plus (maxn + 6) * CNL_STATS_BLOCKS
+// partial counts.
+constexpr int CNL_STATS_BLOCKS = 1024;
 // Grid-boundary flags of a particle's cell: a neighbor cell exists below /
 // above it along x, y, z.
 constexpr int CNL_XLO = 1, CNL_XHI = 2, CNL_YLO = 4, CNL_YHI = 8, CNL_ZLO = 16, CNL_ZHI = 32;
@@ -43,7 +47,8 @@ struct CnlBufs {
   int gx = 0, gy = 0, gz = 0;
   const double *x = nullptr, *y = nullptr, *z = nullptr;  // original order
   int *cx = nullptr, *cy = nullptr, *cz = nullptr, *cell = nullptr, *slot = nullptr;
-  int *ccount = nullptr, *cstart = nullptr;               // per cell
+  int* skey = nullptr;                                      // cell ids in sorted order
+  int *ccount = nullptr, *cstart = nullptr;                 // per cell (cstart: ncells + 1)
   int64_t* scan_scratch = nullptr;
   int *perm = nullptr, *iperm = nullptr;                    // sorted slot <-> particle
   int* sflag = nullptr;                                     // sorted CNL_* grid-boundary flags of the cell
@@ -55,6 +60,8 @@ struct CnlBufs {
 void launch_cell_coords(const double* x, const double* y, const double* z, int n, double re, int* cx, int* cy,
                         int* cz, int* box, hipStream_t st);
 void launch_fill_int(int* p, int64_t n, int v, hipStream_t st);
+// Stable sort by cell and the sorted copies: perm, skey, cstart, spos,
+// sposf, sflag, iperm.
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t st);
 // Neighbor lists and classification work on sorted-order types only; the
 // original-order arrays are produced on demand by the unsort launchers.

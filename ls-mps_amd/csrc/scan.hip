@@ -1,6 +1,8 @@
 // Exclusive prefix sum of int32 counts into offsets (CSR row pointers, cell
 // starts). Three launches: tile sums, one-block scan of the tile sums, tile
 // rescans. Each tile is 256 threads x 8 consecutive items.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace lsmps_hip {
@@ -104,38 +106,44 @@ void launch_exclusive_scan(const int* in, int64_t n, int* out, int64_t* scratch,
   if (ntiles > 0) hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(SCAN_THREADS), 0, st, in, n, scratch, out);
 }
 
-// out[slot[k]] += sum over b of partials[k * nblk + b]: block k adds row k
-// with four independent loads in flight per thread (a row is a few hundred
-// KB at most, so the time is load latency, not bandwidth).
-constexpr int SUM_THREADS = 1024;
+// out[slot[k]] += sum over b of partials[k * nblk + b]. Row k is split over
+// gridDim.y blocks, each adding its share with a few loads in flight per
+// thread and one 64-bit atomic into out (a handful of atomics per launch,
+// so no contention; integer sums, so the order does not matter). One block
+// per row left the whole row to the load latency of 1024 threads (18 us for
+// 47k partials).
+constexpr int SUM_THREADS = 256;
 __global__ __launch_bounds__(SUM_THREADS) void k_sum_counts(const unsigned* __restrict__ partials, int nblk,
                                                             CountSlots slots, unsigned long long* __restrict__ out) {
   __shared__ unsigned long long lds[SUM_THREADS / WAVE];
   const int k = blockIdx.x;
   const unsigned* row = partials + (size_t)k * nblk;
-  unsigned long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  int b = threadIdx.x;
-  for (; b + 3 * SUM_THREADS < nblk; b += 4 * SUM_THREADS) {
+  unsigned long long s0 = 0, s1 = 0;
+  const int stride = SUM_THREADS * gridDim.y;
+  int b = blockIdx.y * SUM_THREADS + threadIdx.x;
+  for (; b + stride < nblk; b += 2 * stride) {
     s0 += row[b];
-    s1 += row[b + SUM_THREADS];
-    s2 += row[b + 2 * SUM_THREADS];
-    s3 += row[b + 3 * SUM_THREADS];
+    s1 += row[b + stride];
   }
-  for (; b < nblk; b += SUM_THREADS) s0 += row[b];
-  const unsigned long long s = wave_sum((s0 + s1) + (s2 + s3));
+  if (b < nblk) s0 += row[b];
+  const unsigned long long s = wave_sum(s0 + s1);
   if ((threadIdx.x & (WAVE - 1)) == 0) lds[threadIdx.x / WAVE] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
     for (int w = 0; w < SUM_THREADS / WAVE; ++w) t += lds[w];
-    out[slots.slot[k]] += t;
+    if (t) atomicAdd(&out[slots.slot[k]], t);
   }
 }
 
 void launch_sum_counts(const unsigned* partials, int nblk, CountSlots slots, unsigned long long* out,
                        hipStream_t st) {
-  if (nblk > 0 && slots.k > 0)
-    hipLaunchKernelGGL(k_sum_counts, dim3(slots.k), dim3(SUM_THREADS), 0, st, partials, nblk, slots, out);
+  if (nblk > 0 && slots.k > 0) {
+    // About four partials per thread.
+    const int per_block = 4 * SUM_THREADS;
+    const unsigned gy = (unsigned)std::min(64, std::max(1, (nblk + per_block - 1) / per_block));
+    hipLaunchKernelGGL(k_sum_counts, dim3(slots.k, gy), dim3(SUM_THREADS), 0, st, partials, nblk, slots, out);
+  }
 }
 
 }  // namespace lsmps_hip
