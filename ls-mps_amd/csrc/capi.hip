@@ -106,6 +106,7 @@ struct lsmps_ctx {
 
   // field and WLS (results in sorted order; original order on download)
   bool have_field = false, field_quadratic = false, have_wls = false;
+  bool spos_field = false;  // spos[].w holds the current field (written by the cell build or the WLS stage)
   QuadField qf{};
   DBuf<double> f, grad_s, lap_s, grad, lap, ge, le;
   DBuf<uint8_t> wstatus_s, wstatus, st_s, ty_s;
@@ -445,6 +446,7 @@ int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   ctx->have_features = false;
   ctx->have_cnl = ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_x0 = false;
   ctx->have_field = false;
+  ctx->spos_field = false;
   ctx->reset_graphs();
   return LSMPS_OK;
   GUARD_END
@@ -577,7 +579,9 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   HIPCHK(ctx->cstart.ensure((size_t)ncells + 1));
   HIPCHK(ctx->scan_scr.ensure(std::max(scan_scratch_elems(ncells), scan_scratch_elems((int64_t)n + 1))));
   const CnlBufs B = cnl_bufs(ctx);
-  launch_cell_sort(B, n, bx, st);
+  const int fmode = !ctx->have_field ? FIELD_NONE : ctx->field_quadratic ? FIELD_QUADRATIC : FIELD_VALUES;
+  launch_cell_sort(B, n, bx, fmode, ctx->qf, fmode == FIELD_VALUES ? ctx->f.p : nullptr, st);
+  ctx->spos_field = fmode != FIELD_NONE;
   launch_gather_types(ctx->perm.p, ctx->type.p, n, ctx->type_s.p, st);
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
   zero_counters(ctx, 16);
@@ -680,6 +684,11 @@ int lsmps_append_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   HIPCHK(hipStreamSynchronize(st));
   ctx->n = n1;
   if (uvw) ctx->have_u = true;
+  // A field given as values has none for the new particles.
+  if (count) {
+    ctx->have_field = ctx->have_field && ctx->field_quadratic;
+    ctx->spos_field = false;
+  }
   return LSMPS_OK;
   GUARD_END
 }
@@ -699,6 +708,7 @@ int lsmps_advance(lsmps_ctx* ctx, double dt) {
   // Fields and everything derived from positions are out of date.
   ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_features = false;
   ctx->have_field = ctx->have_field && ctx->field_quadratic;
+  ctx->spos_field = false;
   return LSMPS_OK;
   GUARD_END
 }
@@ -747,6 +757,7 @@ int lsmps_remove_outside(lsmps_ctx* ctx, const double* lo, const double* hi, uin
     ctx->n_cnl = 0;
     ctx->have_cnl = ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_features = false;
     ctx->have_field = ctx->have_field && ctx->field_quadratic;
+  ctx->spos_field = false;
   }
   if (removed) *removed = (uint64_t)(n - m);
   return LSMPS_OK;
@@ -785,6 +796,7 @@ int lsmps_set_field_quadratic(lsmps_ctx* ctx, const lsmps_quad_field* q) {
   ctx->qf.lap_exact = q->lap_exact;
   ctx->have_field = true;
   ctx->field_quadratic = true;
+  ctx->spos_field = false;
   ctx->have_wls = false;
   return LSMPS_OK;
   GUARD_END
@@ -799,6 +811,7 @@ int lsmps_set_field_values(lsmps_ctx* ctx, const double* f, uint64_t n) {
   if (n) HIPCHK(hipMemcpy(ctx->f.p, f, n * sizeof(double), hipMemcpyHostToDevice));
   ctx->have_field = true;
   ctx->field_quadratic = false;
+  ctx->spos_field = false;
   ctx->have_wls = false;
   return LSMPS_OK;
   GUARD_END
@@ -824,10 +837,13 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   HIPCHK(hipEventRecord(ctx->e0, st));
   // Field into the sorted records: evaluated there for a quadratic,
   // gathered for values.
-  if (ctx->field_quadratic)
-    launch_field_quadratic(ctx->spos.p, n, ctx->qf, st);
-  else
-    launch_gather_field(ctx->perm.p, ctx->f.p, n, ctx->spos.p, st);
+  if (!ctx->spos_field) {
+    if (ctx->field_quadratic)
+      launch_field_quadratic(ctx->spos.p, n, ctx->qf, st);
+    else
+      launch_gather_field(ctx->perm.p, ctx->f.p, n, ctx->spos.p, st);
+    ctx->spos_field = true;
+  }
   zero_counters(ctx, 16);
   const int* nb = ctx->nbr.p;
   const int* nc = ctx->cnt.p;
@@ -1105,6 +1121,7 @@ int lsmps_replay_load(lsmps_ctx* ctx, const int64_t* row_ptr, const int32_t* col
   ctx->have_u = false;
   ctx->have_cnl = true;
   ctx->have_field = ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_x0 = ctx->have_features = false;
+  ctx->spos_field = false;
   ctx->reset_graphs();
   if (out) {
     std::memset(out, 0, sizeof(*out));

@@ -143,8 +143,9 @@ __global__ __launch_bounds__(256) void k_cell_rank(const int* __restrict__ arriv
   perm[b + r] = v;
 }
 
-// Sorted-slot copies of positions ({x, y, z, field} records, the field
-// filled in later), {FP32 offsets from the grid origin, cell id} records
+// Sorted-slot copies of positions ({x, y, z, field} records; the field is
+// evaluated or gathered here when one is set, else 0), {FP32 offsets from
+// the grid origin, cell id} records
 // (for the candidate prefilter), the cell's grid-boundary flags, and the
 // inverse permutation.
 __global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ perm, const int* __restrict__ skey,
@@ -152,12 +153,14 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ p
                                                        const double* __restrict__ y, const double* __restrict__ z,
                                                        int n, double ox, double oy, double oz,
                                                        double4* __restrict__ spos, float4* __restrict__ sposf,
-                                                       int* __restrict__ sflag, int* __restrict__ iperm) {
+                                                       int* __restrict__ sflag, int* __restrict__ iperm, int fmode,
+                                                       QuadField q, const double* __restrict__ fv) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
   const double a = x[i], b = y[i], c = z[i];
-  spos[s] = make_double4(a, b, c, 0.0);
+  const double f = fmode == FIELD_QUADRATIC ? quad_eval(q, a, b, c) : fmode == FIELD_VALUES ? fv[i] : 0.0;
+  spos[s] = make_double4(a, b, c, f);
   const int cl = skey[s];
   sposf[s] = make_float4((float)(a - ox), (float)(b - oy), (float)(c - oz), __int_as_float(cl));
   const int u = cl % gx, t = cl / gx, v = t % gy, w = t / gy;
@@ -527,7 +530,8 @@ void launch_fill_int(int* p, int64_t n, int v, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(blocks(n, 256)), dim3(256), 0, st, p, n, v);
 }
 
-void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t st) {
+void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, const QuadField& q, const double* fv,
+                      hipStream_t st) {
   const int64_t ncells = (int64_t)B.gx * B.gy * B.gz;
   launch_fill_int(B.ccount, ncells, 0, st);
   if (n > 0)
@@ -540,7 +544,7 @@ void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t 
                        B.skey);
     hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n, 256)), dim3(256), 0, st, B.iperm, B.skey, B.cstart, n, B.perm);
     hipLaunchKernelGGL(k_gather_sorted, dim3(blocks(n, 256)), dim3(256), 0, st, B.perm, B.skey, B.gx, B.gy, B.gz, B.x,
-                       B.y, B.z, n, B.ox, B.oy, B.oz, B.spos, B.sposf, B.sflag, B.iperm);
+                       B.y, B.z, n, B.ox, B.oy, B.oz, B.spos, B.sposf, B.sflag, B.iperm, fmode, q, fv);
   }
 }
 

@@ -20,6 +20,14 @@ struct QuadField {
   double c[10];  // c0, cx, cy, cz, cxx, cyy, czz, cxy, cxz, cyz
   double lap_exact;
 };
+// The quadratic at (a, b, c), in one fixed order of operations (the FP64
+// oracle evaluates the same expression).
+__device__ __forceinline__ double quad_eval(const QuadField& q, double a, double b, double c) {
+  return q.c[0] + q.c[1] * a + q.c[2] * b + q.c[3] * c + q.c[4] * a * a + q.c[5] * b * b + q.c[6] * c * c +
+         q.c[7] * a * b + q.c[8] * a * c + q.c[9] * b * c;
+}
+// Field the cell build writes into the sorted records' fourth word.
+enum : int { FIELD_NONE = 0, FIELD_QUADRATIC = 1, FIELD_VALUES = 2 };
 
 // scan.hip
 size_t scan_scratch_elems(int64_t n);
@@ -62,7 +70,10 @@ void launch_cell_coords(const double* x, const double* y, const double* z, int n
 void launch_fill_int(int* p, int64_t n, int v, hipStream_t st);
 // Stable sort by cell and the sorted copies: perm, skey, cstart, spos,
 // sposf, sflag, iperm.
-void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, hipStream_t st);
+// fmode (FIELD_*): the field written into spos.w, from q or from values fv
+// in caller order.
+void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, const QuadField& q, const double* fv,
+                      hipStream_t st);
 // Neighbor lists and classification work on sorted-order types only; the
 // original-order arrays are produced on demand by the unsort launchers.
 // re2: squared list radius; re2c: squared radius of the neighbors that

@@ -343,9 +343,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
 __global__ __launch_bounds__(256) void k_field_quadratic(double4* __restrict__ spos, int n, QuadField q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double a = spos[i].x, b = spos[i].y, c = spos[i].z;
-  spos[i].w = q.c[0] + q.c[1] * a + q.c[2] * b + q.c[3] * c + q.c[4] * a * a + q.c[5] * b * b + q.c[6] * c * c +
-              q.c[7] * a * b + q.c[8] * a * c + q.c[9] * b * c;
+  spos[i].w = quad_eval(q, spos[i].x, spos[i].y, spos[i].z);
 }
 
 // Errors of the sampled particles i = k*stride (original index) against the
