@@ -19,7 +19,7 @@
 //    accumulation, FP32 Cholesky. A row whose FP32 pivots lose more than
 //    three digits (pivot <= 1e-3 * diagonal), or that would need the
 //    reference's regularisation, is appended to a work list and redone by
-//    k_wls_exact_list in FP64 with the reference's ladder.
+//    k_wls_redo in FP64 with the reference's ladder.
 #include "kernels.hpp"
 #include "wls_core.hpp"
 
@@ -134,22 +134,149 @@ __global__ __launch_bounds__(128) void k_wls_exact(const double4* __restrict__ s
   count_status(slot, bpart);
 }
 
-// Redo in FP64 the particles the FP32 kernel could not trust.
-__global__ __launch_bounds__(128) void k_wls_exact_list(const double4* __restrict__ spos,
-                                                        const int* __restrict__ nbr, const int* __restrict__ cnt,
-                                                        int maxn, const int* __restrict__ list,
-                                                        const int* __restrict__ nlist,
-                                                        double re, double l0, double reg, double* __restrict__ grad,
-                                                        double* __restrict__ lap, uint8_t* __restrict__ status,
-                                                        unsigned long long* __restrict__ counts) {
+
+// The FP64 redo of the rows the FP32 kernel could not trust, eight lanes
+// per row, with the same results as wls_exact_one bit for bit:
+//  - the per-neighbor values (weight, the nine scaled monomials, the field
+//    difference and r) are computed by the row's lanes in parallel, a chunk
+//    of RX_CHUNK neighbors at a time, into LDS, with the same expressions;
+//  - each of the 55 sums (45 moments, 9 right-hand sides, the weight sum) is
+//    owned by one lane, which adds the chunk's terms in neighbor order, so
+//    every sum sees the same additions in the same order;
+//  - the moments do not depend on the regularisation step, so the five
+//    steps of the ladder are factorised side by side by lanes 0..4 and the
+//    first that succeeds is taken.
+// A row costs about a quarter of the sequential latency; the list is short
+// (rows whose FP32 pivots lost three digits), so that latency is the time.
+constexpr int RX_LANES = 8, RX_ROWS = WAVE / RX_LANES, RX_CHUNK = 32, RX_REC = 12, RX_OWN = 7;
+
+__global__ __launch_bounds__(WAVE) void k_wls_redo(const double4* __restrict__ spos, const int* __restrict__ nbr,
+                                                   const int* __restrict__ cnt, int maxn, const int* __restrict__ list,
+                                                   const int* __restrict__ nlist, double re, double l0, double reg,
+                                                   double* __restrict__ grad, double* __restrict__ lap,
+                                                   uint8_t* __restrict__ status,
+                                                   unsigned long long* __restrict__ counts) {
+  __shared__ double rec[RX_ROWS][RX_CHUNK][RX_REC];  // w, q[0..8], df, r
+  __shared__ int okn[RX_ROWS][RX_CHUNK];
+  __shared__ double acc[RX_ROWS][56];
+  const int lane = threadIdx.x, g = lane % RX_LANES, sub = lane / RX_LANES;
+  const double h2 = 2 * l0;
+  // Sums owned by this lane: j = g + 8 t (t < 7, j < 55); for a moment,
+  // its monomial pair (a, c) with j = tri(a, c).
+  int own_a[RX_OWN], own_c[RX_OWN];
+#pragma unroll
+  for (int t = 0; t < RX_OWN; ++t) {
+    const int j = g + RX_LANES * t;
+    int a = 0;
+    while (a < 8 && tri(a + 1, 0) <= j) ++a;
+    own_a[t] = j < 45 ? a : j < 54 ? j - 45 : -1;
+    own_c[t] = j < 45 ? j - tri(a, 0) : -1;
+  }
   const int nl = *nlist;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x) {
-    const int i = list[k];
+  for (int k0 = blockIdx.x * RX_ROWS; k0 < nl; k0 += gridDim.x * RX_ROWS) {
+    const int k = k0 + sub;
+    const bool live = k < nl;
+    const int i = live ? list[k] : 0;
+    const int kn = live ? cnt[i] : 0;
+    const double4 pi = spos[i];
+    const int* col = nbr + (size_t)i * maxn;
+    double own[RX_OWN];
+#pragma unroll
+    for (int t = 0; t < RX_OWN; ++t) own[t] = 0.0;
+    const int knmax = wave_max(kn);
+    for (int c0 = 0; c0 < knmax; c0 += RX_CHUNK) {
+      for (int u = g; u < RX_CHUNK; u += RX_LANES) {
+        const int kk = c0 + u;
+        int ok = 0;
+        if (kk < kn) {
+          const double4 pj = spos[col[kk]];
+          const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+          const double r = sqrt(dx * dx + dy * dy + dz * dz);
+          if (!(r <= 1e-12 || r >= re)) {
+            const double tq = re / r - 1.0;
+            const double w = tq * tq;
+            if (w != 0.0) {
+              const double inv = 1.0 / r, h = h2 * r, gg = l0 * r;
+              double* o = rec[sub][u];
+              o[0] = w;
+              o[1] = dx * inv;
+              o[2] = dy * inv;
+              o[3] = dz * inv;
+              o[4] = dx * dx / h;
+              o[5] = dy * dy / h;
+              o[6] = dz * dz / h;
+              o[7] = dx * dy / gg;
+              o[8] = dx * dz / gg;
+              o[9] = dy * dz / gg;
+              o[10] = pj.w - pi.w;
+              o[11] = r;
+              ok = 1;
+            }
+          }
+        }
+        okn[sub][u] = ok;
+      }
+      wave_sync();
+      const int cend = kn - c0 < RX_CHUNK ? kn - c0 : RX_CHUNK;
+      for (int u = 0; u < cend; ++u) {
+        if (!okn[sub][u]) continue;
+        const double* o = rec[sub][u];
+        const double w = o[0];
+#pragma unroll
+        for (int t = 0; t < RX_OWN; ++t) {
+          const int a = own_a[t], c = own_c[t];
+          if (c >= 0) own[t] += (w * o[1 + a]) * o[1 + c];
+          else if (a >= 0) own[t] += (w * o[1 + a]) * o[10] / o[11];
+          else if (g + RX_LANES * t == 54) own[t] += w;
+        }
+      }
+      wave_sync();
+    }
+#pragma unroll
+    for (int t = 0; t < RX_OWN; ++t)
+      if (g + RX_LANES * t < 55) acc[sub][g + RX_LANES * t] = own[t];
+    wave_sync();
+    // Lanes 0..4 of the row try the five steps of the ladder.
+    uint8_t st = WLS_FAILED;
     double g3[3] = {0.0, 0.0, 0.0}, L = 0.0;
-    const uint8_t st = wls_exact_one(i, spos, nbr, cnt, maxn, re, l0, reg, g3, &L);
-    store_result(i, st, g3, L, grad, lap, status);
-    atomicAdd(&counts[st], 1ull);
-    atomicAdd(&counts[WLS_REDO_SLOT], 1ull);
+    const double ws = acc[sub][54];
+    bool solved = false;
+    if (ws <= 0) {
+      st = WLS_NO_WEIGHT;
+    } else if (g < 5) {
+      const double add = g == 0 ? 0.0 : g == 1 ? reg * 1.0 : g == 2 ? reg * 10.0 : g == 3 ? reg * 100.0 : reg * 1000.0;
+      if (g == 0 || add != 0.0) {
+        double M[45], B[9];
+#pragma unroll
+        for (int j = 0; j < 45; ++j) M[j] = acc[sub][j] / ws;
+#pragma unroll
+        for (int a = 0; a < 9; ++a) B[a] = acc[sub][45 + a] / ws;
+        if (add != 0.0) {
+#pragma unroll
+          for (int a = 0; a < 9; ++a) M[tri(a, a)] += add;
+        }
+        const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (chol9<double>(M, 1e-14, 0.0, zero9)) {
+          double sol[9];
+          chol9_solve<double>(M, B, sol);
+          g3[0] = sol[0];
+          g3[1] = sol[1];
+          g3[2] = sol[2];
+          L = (sol[3] + sol[4] + sol[5]) / l0;
+          st = g == 0 ? WLS_OK : WLS_REGULARIZED;
+          solved = true;
+        }
+      }
+    }
+    // The row's result: the first step that succeeded, else its lane 0.
+    const unsigned mk = (unsigned)(__builtin_amdgcn_ballot_w64(live && solved) >> (sub * RX_LANES)) & 0xFFu;
+    const int winner = mk ? __builtin_ctz(mk) : 0;
+    if (live && g == winner) {
+      store_result(i, st, g3, L, grad, lap, status);
+      atomicAdd(&counts[st], 1ull);
+      atomicAdd(&counts[WLS_REDO_SLOT], 1ull);
+    }
+    wave_sync();  // acc and rec are reused by the next rows
   }
 }
 
@@ -438,9 +565,9 @@ void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, 
   launch_sum_counts(bpart, (int)gm, cs, counts, st);
   // The redo list length is only known on the device: a bounded grid
   // strides over it (typical lists are empty or a few thousand entries).
-  const unsigned g = blocks(n, 128) < 512u ? blocks(n, 128) : 512u;
-  hipLaunchKernelGGL(k_wls_exact_list, dim3(g), dim3(128), 0, st, spos, nbr, cnt, maxn, redo, nredo, re, l0, reg, grad,
-                     lap, status, counts);
+  const unsigned gr = blocks(n, RX_ROWS) < 1024u ? blocks(n, RX_ROWS) : 1024u;
+  hipLaunchKernelGGL(k_wls_redo, dim3(gr), dim3(WAVE), 0, st, spos, nbr, cnt, maxn, redo, nredo, re, l0, reg, grad, lap,
+                     status, counts);
 }
 
 void launch_wls_sample_errors(const double4* spos, const uint8_t* type, const double* grad, const double* lap,

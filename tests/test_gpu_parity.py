@@ -194,6 +194,31 @@ def test_wls_other_field_and_mixed(pipe):
     assert np.sqrt(np.mean((l32[okm] - 4.0) ** 2)) < 5e-2
 
 
+def test_mixed_wls_redo_rows_equal_fp64(pipe):
+    """Rows the mixed WLS redoes in FP64 (eight lanes per row, the five
+    regularisation steps side by side) carry exactly the FP64 kernel's
+    results: every row that needed the ladder or failed in FP64, and at
+    least as many rows as were redone, match bit for bit."""
+    l0 = 0.004
+    re = 2.1 * l0
+    xyz, t = L.generate_case("rain_plate", 30000, 5, l0)
+    say("mixed redo rows", len(xyz))
+    pipe.upload(xyz, t)
+    pipe.neighbors_classify(re, 128, l0)
+    pipe.set_field(L.C1_FIELD)
+    pipe.wls(re, l0, L.WLS_FP64)
+    g64, l64, s64 = pipe.wls_results()
+    r32 = pipe.wls(re, l0, L.WLS_MIXED)
+    g32, l32, s32 = pipe.wls_results()
+    say("mixed wls", r32)
+    same = (s32 == s64) & np.all(g32 == g64, axis=1) & (l32 == l64)
+    assert r32["fp64_redo"] > 0
+    assert int(same.sum()) >= r32["fp64_redo"]
+    ladder = (s64 == L.WLS_REGULARIZED) | (s64 == L.WLS_FAILED) | (s64 == L.WLS_NO_WEIGHT)
+    say("ladder rows", int(ladder.sum()), "redone", r32["fp64_redo"])
+    assert np.all(same[ladder])
+
+
 def test_solvers_reach_tolerance_and_agree(pipe):
     xyz, t = L.generate_case("bulk_lattice", 50000, 7, 0.01)
     say("solvers C1", len(xyz))
