@@ -300,8 +300,8 @@ constexpr int WLS_NACC = 55;   // 45 moments + 9 RHS + weight sum
 constexpr int WLS_STRIDE = 57; // LDS floats per particle (odd: no bank conflicts)
 constexpr int WLS_HALF = 32;   // particles per phase-2 batch
 
-__device__ __forceinline__ void wls_accumulate(const double4 pj, const double4 pi, double inv_re, float inv_h2,
-                                               float inv_g, float rmin2, float (&a)[WLS_NACC]) {
+__device__ __forceinline__ void wls_accumulate(const double4 pj, const double4 pi, double inv_re, float rmin2,
+                                               float (&a)[WLS_NACC]) {
   const float dx = (float)((pj.x - pi.x) * inv_re), dy = (float)((pj.y - pi.y) * inv_re),
               dz = (float)((pj.z - pi.z) * inv_re);
   const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
@@ -309,17 +309,21 @@ __device__ __forceinline__ void wls_accumulate(const double4 pj, const double4 p
   const float inv = __builtin_amdgcn_rsqf(r2);
   const float tq = inv - 1.0f;
   const float w = tq * tq;
+  // Monomials over r without the reference's constant factors 1/(2 l0) and
+  // 1/l0 on the quadratic ones (the diagonal scaling S): the moments are
+  // then S^-1 M S^-1, whose solution is S times the reference's, undone on
+  // the solution; the pivot test is relative to each diagonal entry, which
+  // that scaling does not change. One product per quadratic monomial.
   float q[9];
   q[0] = dx * inv;
   q[1] = dy * inv;
   q[2] = dz * inv;
-  const float ih = inv_h2 * inv, ig = inv_g * inv;
-  q[3] = dx * dx * ih;
-  q[4] = dy * dy * ih;
-  q[5] = dz * dz * ih;
-  q[6] = dx * dy * ig;
-  q[7] = dx * dz * ig;
-  q[8] = dy * dz * ig;
+  q[3] = q[0] * dx;
+  q[4] = q[1] * dy;
+  q[5] = q[2] * dz;
+  q[6] = q[0] * dy;
+  q[7] = q[0] * dz;
+  q[8] = q[1] * dz;
   const float dfr = (float)((pj.w - pi.w) * inv_re) * inv;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -390,7 +394,6 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
   const int lane = threadIdx.x;
   const double inv_re = 1.0 / re;
   const float l0s = (float)(l0 * inv_re);
-  const float inv_h2 = 1.0f / (2.0f * l0s), inv_g = 1.0f / l0s;
   const float rmin = (float)(1e-12 * inv_re), rmin2 = rmin * rmin;
   const int s0 = (int)xcd_block(blockIdx.x, gridDim.x) * WAVE;
   int n_ok = 0, n_skip = 0;
@@ -421,7 +424,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
             for (int u = 0; u < 4; ++u) pj[u] = j[u] >= 0 ? spos[j[u]] : pi;
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-              if (j[u] >= 0) wls_accumulate(pj[u], pi, inv_re, inv_h2, inv_g, rmin2, a);
+              if (j[u] >= 0) wls_accumulate(pj[u], pi, inv_re, rmin2, a);
           }
         }
       }
@@ -446,8 +449,11 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
         } else {
           float sol[9];
           if (wls_solve_f32(sums + lane * WLS_STRIDE, sol)) {
+            // The reference's quadratic coefficients are the ones of the
+            // unscaled monomials times 2 l0 / re.
             const double g3[3] = {(double)sol[0], (double)sol[1], (double)sol[2]};
-            store_result(s, WLS_OK, g3, (double)(sol[3] + sol[4] + sol[5]) / l0, grad, lap, status);
+            const float quad = (sol[3] + sol[4] + sol[5]) * (2.0f * l0s);
+            store_result(s, WLS_OK, g3, (double)quad / l0, grad, lap, status);
             ++n_ok;
           } else {
             const int at = atomicAdd(nredo, 1);
