@@ -66,9 +66,12 @@ struct GraphCache {
 
 }  // namespace
 
-// Pinned readback area: 16 64-bit counters, then up to 512 histogram bins
-// (max_neighbors <= 256).
-constexpr size_t HOST_RB_BYTES = 16 * sizeof(unsigned long long) + 512 * sizeof(int);
+// Pinned readback area: 16 64-bit counters, then (as ints) up to 512
+// histogram bins (max_neighbors <= 256) and the 8-int cell box.
+constexpr int HOST_RB_HIST = 32, HOST_RB_BOX = HOST_RB_HIST + 512;  // int offsets
+constexpr size_t HOST_RB_BYTES = (HOST_RB_BOX + 8) * sizeof(int);
+// Cells of margin around the box when a build's grid is kept for the next.
+constexpr int CNL_GRID_MARGIN = 2;
 
 struct lsmps_ctx {
   int device = 0;
@@ -107,6 +110,13 @@ struct lsmps_ctx {
   // field and WLS (results in sorted order; original order on download)
   bool have_field = false, field_quadratic = false, have_wls = false;
   bool spos_field = false;  // spos[].w holds the current field (written by the cell build or the WLS stage)
+  // Grid for the next build (first cell and extent per axis, cell size
+  // spec_rl): the last box plus CNL_GRID_MARGIN cells. A build on it needs
+  // no round trip for the box; it is redone on the exact box when a
+  // particle falls outside.
+  int spec[6] = {0, 0, 0, 0, 0, 0};
+  bool spec_ok = false;
+  double spec_rl = 0.0;
   QuadField qf{};
   DBuf<double> f, grad_s, lap_s, grad, lap, ge, le;
   DBuf<uint8_t> wstatus_s, wstatus, st_s, ty_s;
@@ -447,6 +457,7 @@ int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   ctx->have_cnl = ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_x0 = false;
   ctx->have_field = false;
   ctx->spos_field = false;
+  ctx->spec_ok = false;  // a new particle set gets an exact grid first
   ctx->reset_graphs();
   return LSMPS_OK;
   GUARD_END
@@ -556,15 +567,26 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   HIPCHK(hipEventRecord(ctx->e0, st));
   launch_cell_coords(ctx->x.p, ctx->y.p, ctx->z.p, n, rl, ctx->cx.p, ctx->cy.p, ctx->cz.p, ctx->box.p, st);
   if (int rc = check_launch(ctx, "cell_coords")) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->host_small, ctx->box.p, 6 * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
   int bx[6];
-  std::memcpy(bx, ctx->host_small, sizeof(bx));
   int64_t g[3] = {1, 1, 1};
-  if (n > 0)
-    for (int k = 0; k < 3; ++k) g[k] = (int64_t)bx[3 + k] - bx[k] + 1;
-  else
-    bx[0] = bx[1] = bx[2] = 0;
+  const bool spec = n > 0 && ctx->spec_ok && ctx->spec_rl == rl;
+  if (spec) {
+    // The grid of the last build (its box plus a margin), checked on the
+    // device by k_cell_key.
+    for (int k = 0; k < 3; ++k) {
+      bx[k] = ctx->spec[k];
+      g[k] = ctx->spec[3 + k];
+      bx[3 + k] = bx[k] + (int)g[k] - 1;
+    }
+  } else {
+    HIPCHK(hipMemcpyAsync(ctx->host_small, ctx->box.p, 6 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::memcpy(bx, ctx->host_small, sizeof(bx));
+    if (n > 0)
+      for (int k = 0; k < 3; ++k) g[k] = (int64_t)bx[3 + k] - bx[k] + 1;
+    else
+      bx[0] = bx[1] = bx[2] = 0;
+  }
   const int64_t ncells = g[0] * g[1] * g[2];
   // Dense grid: bounded so a sparse outlier cannot blow up memory.
   if (ncells > ((int64_t)1 << 30) || ncells > 64 * (int64_t)n + ((int64_t)1 << 24))
@@ -580,7 +602,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   HIPCHK(ctx->scan_scr.ensure(std::max(scan_scratch_elems(ncells), scan_scratch_elems((int64_t)n + 1))));
   const CnlBufs B = cnl_bufs(ctx);
   const int fmode = !ctx->have_field ? FIELD_NONE : ctx->field_quadratic ? FIELD_QUADRATIC : FIELD_VALUES;
-  launch_cell_sort(B, n, bx, fmode, ctx->qf, fmode == FIELD_VALUES ? ctx->f.p : nullptr, st);
+  launch_cell_sort(B, n, bx, fmode, ctx->qf, fmode == FIELD_VALUES ? ctx->f.p : nullptr, ctx->box.p + 6, st);
   ctx->spos_field = fmode != FIELD_NONE;
   launch_gather_types(ctx->perm.p, ctx->type.p, n, ctx->type_s.p, st);
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
@@ -592,10 +614,33 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   HIPCHK(hipEventRecord(ctx->e1, st));
   std::vector<int> hist((size_t)maxn + 1);
   unsigned long long c[16];
-  int* hist_rb = reinterpret_cast<int*>(ctx->host_rb + 16);
+  int* hist_rb = reinterpret_cast<int*>(ctx->host_rb) + HOST_RB_HIST;
+  int* box_rb = reinterpret_cast<int*>(ctx->host_rb) + HOST_RB_BOX;
   HIPCHK(hipMemcpyAsync(hist_rb, ctx->hist.p, hist.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(box_rb, ctx->box.p, 7 * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(read_counters(ctx, c, 16));
   std::memcpy(hist.data(), hist_rb, hist.size() * sizeof(int));
+  int box_now[7];
+  std::memcpy(box_now, box_rb, sizeof(box_now));
+  if (n > 0) {
+    // Next build's grid: this box plus the margin (kept within the int
+    // range of the cell coordinates).
+    for (int k = 0; k < 3; ++k) {
+      const int64_t lo = (int64_t)box_now[k] - CNL_GRID_MARGIN, hi = (int64_t)box_now[3 + k] + CNL_GRID_MARGIN;
+      ctx->spec[k] = (int)std::max<int64_t>(lo, INT_MIN / 2);
+      ctx->spec[3 + k] = (int)std::min<int64_t>(hi - ctx->spec[k] + 1, INT_MAX / 2);
+    }
+    const int64_t sc = (int64_t)ctx->spec[3] * ctx->spec[4] * ctx->spec[5];
+    ctx->spec_ok = sc <= ((int64_t)1 << 30) && sc <= 64 * (int64_t)n + ((int64_t)1 << 24);
+    ctx->spec_rl = rl;
+  } else {
+    ctx->spec_ok = false;
+  }
+  if (spec && box_now[6]) {
+    // A particle left the grid of the last build: redo on the exact box.
+    ctx->spec_ok = false;
+    return build_cell_list(ctx, re, skin, maxn, classify, l0, out, tout);
+  }
   lsmps_cnl_stats cs;
   fill_cnl_stats(ctx, hist, c[0], &cs);
   if (cs.pairs > (uint64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "neighbor pairs beyond int32 indexing");

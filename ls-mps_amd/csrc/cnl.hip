@@ -32,9 +32,12 @@
 
 namespace lsmps_hip {
 
+// box[0..2] min and box[3..5] max cell coordinates; box[6] is set when a
+// particle falls outside the grid a build assumed (k_cell_key).
 __global__ void k_init_box(int* box) {
   if (threadIdx.x < 3) box[threadIdx.x] = INT_MAX;
   else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
+  else if (threadIdx.x == 6) box[6] = 0;
 }
 
 // Cell coordinates of every particle and their bounding box: grid-stride,
@@ -94,12 +97,24 @@ __global__ void k_fill_int(int* __restrict__ p, int64_t n, int v) {
 //                perm holds every cell's particles in ascending index order.
 __global__ __launch_bounds__(256) void k_cell_key(const int* __restrict__ cx, const int* __restrict__ cy,
                                                   const int* __restrict__ cz, int n, int bx, int by, int bz,
-                                                  int gx, int gy, int* __restrict__ cell, int* __restrict__ slot,
-                                                  int* __restrict__ count) {
+                                                  int gx, int gy, int gz, int* __restrict__ cell,
+                                                  int* __restrict__ slot, int* __restrict__ count,
+                                                  int* __restrict__ outside) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (WAVE - 1);
   const bool valid = i < n;
-  const int c = valid ? ((cz[i] - bz) * gy + (cy[i] - by)) * gx + (cx[i] - bx) : -1;
+  int c = -1;
+  if (valid) {
+    const int u = cx[i] - bx, v = cy[i] - by, w = cz[i] - bz;
+    // A grid taken from an earlier build may not hold every particle: flag
+    // it (the build is then redone on the exact box) and use cell 0.
+    if ((unsigned)u < (unsigned)gx && (unsigned)v < (unsigned)gy && (unsigned)w < (unsigned)gz) {
+      c = (w * gy + v) * gx + u;
+    } else {
+      c = 0;
+      atomicOr(outside, 1);
+    }
+  }
   const int prev = __shfl_up(c, 1);
   const bool head = valid && (lane == 0 || c != prev);
   const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
@@ -531,12 +546,12 @@ void launch_fill_int(int* p, int64_t n, int v, hipStream_t st) {
 }
 
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, const QuadField& q, const double* fv,
-                      hipStream_t st) {
+                      int* outside, hipStream_t st) {
   const int64_t ncells = (int64_t)B.gx * B.gy * B.gz;
   launch_fill_int(B.ccount, ncells, 0, st);
   if (n > 0)
     hipLaunchKernelGGL(k_cell_key, dim3(blocks(n, 256)), dim3(256), 0, st, B.cx, B.cy, B.cz, n, box_host[0],
-                       box_host[1], box_host[2], B.gx, B.gy, B.cell, B.slot, B.ccount);
+                       box_host[1], box_host[2], B.gx, B.gy, B.gz, B.cell, B.slot, B.ccount, outside);
   launch_exclusive_scan(B.ccount, ncells, B.cstart, B.scan_scratch, st);
   if (n > 0) {
     // iperm holds the arrival order until k_gather_sorted writes it.

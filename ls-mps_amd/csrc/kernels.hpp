@@ -71,9 +71,10 @@ void launch_fill_int(int* p, int64_t n, int v, hipStream_t st);
 // Stable sort by cell and the sorted copies: perm, skey, cstart, spos,
 // sposf, sflag, iperm.
 // fmode (FIELD_*): the field written into spos.w, from q or from values fv
-// in caller order.
+// in caller order. box_host: the grid's first cell per axis; *outside is set
+// when a particle lies outside the grid (B.gx, B.gy, B.gz).
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, const QuadField& q, const double* fv,
-                      hipStream_t st);
+                      int* outside, hipStream_t st);
 // Neighbor lists and classification work on sorted-order types only; the
 // original-order arrays are produced on demand by the unsort launchers.
 // re2: squared list radius; re2c: squared radius of the neighbors that

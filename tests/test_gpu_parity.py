@@ -219,6 +219,47 @@ def test_mixed_wls_redo_rows_equal_fp64(pipe):
     assert np.all(same[ladder])
 
 
+def test_kept_grid_reused_and_outgrown(pipe):
+    """A build reuses the last build's grid (box plus a margin) without a
+    round trip for the box; lists and types match a fresh context's whether
+    the particles stay inside that grid or some leave it (the build is then
+    redone on the exact box)."""
+    l0 = 0.01
+    re = 2.1 * l0
+    xyz, t = L.generate_case("wall_film", 6000, 3, l0)
+    say("kept grid", len(xyz))
+
+    def fresh(points, types):
+        q = L.Pipeline(0)
+        q.upload(points, types)
+        q.neighbors_classify(re, 96, l0)
+        out = q.neighbor_lists(), q.types()
+        q.close()
+        return out
+
+    pipe.upload(xyz, t)
+    pipe.neighbors_classify(re, 96, l0)
+    (r0, c0), t0 = fresh(xyz, t)
+    # Same particles again: the kept grid holds them.
+    pipe.neighbors_classify(re, 96, l0)
+    r1, c1 = pipe.neighbor_lists()
+    assert np.array_equal(r0, r1) and np.array_equal(c0, c1)
+    # Particles well outside the last box: the kept grid is outgrown.
+    lo, hi = xyz.min(0), xyz.max(0)
+    extra = lo + (hi - lo) * np.array([[1.0, 0.5, 0.5]]) + np.array([[12 * re, 0.0, 0.0]])
+    extra = extra + l0 * np.arange(3)[:, None] * np.array([[1.0, 0.0, 0.0]])
+    te = np.zeros(len(extra), np.uint8)
+    pipe.append(extra, te)
+    types_now = pipe.types()
+    st = pipe.neighbors_classify(re, 96, l0)
+    say("after append", st[0]["pairs"])
+    both = np.concatenate([xyz, extra])
+    (r2, c2), t2 = fresh(both, types_now)
+    r3, c3 = pipe.neighbor_lists()
+    assert np.array_equal(r2, r3) and np.array_equal(c2, c3)
+    assert np.array_equal(t2, pipe.types())
+
+
 def test_solvers_reach_tolerance_and_agree(pipe):
     xyz, t = L.generate_case("bulk_lattice", 50000, 7, 0.01)
     say("solvers C1", len(xyz))
