@@ -47,8 +47,8 @@ __global__ __launch_bounds__(256) void k_cell_coords(const double* __restrict__ 
                                                      int* __restrict__ cx, int* __restrict__ cy,
                                                      int* __restrict__ cz, int* __restrict__ box) {
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int a = (int)floor(x[i] / re), b = (int)floor(y[i] / re), c = (int)floor(z[i] / re);
+  auto one = [&](double xv, double yv, double zv, int i) {
+    const int a = (int)floor(xv / re), b = (int)floor(yv / re), c = (int)floor(zv / re);
     cx[i] = a;
     cy[i] = b;
     cz[i] = c;
@@ -58,7 +58,22 @@ __global__ __launch_bounds__(256) void k_cell_coords(const double* __restrict__ 
     hi[1] = max(hi[1], b);
     lo[2] = min(lo[2], c);
     hi[2] = max(hi[2], c);
+  };
+  // Four particles per thread and step, their twelve loads issued together.
+  const int stride = gridDim.x * blockDim.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    double xv[4], yv[4], zv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xv[u] = x[i + u * stride];
+      yv[u] = y[i + u * stride];
+      zv[u] = z[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) one(xv[u], yv[u], zv[u], i + u * stride);
   }
+  for (; i < n; i += stride) one(x[i], y[i], z[i], i);
   __shared__ int red[6][256 / WAVE];
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
 #pragma unroll
