@@ -236,14 +236,15 @@ __device__ __forceinline__ uint8_t classify_rule(uint8_t t0, int kept, double z,
 }
 
 // Eight lanes per particle, eight consecutive sorted particles (nearly
-// always from the same few cells) per wavefront. The lanes step through
-// each candidate run thirty-two at a time (four loads in flight per lane),
-// so position loads are coalesced; a ballot gives every hit its rank among
-// the group's hits, which keeps the reference's visiting order (and the
-// cut at max_neighbors). Hits are staged in LDS; afterwards the group
+// always from the same few cells) per wavefront. For each of the nine
+// candidate runs the wavefront stages the union of its particles' runs in
+// LDS with one coalesced load per lane, and each group steps through its
+// own run eight candidates at a time; a ballot gives every hit its rank
+// among the group's hits, which keeps the reference's visiting order (and
+// the cut at max_neighbors). Hits are staged in LDS; afterwards the group
 // writes the row out with coalesced stores and, when CLASSIFY, takes the
 // direction sectors of the kept neighbors only (~30) rather than of every
-// candidate (~250).
+// candidate (~200).
 //
 // Candidate test: the squared distance in FP32 from 16-byte offset records
 // decides every pair outside [lo, hi], a band around re^2 wider than the
