@@ -204,7 +204,9 @@ int lsmps_build_neighbors_skin(lsmps_ctx* ctx, double re, double skin, int32_t m
 /* Velocities of all particles, 3 per particle, caller order. */
 int lsmps_set_velocities(lsmps_ctx* ctx, const double* uvw, uint64_t n);
 /* Appends particles (xyz 3 per particle, types, velocities or NULL) after
- * the existing ones; they join the stages at the next neighbor build. */
+ * the existing ones; they join the stages at the next neighbor build. A
+ * field given as values (lsmps_set_field_values) has none for them, so it
+ * must be set again; a quadratic field carries over. */
 int lsmps_append_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* type, const double* uvw,
                            uint64_t count);
 /* x += u dt for every particle (and the sorted copies the neighbor lists
