@@ -520,6 +520,8 @@ CnlBufs cnl_bufs(lsmps_ctx* ctx) {
   B.perm = ctx->perm.p;
   B.iperm = ctx->iperm.p;
   B.sflag = ctx->sflag.p;
+  B.type = ctx->type.p;
+  B.type_s = ctx->type_s.p;
   B.spos = ctx->spos.p;
   B.sposf = ctx->sposf.p;
   B.ox = ctx->ox;
@@ -554,7 +556,13 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   hipStream_t st = ctx->st;
   if ((uint64_t)n * (uint64_t)std::max(1, maxn) > ((uint64_t)1 << 40))
     return fail(ctx, LSMPS_E_LIMIT, "neighbor table too large");
-  sync_types(ctx);  // with the previous order, before it is replaced
+  // Types classified by the previous build live in its sorted order. With
+  // the same particle count no buffer below is reallocated, and the cell
+  // coordinate kernel brings them back to caller order; otherwise they are
+  // brought back now, before the buffers may move.
+  int n_unsort = 0;
+  if (ctx->types_stale && (uint64_t)n == ctx->n_cnl) n_unsort = n;
+  else sync_types(ctx);
   DBuf<int>* per_particle[] = {&ctx->cx,   &ctx->cy,    &ctx->cz,    &ctx->cell, &ctx->slot, &ctx->skey,
                                &ctx->perm, &ctx->iperm, &ctx->sflag, &ctx->cnt,  &ctx->cnt_orig};
   for (auto* b : per_particle) HIPCHK(b->ensure(n));
@@ -565,7 +573,9 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   HIPCHK(ctx->hist.ensure((size_t)maxn + 1 + (size_t)(maxn + 6) * CNL_STATS_BLOCKS));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
-  launch_cell_coords(ctx->x.p, ctx->y.p, ctx->z.p, n, rl, ctx->cx.p, ctx->cy.p, ctx->cz.p, ctx->box.p, st);
+  launch_cell_coords(ctx->x.p, ctx->y.p, ctx->z.p, n, rl, ctx->cx.p, ctx->cy.p, ctx->cz.p, ctx->box.p, ctx->iperm.p,
+                     ctx->type_s.p, n_unsort, ctx->type.p, st);
+  ctx->types_stale = false;
   if (int rc = check_launch(ctx, "cell_coords")) return rc;
   int bx[6];
   int64_t g[3] = {1, 1, 1};
@@ -604,7 +614,6 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   const int fmode = !ctx->have_field ? FIELD_NONE : ctx->field_quadratic ? FIELD_QUADRATIC : FIELD_VALUES;
   launch_cell_sort(B, n, bx, fmode, ctx->qf, fmode == FIELD_VALUES ? ctx->f.p : nullptr, ctx->box.p + 6, st);
   ctx->spos_field = fmode != FIELD_NONE;
-  launch_gather_types(ctx->perm.p, ctx->type.p, n, ctx->type_s.p, st);
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
   zero_counters(ctx, 16);
   if (n == 0) launch_fill_int(ctx->hist.p, maxn + 1, 0, st);  // the pass's reduction writes every bin

@@ -41,11 +41,16 @@ __global__ void k_init_box(int* box) {
 }
 
 // Cell coordinates of every particle and their bounding box: grid-stride,
-// block reduction, one atomic per block and axis.
+// block reduction, one atomic per block and axis. For i < n_unsort it also
+// brings the previous build's sorted types back to caller order
+// (type[i] = type_s_old[iperm_old[i]]) before they are sorted again.
 __global__ __launch_bounds__(256) void k_cell_coords(const double* __restrict__ x, const double* __restrict__ y,
                                                      const double* __restrict__ z, int n, double re,
                                                      int* __restrict__ cx, int* __restrict__ cy,
-                                                     int* __restrict__ cz, int* __restrict__ box) {
+                                                     int* __restrict__ cz, int* __restrict__ box,
+                                                     const int* __restrict__ iperm_old,
+                                                     const uint8_t* __restrict__ type_s_old, int n_unsort,
+                                                     uint8_t* __restrict__ type) {
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
   auto one = [&](double xv, double yv, double zv, int i) {
     const int a = (int)floor(xv / re), b = (int)floor(yv / re), c = (int)floor(zv / re);
@@ -58,6 +63,7 @@ __global__ __launch_bounds__(256) void k_cell_coords(const double* __restrict__ 
     hi[1] = max(hi[1], b);
     lo[2] = min(lo[2], c);
     hi[2] = max(hi[2], c);
+    if (i < n_unsort) type[i] = type_s_old[iperm_old[i]];
   };
   // Four particles per thread and step, their twelve loads issued together.
   const int stride = gridDim.x * blockDim.x;
@@ -184,7 +190,8 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ p
                                                        int n, double ox, double oy, double oz,
                                                        double4* __restrict__ spos, float4* __restrict__ sposf,
                                                        int* __restrict__ sflag, int* __restrict__ iperm, int fmode,
-                                                       QuadField q, const double* __restrict__ fv) {
+                                                       QuadField q, const double* __restrict__ fv,
+                                                       const uint8_t* __restrict__ type, uint8_t* __restrict__ type_s) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
@@ -197,6 +204,7 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ p
   sflag[s] = (u > 0 ? CNL_XLO : 0) | (u < gx - 1 ? CNL_XHI : 0) | (v > 0 ? CNL_YLO : 0) | (v < gy - 1 ? CNL_YHI : 0) |
              (w > 0 ? CNL_ZLO : 0) | (w < gz - 1 ? CNL_ZHI : 0);
   iperm[i] = s;
+  type_s[s] = type[i];
 }
 
 // 14-sector bit of one offset: axis-dominant direction (ties x, then y,
@@ -538,22 +546,16 @@ __global__ __launch_bounds__(256) void k_ell_to_csr(const int* __restrict__ nbr,
   for (int q = 0; q < k; ++q) out[q] = perm[row[q]];
 }
 
-// Types in sorted order from the original-order array (after an upload or
-// after a caller changes types without re-running classification).
-__global__ __launch_bounds__(256) void k_gather_types(const int* __restrict__ perm, const uint8_t* __restrict__ type,
-                                                      int n, uint8_t* __restrict__ type_s) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n) type_s[s] = type[perm[s]];
-}
-
 static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 void launch_cell_coords(const double* x, const double* y, const double* z, int n, double re, int* cx, int* cy,
-                        int* cz, int* box, hipStream_t st) {
+                        int* cz, int* box, const int* iperm_old, const uint8_t* type_s_old, int n_unsort,
+                        uint8_t* type, hipStream_t st) {
   hipLaunchKernelGGL(k_init_box, dim3(1), dim3(64), 0, st, box);
   if (n > 0) {
     const unsigned g = blocks(n, 256) < 1024u ? blocks(n, 256) : 1024u;
-    hipLaunchKernelGGL(k_cell_coords, dim3(g), dim3(256), 0, st, x, y, z, n, re, cx, cy, cz, box);
+    hipLaunchKernelGGL(k_cell_coords, dim3(g), dim3(256), 0, st, x, y, z, n, re, cx, cy, cz, box, iperm_old, type_s_old,
+                       n_unsort, type);
   }
 }
 
@@ -575,7 +577,8 @@ void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, c
                        B.skey);
     hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n, 256)), dim3(256), 0, st, B.iperm, B.skey, B.cstart, n, B.perm);
     hipLaunchKernelGGL(k_gather_sorted, dim3(blocks(n, 256)), dim3(256), 0, st, B.perm, B.skey, B.gx, B.gy, B.gz, B.x,
-                       B.y, B.z, n, B.ox, B.oy, B.oz, B.spos, B.sposf, B.sflag, B.iperm, fmode, q, fv);
+                       B.y, B.z, n, B.ox, B.oy, B.oz, B.spos, B.sposf, B.sflag, B.iperm, fmode, q, fv, B.type,
+                       B.type_s);
   }
 }
 
@@ -664,10 +667,6 @@ void launch_ell_to_csr(const CnlBufs& B, int n, int maxn, const int* row_ptr, in
   if (n > 0)
     hipLaunchKernelGGL(k_ell_to_csr, dim3(blocks(n, 256)), dim3(256), 0, st, B.nbr, B.cnt, B.iperm, B.perm, n, maxn,
                        row_ptr, col);
-}
-
-void launch_gather_types(const int* perm, const uint8_t* type, int n, uint8_t* type_s, hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_gather_types, dim3(blocks(n, 256)), dim3(256), 0, st, perm, type, n, type_s);
 }
 
 }  // namespace lsmps_hip

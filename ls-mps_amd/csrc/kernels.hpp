@@ -60,13 +60,18 @@ struct CnlBufs {
   int64_t* scan_scratch = nullptr;
   int *perm = nullptr, *iperm = nullptr;                    // sorted slot <-> particle
   int* sflag = nullptr;                                     // sorted CNL_* grid-boundary flags of the cell
+  const uint8_t* type = nullptr;                            // caller order
+  uint8_t* type_s = nullptr;                                // sorted order
   double4* spos = nullptr;                                  // sorted {x, y, z, field}
   float4* sposf = nullptr;  // sorted {FP32 offsets from (ox, oy, oz), cell id as int bits}, n + CNL_PAD
   double ox = 0.0, oy = 0.0, oz = 0.0;                      // grid origin
   int *nbr = nullptr, *cnt = nullptr, *cnt_orig = nullptr;  // ELL table
 };
+// Also type[i] = type_s_old[iperm_old[i]] for i < n_unsort (the previous
+// build's sorted types back in caller order).
 void launch_cell_coords(const double* x, const double* y, const double* z, int n, double re, int* cx, int* cy,
-                        int* cz, int* box, hipStream_t st);
+                        int* cz, int* box, const int* iperm_old, const uint8_t* type_s_old, int n_unsort,
+                        uint8_t* type, hipStream_t st);
 void launch_fill_int(int* p, int64_t n, int v, hipStream_t st);
 // Stable sort by cell and the sorted copies: perm, skey, cstart, spos,
 // sposf, sflag, iperm.
@@ -87,7 +92,6 @@ void launch_classify(const CnlBufs& B, int n, int maxn, double zlim, uint8_t* ty
 void launch_unsort_u8(const int* iperm, const uint8_t* src, int n, uint8_t* dst, hipStream_t st);
 void launch_unsort_int(const int* iperm, const int* src, int n, int* dst, hipStream_t st);
 void launch_ell_to_csr(const CnlBufs& B, int n, int maxn, const int* row_ptr, int* col, hipStream_t st);
-void launch_gather_types(const int* perm, const uint8_t* type, int n, uint8_t* type_s, hipStream_t st);
 
 // wls.hip (sorted {x, y, z, field} records, types, neighbor rows and
 // outputs in sorted order)
