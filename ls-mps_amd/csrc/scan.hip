@@ -73,13 +73,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int* __restri
                                                              const int64_t* __restrict__ tile_off,
                                                              int* __restrict__ out) {
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  // A lane's eight items are 32 contiguous bytes: two 16-byte accesses when
+  // aligned and whole (the wavefront's accesses are then contiguous), else
+  // item by item.
+  const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0 &&
+                   base + SCAN_ITEMS <= n;
   int v[SCAN_ITEMS];
   int tsum = 0;
+  if (vec) {
+    const int4 a = reinterpret_cast<const int4*>(in + base)[0], b = reinterpret_cast<const int4*>(in + base)[1];
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+  } else {
 #pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    v[k] = (base + k < n) ? in[base + k] : 0;
-    tsum += v[k];
+    for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = (base + k < n) ? in[base + k] : 0;
   }
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) tsum += v[k];
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
   int inc = wave_incl_scan(tsum);
   __shared__ int wsum[SCAN_THREADS / WAVE];
@@ -88,10 +97,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int* __restri
   int wofs = 0;
   for (int w = 0; w < wid; ++w) wofs += wsum[w];
   int64_t run = tile_off[blockIdx.x] + wofs + inc - tsum;
+  int o[SCAN_ITEMS];
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    if (base + k < n) out[base + k] = (int)run;
+    o[k] = (int)run;
     run += v[k];
+  }
+  if (vec) {
+    reinterpret_cast<int4*>(out + base)[0] = make_int4(o[0], o[1], o[2], o[3]);
+    reinterpret_cast<int4*>(out + base)[1] = make_int4(o[4], o[5], o[6], o[7]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+      if (base + k < n) out[base + k] = o[k];
   }
 }
 
