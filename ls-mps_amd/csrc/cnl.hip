@@ -101,9 +101,15 @@ __global__ __launch_bounds__(256) void k_cell_coords(const double* __restrict__ 
   }
 }
 
+// Four items per thread: one 16-byte store when p is 16-byte aligned and the
+// four are whole, else item by item.
 __global__ void k_fill_int(int* __restrict__ p, int64_t n, int v) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  const int64_t i = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (i + 4 <= n && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+    reinterpret_cast<int4*>(p + i)[0] = make_int4(v, v, v, v);
+  } else {
+    for (int64_t k = i; k < n && k < i + 4; ++k) p[k] = v;
+  }
 }
 
 // Counting sort by cell, stable by particle index, in four launches:
@@ -560,7 +566,7 @@ void launch_cell_coords(const double* x, const double* y, const double* z, int n
 }
 
 void launch_fill_int(int* p, int64_t n, int v, hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(blocks(n, 256)), dim3(256), 0, st, p, n, v);
+  if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(blocks((n + 3) / 4, 256)), dim3(256), 0, st, p, n, v);
 }
 
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, const QuadField& q, const double* fv,
