@@ -231,14 +231,17 @@ __device__ __forceinline__ unsigned sector_bits(double dx, double dy, double dz)
 // to call at that error; then the FP64 offset decides.
 __device__ __forceinline__ unsigned sector_bits_f32(float dx, float dy, float dz, float es, float ec, bool* sure) {
   const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
-  // 0/1 flags in integer arithmetic (a chain of selects would become
-  // branches): negative signs, and which axis dominates (x, else y, else z).
-  const unsigned nx = dx >= 0.f ? 0u : 1u, ny = dy >= 0.f ? 0u : 1u, nz = dz >= 0.f ? 0u : 1u;
-  const unsigned isx = (ax >= ay) & (ax >= az), isy = (1u - isx) & (ay >= az), isz = 1u - isx - isy;
-  const unsigned axis = isx * nx + isy * (2u + ny) + isz * (4u + nz);
-  const unsigned oct = 6u + (1u - nx) + 2u * (1u - ny) + 4u * (1u - nz);
   *sure = fminf(fminf(ax, ay), az) > es && fminf(fminf(fabsf(ax - ay), fabsf(ax - az)), fabsf(ay - az)) > ec;
-  return (1u << axis) | (1u << oct);
+  // Sign bits (1 = negative). A component at or near zero makes the result
+  // unsure above, so the sign of a zero never decides.
+  const unsigned nx = __float_as_uint(dx) >> 31, ny = __float_as_uint(dy) >> 31, nz = __float_as_uint(dz) >> 31;
+  // Octant bit 6 + (dx >= 0) + 2 (dy >= 0) + 4 (dz >= 0) = 13 - (nx + 2 ny + 4 nz).
+  const unsigned oct = 0x2000u >> (nx | (ny << 1) | (nz << 2));
+  // Dominant axis (ties x, then y): bit 0/1 for +x/-x, 2/3 for y, 4/5 for z.
+  const bool xmax = ax >= ay && ax >= az, ymax = !xmax && ay >= az;
+  const unsigned bx = 1u + nx, by = 4u << ny, bz = 16u << nz;
+  const unsigned axis = xmax ? bx : (ymax ? by : bz);
+  return axis | oct;
 }
 
 __device__ __forceinline__ uint8_t classify_rule(uint8_t t0, int kept, double z, double zlim, unsigned mask) {
