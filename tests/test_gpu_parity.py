@@ -260,6 +260,36 @@ def test_kept_grid_reused_and_outgrown(pipe):
     assert np.array_equal(t2, pipe.types())
 
 
+def test_field_values_and_quadratic_paths_agree(pipe):
+    """The field reaches the sorted records four ways: evaluated (quadratic)
+    or gathered (values), by the neighbor build when the field is set before
+    it, else by the WLS stage. The host values of the quadratic are the
+    device's bit for bit, so all four FP64 WLS results are identical."""
+    l0 = 0.01
+    re = 2.1 * l0
+    xyz, t = L.generate_case("wall_film", 8000, 3, l0)
+    vals = L.cases.field_values(xyz, L.C1_FIELD)
+    say("field paths", len(xyz))
+    pipe.upload(xyz, t)
+    pipe.set_field(vals)  # values, gathered by the build
+    pipe.neighbors_classify(re, 96, l0)
+    pipe.wls(re, l0, L.WLS_FP64)
+    ref = pipe.wls_results()
+    pipe.set_field(L.C1_FIELD)  # quadratic, evaluated by the WLS stage
+    pipe.wls(re, l0, L.WLS_FP64)
+    out = [pipe.wls_results()]
+    pipe.set_field(vals)  # values, gathered by the WLS stage
+    pipe.wls(re, l0, L.WLS_FP64)
+    out.append(pipe.wls_results())
+    pipe.set_field(L.C1_FIELD)  # quadratic, evaluated by the build
+    pipe.neighbors_classify(re, 96, l0)
+    pipe.wls(re, l0, L.WLS_FP64)
+    out.append(pipe.wls_results())
+    for g, lp, st in out:
+        assert np.array_equal(g, ref[0]) and np.array_equal(lp, ref[1]) and np.array_equal(st, ref[2])
+    assert int((ref[2] <= 1).sum()) > 0
+
+
 def test_solvers_reach_tolerance_and_agree(pipe):
     xyz, t = L.generate_case("bulk_lattice", 50000, 7, 0.01)
     say("solvers C1", len(xyz))
