@@ -345,7 +345,7 @@ int lsmps_create(int device, lsmps_ctx** out) {
       hipHostMalloc(reinterpret_cast<void**>(&ctx->host_small), 64 * sizeof(int64_t)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->host_rb), HOST_RB_BYTES) != hipSuccess ||
       ctx->counters.ensure(16) != hipSuccess || ctx->sc.ensure(1) != hipSuccess ||
-      ctx->partials.ensure(4 * SOLVER_MAX_GRID) != hipSuccess || ctx->box.ensure(8) != hipSuccess ||
+      ctx->partials.ensure(5 * SOLVER_MAX_GRID) != hipSuccess || ctx->box.ensure(8) != hipSuccess ||
       ctx->nredo.ensure(1) != hipSuccess) {
     lsmps_destroy(ctx);
     return LSMPS_E_HIP;

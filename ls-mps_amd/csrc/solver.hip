@@ -18,15 +18,18 @@
 // and neighboring rows (neighboring particles) gather neighboring vector
 // entries. Padding entries hold value 0 and the row's own column.
 //
-// One iteration is five streaming launches and three one-block launches
-// that add the block partials in fixed order and update the
-// device-resident scalars (no host round trip):
+// One iteration is five streaming launches and one one-block launch, with
+// the scalars device-resident (no host round trip):
 //   k_bicg_p   p = r + beta (p - omega v)
-//   k_bicg_v   v = A p, partial r0.v                  -> k_fin_alpha
-//   k_bicg_s   s = r - alpha v
-//   k_bicg_t   t = A s, partial s.s, t.t, t.s         -> k_fin_omega
-//   k_bicg_xr  y += alpha p + omega s (FP64), r = s - omega t, partial
-//              r.r, r0.r                              -> k_fin_iter
+//   k_bicg_v   v = A p, partial r0.v
+//   k_bicg_s   alpha = rho / r0.v, s = r - alpha v
+//   k_bicg_t   t = A s, partial s.s, t.t, t.s
+//   k_bicg_xr  omega = t.s / t.t, y += alpha p + omega s (FP64),
+//              r = s - omega t, partial r.r, r0.r     -> k_fin_iter
+// Every block of k_bicg_s and k_bicg_xr adds the previous launch's block
+// partials itself, in the same fixed order, so all blocks hold the same
+// alpha (omega) bit for bit and no one-block launch sits between the two
+// streaming ones; block 0 stores the scalar for the later launches.
 // so each matrix entry gathers one vector entry. Columns inside a row are
 // stored in ascending order, so the 64 rows of a slice (64 neighboring
 // particles) gather neighboring entries.
@@ -152,27 +155,25 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_v(const int* __restrict
   block_partials<1>(vv, partials, lds);
 }
 
-__global__ __launch_bounds__(SOLVER_THREADS) void k_fin_alpha(const double* __restrict__ partials, int nblk,
-                                                              SolverScalars* sc) {
+// alpha from the r0.v block partials of k_bicg_v (every block adds them in
+// the same order), then s = r - alpha v. A breakdown (r0.v ~ 0) is seen by
+// every block alike; block 0 records it and alpha.
+__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_s(int m, const double* __restrict__ r,
+                                                           const double* __restrict__ v, double* __restrict__ s,
+                                                           SolverScalars* sc, const double* __restrict__ partials,
+                                                           int nblk) {
   if (!iter_active(sc)) return;
   __shared__ double lds[SOLVER_THREADS / WAVE];
   double tot[1];
   finalize_sum<1>(partials, nblk, tot, lds);
-  if (threadIdx.x == 0) {
-    const double den = tot[0];
-    if (fabs(den) < 1e-30 || !isfinite(den)) {
-      sc->state = SOLVE_BREAKDOWN;
-    } else {
-      sc->alpha = sc->rho / den;
-    }
+  const double den = tot[0];
+  const bool broke = fabs(den) < 1e-30 || !isfinite(den);
+  const double alpha = broke ? 0.0 : sc->rho / den;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (broke) sc->state = SOLVE_BREAKDOWN;
+    else sc->alpha = alpha;
   }
-}
-
-__global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_s(int m, const double* __restrict__ r,
-                                                           const double* __restrict__ v, double* __restrict__ s,
-                                                           const SolverScalars* sc) {
-  if (!iter_active(sc)) return;
-  const double alpha = sc->alpha;
+  if (broke) return;
   for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS)
     s[i] = fma(-alpha, v[i], r[i]);
 }
@@ -197,36 +198,41 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_t(const int* __restrict
   block_partials<3>(vv, partials, lds);
 }
 
-__global__ __launch_bounds__(SOLVER_THREADS) void k_fin_omega(const double* __restrict__ partials, int nblk,
-                                                              SolverScalars* sc) {
-  if (!iter_active(sc)) return;
-  __shared__ double lds[3 * SOLVER_THREADS / WAVE];
-  double tot[3];
-  finalize_sum<3>(partials, nblk, tot, lds);
-  if (threadIdx.x == 0) {
-    const double sn = sqrt(tot[0]);
-    sc->sn = sn;
-    if ((sc->flags & SOLVE_S_EXIT) && sn <= sc->thresh) {
-      sc->state = SOLVE_S_CONVERGED;
-      sc->iterations = sc->it + 1;
-    } else if (tot[1] < 1e-30 || !isfinite(tot[1]) || !isfinite(tot[2])) {
-      sc->state = SOLVE_BREAKDOWN;
-    } else {
-      sc->omega = tot[2] / tot[1];
-    }
-  }
-}
-
+// omega from the s.s, t.t, t.s block partials of k_bicg_t (every block
+// adds them in the same order), then y += alpha p + omega s,
+// r = s - omega t, and this launch's r.r and r0.r partials at
+// partials + 3 * nblk (the first 3 * nblk are still being read by other
+// blocks). When ||s|| already meets the tolerance (SOLVE_S_EXIT) the update
+// is y += alpha p and the iteration ends there. Block 0 records ||s||,
+// omega and the state for the later launches. The state read at entry is
+// RUNNING, or S_CONVERGED when block 0 of this launch has already stored
+// the decision every block reaches.
 __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const double* __restrict__ p,
                                                             const double* __restrict__ s, const double* __restrict__ t,
                                                             const double* __restrict__ r0, double* __restrict__ r,
                                                             double* __restrict__ y, SolverScalars* sc,
-                                                            double* __restrict__ partials) {
-  const int state = sc->state;
-  const bool sconv = state == SOLVE_S_CONVERGED;
-  if (!(sconv || iter_active(sc))) return;
-  __shared__ double lds[2 * SOLVER_THREADS / WAVE];
-  const double alpha = sc->alpha, omega = sc->omega;
+                                                            double* __restrict__ partials, int nblk) {
+  if (!(iter_active(sc) || sc->state == SOLVE_S_CONVERGED)) return;
+  __shared__ double lds[3 * SOLVER_THREADS / WAVE];
+  double tot[3];
+  finalize_sum<3>(partials, nblk, tot, lds);
+  const double sn = sqrt(tot[0]);
+  const bool sconv = (sc->flags & SOLVE_S_EXIT) && sn <= sc->thresh;
+  const bool broke = !sconv && (tot[1] < 1e-30 || !isfinite(tot[1]) || !isfinite(tot[2]));
+  const double omega = sconv || broke ? 0.0 : tot[2] / tot[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->sn = sn;
+    if (sconv) {
+      sc->state = SOLVE_S_CONVERGED;
+      sc->iterations = sc->it + 1;
+    } else if (broke) {
+      sc->state = SOLVE_BREAKDOWN;
+    } else {
+      sc->omega = omega;
+    }
+  }
+  if (broke) return;
+  const double alpha = sc->alpha;
   double rr = 0.0, r0r = 0.0;
   for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS) {
     if (sconv) {
@@ -241,7 +247,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const double*
     }
   }
   double vv[2] = {rr, r0r};
-  block_partials<2>(vv, partials, lds);
+  block_partials<2>(vv, partials + 3 * (size_t)nblk, lds);
 }
 
 __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_iter(const double* __restrict__ partials, int nblk,
@@ -480,12 +486,10 @@ void launch_solver_iteration(const SolverBufs<TM>& B, bool with_replace, SolverS
   const dim3 one(1), blk(SOLVER_THREADS), grid(g);
   hipLaunchKernelGGL(k_bicg_p, grid, blk, 0, st, m, B.r, B.v, B.p, sc);
   hipLaunchKernelGGL((k_bicg_v<TM>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.p, B.r0, B.v, sc, B.partials);
-  hipLaunchKernelGGL(k_fin_alpha, one, blk, 0, st, B.partials, (int)g, sc);
-  hipLaunchKernelGGL(k_bicg_s, grid, blk, 0, st, m, B.r, B.v, B.s, sc);
+  hipLaunchKernelGGL(k_bicg_s, grid, blk, 0, st, m, B.r, B.v, B.s, sc, (const double*)B.partials, (int)g);
   hipLaunchKernelGGL((k_bicg_t<TM>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.s, B.t, sc, B.partials);
-  hipLaunchKernelGGL(k_fin_omega, one, blk, 0, st, B.partials, (int)g, sc);
-  hipLaunchKernelGGL(k_bicg_xr, grid, blk, 0, st, m, B.p, B.s, B.t, B.r0, B.r, B.y, sc, B.partials);
-  hipLaunchKernelGGL(k_fin_iter, one, blk, 0, st, B.partials, (int)g, sc);
+  hipLaunchKernelGGL(k_bicg_xr, grid, blk, 0, st, m, B.p, B.s, B.t, B.r0, B.r, B.y, sc, B.partials, (int)g);
+  hipLaunchKernelGGL(k_fin_iter, one, blk, 0, st, B.partials + 3 * (size_t)g, (int)g, sc);
   if (with_replace) {
     hipLaunchKernelGGL(k_bicg_true_residual<false>, grid, blk, 0, st, B.sptr, B.col, B.val64, m, B.b, B.scale,
                        (const double*)B.y, B.r, B.r0, B.p, B.v, sc, B.partials);

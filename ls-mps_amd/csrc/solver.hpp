@@ -54,7 +54,7 @@ struct SolverBufs {
   double* v = nullptr;
   double* s = nullptr;
   double* t = nullptr;
-  double* partials = nullptr;      // >= 3 * SOLVER_MAX_GRID doubles
+  double* partials = nullptr;      // >= 5 * SOLVER_MAX_GRID doubles
 };
 
 unsigned solver_grid(int m);
