@@ -343,6 +343,25 @@ def test_solver_max_iter_and_breakdown_paths(pipe):
     assert st["converged"] == 0 and st["iterations"] == 0
 
 
+def test_plain_solver_stops_on_half_step_residual(pipe):
+    # One unknown: the identity pin row, A = [1]. The reference's BiCGStab
+    # (src/lsmps.cpp:67) then has alpha = 1 and s = r - alpha A p = 0 in its
+    # first iteration and stops on ||s||, with x = alpha p = b = x*, exactly.
+    d = load_fixture("film_1200")
+    m = d["meta"]
+    re = m["re_ratio"] * m["l0"]
+    pipe.upload(d["xyz"], d["type_in"])
+    pipe.neighbors(re, m["max_neighbors"])
+    pipe.classify(m["l0"])
+    pipe.assemble(re, 1)
+    A = pipe.ppe()
+    assert len(A["b"]) == 1 and A["val"].tolist() == [1.0]
+    st = pipe.solve(L.SOLVE_PLAIN, 100, 1e-7)
+    say("one-row plain solve", st)
+    assert st["converged"] == 1 and st["iterations"] == 1
+    assert pipe.solution().tolist() == A["x_exact"].tolist()
+
+
 def test_deterministic(pipe):
     xyz, t = L.generate_case("rain_injection", 30000, 2, 0.01)
     outs = []
