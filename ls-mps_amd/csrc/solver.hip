@@ -59,8 +59,18 @@ namespace lsmps_hip {
 constexpr int SOLVER_THREADS = 256;
 constexpr int SLICE = 64;
 
+// The state word is read by every block of k_bicg_s and k_bicg_xr while
+// block 0 of the same launch may store a decision that every block reaches
+// by itself; relaxed atomic accesses make that concurrent access defined.
+__device__ __forceinline__ int load_state(const SolverScalars* sc) {
+  return __hip_atomic_load(&sc->state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_state(SolverScalars* sc, int state) {
+  __hip_atomic_store(&sc->state, state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ bool iter_active(const SolverScalars* sc) {
-  return sc->state == SOLVE_RUNNING && sc->it < sc->max_iter;
+  return load_state(sc) == SOLVE_RUNNING && sc->it < sc->max_iter;
 }
 
 // After a step that leaves r updated: decide stop/continue and next beta.
@@ -170,7 +180,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_s(int m, const double* 
   const bool broke = fabs(den) < 1e-30 || !isfinite(den);
   const double alpha = broke ? 0.0 : sc->rho / den;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (broke) sc->state = SOLVE_BREAKDOWN;
+    if (broke) store_state(sc, SOLVE_BREAKDOWN);
     else sc->alpha = alpha;
   }
   if (broke) return;
@@ -212,7 +222,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const double*
                                                             const double* __restrict__ r0, double* __restrict__ r,
                                                             double* __restrict__ y, SolverScalars* sc,
                                                             double* __restrict__ partials, int nblk) {
-  if (!(iter_active(sc) || sc->state == SOLVE_S_CONVERGED)) return;
+  if (!(iter_active(sc) || load_state(sc) == SOLVE_S_CONVERGED)) return;
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
   double tot[3];
   finalize_sum<3>(partials, nblk, tot, lds);
@@ -223,10 +233,10 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const double*
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->sn = sn;
     if (sconv) {
-      sc->state = SOLVE_S_CONVERGED;
+      store_state(sc, SOLVE_S_CONVERGED);
       sc->iterations = sc->it + 1;
     } else if (broke) {
-      sc->state = SOLVE_BREAKDOWN;
+      store_state(sc, SOLVE_BREAKDOWN);
     } else {
       sc->omega = omega;
     }
