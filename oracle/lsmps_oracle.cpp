@@ -778,6 +778,7 @@ void orc_classify(orc_state* s, double l0) {
   s->type.swap(out);
 }
 void orc_get_types(const orc_state* s, uint8_t* type) { std::memcpy(type, s->type.data(), s->type.size()); }
+double orc_hashed_uniform(uint32_t seed, uint64_t k) { return hashed_uniform(seed, k); }
 void orc_set_types(orc_state* s, const uint8_t* type) { std::memcpy(s->type.data(), type, s->type.size()); }
 
 void orc_wls_all(const orc_state* s, const double* f, double re, double l0, double reg, double* grad3,

@@ -97,6 +97,9 @@ double orc_wls_moments(const orc_state* s, const double* f, double re, double l0
 /* The reference's sampled quadratic test (src/lsmps.cpp:28). */
 void orc_wls_test(const orc_state* s, double re, double l0, double reg, int wls_samples, orc_wls_stats* out);
 
+/* The x* of row k under xmode 1: the splitmix64 finaliser of seed * 2^64/phi + k,
+ * top 53 bits, mapped to [-1, 1). */
+double orc_hashed_uniform(uint32_t seed, uint64_t k);
 /* PPE assembly (src/lsmps.cpp:29-64). xmode 0: x* = x^2+y^2 (reference);
  * xmode 1: x* ~ U(-1,1) from the splitmix64 hash of (xseed, row). shift adds shift*diag to each
  * diagonal after assembly (0 = reference). */

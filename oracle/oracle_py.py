@@ -90,6 +90,8 @@ def lib():
         L.orc_run.argtypes = [_p, _d, _d, C.c_int, C.c_int, C.c_int, C.c_int, _d, C.c_int,
                               C.POINTER(RunStats), C.POINTER(WlsStats), C.POINTER(PpeStats)]
         L.orc_features.argtypes = [_p, _d, _p, C.POINTER(_d)]
+        L.orc_hashed_uniform.restype = _d
+        L.orc_hashed_uniform.argtypes = [C.c_uint32, _u64]
         L.orc_mlp_forward.argtypes = [_p, _p, _u64, _p]
         L.orc_mlp_loss_grad.restype = _d
         L.orc_mlp_loss_grad.argtypes = [_p, _p, _p, _p, _u64, _p]

@@ -439,3 +439,18 @@ def test_wls_ppe_fallback_rows():
     assert A["row"][-1] - A["row"][-2] == 1 and A["val"][-1] == 1.0
     st = o.assemble_wls(re, l0, 1 << 30, xmode=1, xseed=2)
     assert st["graph_rows"] < len(xyz) and st["failed_rows"] == 1
+
+
+def test_hashed_uniform_known_answers():
+    # The x* of xmode 1 is the splitmix64 finaliser of seed * 2^64/phi + k.
+    # The published splitmix64 outputs for state 1234567 (Vigna's reference
+    # splitmix64.c) are the finaliser of 1234567 + j * 2^64/phi, j = 1..5:
+    # that is the hash at seed j, row 1234567.
+    published = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                 4593380528125082431, 16408922859458223821]
+    L = O.lib()
+    for j, v in enumerate(published, start=1):
+        want = -1.0 + 2.0 * ((v >> 11) * 2.0 ** -53)
+        assert L.orc_hashed_uniform(j, 1234567) == want
+    # Seed 0 is the finaliser of the row itself; the finaliser maps 0 to 0.
+    assert L.orc_hashed_uniform(0, 0) == -1.0
