@@ -1,0 +1,68 @@
+"""Parity at the benchmark's full size: the 3,000,000-particle industrial_mix
+scene (l0 = 0.004, re = 2.1 l0, max_neighbors 128), GPU against the CPU
+oracle, bit for bit: neighbor lists in the reference's order
+(src/lsmps.cpp:26), types (src/lsmps.cpp:27), the FP64 WLS gradient,
+Laplacian and status of every particle (src/lsmps.cpp:28) and the PPE CSR,
+b and x* (src/lsmps.cpp:29-64). The oracle takes about 10 s of CPU here.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "ls-mps_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import lsmps_amd as L  # noqa: E402
+import oracle_py as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+N, L0, MAXN = 3_000_000, 0.004, 128
+RE = 2.1 * L0
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+
+
+def test_industrial_mix_3m_bitexact():
+    t0 = time.time()
+    xyz, types = L.generate_case("industrial_mix", N, 23, L0)
+    say("industrial_mix", N, "particles generated")
+    p = L.Pipeline(0)
+    try:
+        p.upload(xyz, types)
+        cs, _ = p.neighbors_classify(RE, MAXN, L0)
+        row, col = p.neighbor_lists()
+        gtypes = p.types()
+        p.set_field(L.C1_FIELD)
+        p.wls(RE, L0, L.WLS_FP64)
+        g, lap, st = p.wls_results()
+        p.assemble(RE, 1 << 30)
+        A = p.ppe()
+    finally:
+        p.close()
+    say("GPU done: pairs", int(row[-1]), "unknowns", len(A["b"]), f"{time.time() - t0:.1f} s")
+
+    o = O.Oracle(xyz, types)
+    pairs, trunc = o.neighbors(RE, MAXN)
+    orow, ocol = o.get_neighbors()
+    assert cs["truncated"] == trunc
+    assert np.array_equal(row, orow) and np.array_equal(col, ocol)
+    del orow, ocol, col
+    say("neighbor lists equal", f"{time.time() - t0:.1f} s")
+    o.classify(L0)
+    assert np.array_equal(gtypes, o.types())
+    og, ol, ost = o.wls_all(L.cases.field_values(xyz, L.C1_FIELD), RE, L0)
+    assert np.array_equal(st, ost)
+    assert np.array_equal(g, og) and np.array_equal(lap, ol)
+    say("types and FP64 WLS equal", f"{time.time() - t0:.1f} s")
+    o.assemble(RE, 1 << 30)
+    B = o.get_ppe()
+    for k in ("row", "col", "val", "b", "x_exact"):
+        assert np.array_equal(A[k], B[k]), k
+    say("PPE equal, nnz", len(B["val"]), f"{time.time() - t0:.1f} s")
