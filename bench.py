@@ -49,9 +49,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--particles", type=int, default=3_000_000)
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--solvers", default="fp64_jacobi,mixed,learned_mixed,wls_rows",
+    ap.add_argument("--solvers", default="fp64_jacobi,mixed,learned_mixed,wls_rows,xstar_quadratic",
                     help="comma list of the PPE solves to time (fp64_jacobi, mixed, learned_mixed; wls_rows: "
-                         "the WLS-Laplacian rows, assembled and solved with FP64 Jacobi and mixed)")
+                         "the WLS-Laplacian rows, assembled and solved with FP64 Jacobi and mixed; "
+                         "xstar_quadratic: the reference's x* = x^2 + y^2)")
+    ap.add_argument("--xstar-seeds", type=int, default=5,
+                    help="seeded right-hand sides over which time-to-1e-8 is reported (median)")
+    ap.add_argument("--no-components", action="store_true",
+                    help="skip the per-component error of the solution (needs the CSR on the host)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-max-iter", type=int, default=40000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -73,6 +78,11 @@ def cnl_bytes_per_particle(k_mean):
     # k_cnl_pass (neighbor search + classification): position 24 and type 1
     # read; neighbor indices 4k, list length 4, new type 1 written.
     return 24 + 1 + 4 * k_mean + 4 + 1
+
+
+def wls_exact_bytes_per_particle(k_mean):
+    # k_wls_exact: the same inputs and outputs as the mixed kernel.
+    return wls_bytes_per_particle(k_mean)
 
 
 def assembly_bytes(n, m, nnz, k_mean):
@@ -141,6 +151,33 @@ def cpu_baseline(xyz, types, l0, maxn):
         ref = O.RefHarness()
     except (FileNotFoundError, OSError):
         ref = None
+    # One host core, as taskset -c would pin it: the process's affinity is
+    # narrowed to the first core it may run on for the timed CPU stages.
+    allowed = sorted(os.sched_getaffinity(0))
+    core = allowed[0]
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    host = {"cores": 1, "threads": 1, "pinned_core": core, "cpu_model": model, "host_cpus": len(allowed)}
+    os.sched_setaffinity(0, {core})
+    try:
+        return _cpu_baseline_timed(ref, xyz, types, l0, maxn, n, host)
+    finally:
+        os.sched_setaffinity(0, set(allowed))
+
+
+def _cpu_baseline_timed(ref, xyz, types, l0, maxn, n, host):
+    import ctypes as C
+
+    import numpy as np
+    import oracle_py as O
+
     if ref is not None:
         # wls_samples = n makes the reference's wls_test visit every particle
         # (stride 1), so its WLS stage covers the whole workload.
@@ -148,16 +185,18 @@ def cpu_baseline(xyz, types, l0, maxn):
         out = np.zeros(4)
         ref.L.ref_time_stages(out.ctypes.data_as(C.c_void_p))
         cpu_ms = out[0] + out[1] + out[2]
-        return {
+        return dict({
             "value": round(n / cpu_ms / 1e3, 4),
             "unit": "Mparticles/s",
-            "cores": 1,
             "kind": "reference",
+            "precision": "FP64 WLS (compare value_fp64)",
+            "stage_ms": {"neighbors": round(out[0], 1), "classify": round(out[1], 1), "wls_all": round(out[2], 1),
+                         "assemble_ppe": round(out[3], 1)},
             "sample": f"the full {n}-particle workload through the reference's src/lsmps.cpp (g++ -O3, its own "
-                      f"flags), one thread: neighbors {out[0]:.0f} ms + classify {out[1]:.0f} ms + wls_test over "
-                      f"all particles {out[2]:.0f} ms; its assemble_ppe over all unknowns took {out[3]:.0f} ms "
-                      f"(not in value)",
-        }
+                      f"flags), one thread pinned to core {host['pinned_core']}: neighbors {out[0]:.0f} ms + "
+                      f"classify {out[1]:.0f} ms + wls_test over all particles {out[2]:.0f} ms; its assemble_ppe "
+                      f"over all unknowns took {out[3]:.0f} ms (not in value)",
+        }, **host)
     import lsmps_amd as L
 
     n_sample = max(10_000, n // 10)
@@ -166,14 +205,34 @@ def cpu_baseline(xyz, types, l0, maxn):
     rs, _, _ = o.run(l0, 2.1, maxn, 8192, 0, 0, 0.0, do_solve=False)
     cpu_ms = rs["neighbor_ms"] + rs["classify_ms"] + rs["wls_all_ms"]
     return {
+        **host,
         "value": round(n_sample / cpu_ms / 1e3, 4),
         "unit": "Mparticles/s",
-        "cores": 1,
         "kind": "port",
         "sample": f"industrial_mix scene of {n_sample} particles (same generator): neighbors "
                   f"{rs['neighbor_ms']:.0f} ms + classify {rs['classify_ms']:.0f} ms + WLS over all particles "
                   f"{rs['wls_all_ms']:.0f} ms, single thread",
     }
+
+
+def component_error(A, x):
+    """Relative L2 and max error of x against x* after removing, on every
+    connected component of the PPE graph, the mean of x - x*: the graph
+    Laplacian pinned only at row 0 determines the solution on the other
+    components up to a constant, which any converged solver may carry."""
+    import numpy as np
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    m = len(A["b"])
+    g = csr_matrix((np.ones(len(A["col"]), np.int8), A["col"], A["row"]), shape=(m, m))
+    ncomp, lab = connected_components(g, directed=False)
+    e = x - A["x_exact"]
+    mean = np.bincount(lab, weights=e, minlength=ncomp) / np.maximum(np.bincount(lab, minlength=ncomp), 1)
+    mean[lab[0]] = 0.0  # the pinned component is determined exactly
+    e = e - mean[lab]
+    return {"components": int(ncomp), "relative_l2_error": float(np.linalg.norm(e) / np.linalg.norm(A["x_exact"])),
+            "max_error": float(np.abs(e).max())}
 
 
 def main():
@@ -197,39 +256,48 @@ def main():
     p.upload(xyz, types)
     p.set_field(L.C1_FIELD)
 
-    def step():
-        cs, ts = p.neighbors_classify(re, maxn, l0)
-        wr = p.wls(re, l0, L.WLS_MIXED)
-        return cs, ts, wr
+    def run_steps(precision, k):
+        stage = {"cnl": [], "wls": [], "k_cnl_pass": [], "k_wls": []}
+        p.L.lsmps_synchronize(p.h)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            cs, ts = p.neighbors_classify(re, maxn, l0)
+            wr = p.wls(re, l0, precision)
+            stage["cnl"].append(cs["ms"])
+            stage["wls"].append(wr["ms"])
+            stage["k_cnl_pass"].append(cs["kernel_ms"])
+            stage["k_wls"].append(wr["kernel_ms"])
+        p.L.lsmps_synchronize(p.h)
+        wall = (time.perf_counter() - t0) * 1e3 / max(1, k)
+        return wall, {key: float(np.mean(v)) if v else 0.0 for key, v in stage.items()}, cs, ts, wr
 
-    for _ in range(args.warmup):
-        step()
-    log(f"warmup done ({args.warmup}); timing {args.steps} steps")
-    p.L.lsmps_synchronize(p.h)
-    t0 = time.perf_counter()
-    stage = {"cnl": [], "wls": [], "k_cnl_pass": [], "k_wls_mixed": []}
-    for _ in range(args.steps):
-        cs, ts, wr = step()
-        stage["cnl"].append(cs["ms"])
-        stage["wls"].append(wr["ms"])
-        stage["k_cnl_pass"].append(cs["kernel_ms"])
-        stage["k_wls_mixed"].append(wr["kernel_ms"])
-    p.L.lsmps_synchronize(p.h)
-    wall_ms = (time.perf_counter() - t0) * 1e3 / max(1, args.steps)
-    avg = {k: float(np.mean(v)) if v else 0.0 for k, v in stage.items()}
+    # The step at the reference's precision (FP64 WLS, bit-exact with
+    # wls_test) and with the mixed WLS (FP32 moments, FP64 redo of the rows
+    # the reference would regularise); the same neighbor pass in both.
+    run_steps(L.WLS_FP64, args.warmup)
+    log(f"warmup done ({args.warmup}); timing {args.steps} FP64 steps")
+    wall64, avg64, cs, ts, wr64 = run_steps(L.WLS_FP64, args.steps)
+    run_steps(L.WLS_MIXED, args.warmup)
+    log(f"timing {args.steps} mixed steps")
+    wall32, avg32, cs, ts, wr = run_steps(L.WLS_MIXED, args.steps)
     kmean = cs["mean"]
-    log(f"step wall {wall_ms:.3f} ms, gpu stages {avg}, k_mean {kmean:.2f}, wls {wr}")
+    log(f"step wall fp64 {wall64:.3f} ms {avg64}; mixed {wall32:.3f} ms {avg32}; k_mean {kmean:.2f}; wls {wr}")
+    avg = {"cnl": avg32["cnl"], "wls_mixed": avg32["wls"], "wls_fp64": avg64["wls"],
+           "k_cnl_pass": 0.5 * (avg32["k_cnl_pass"] + avg64["k_cnl_pass"]), "k_wls_mixed": avg32["k_wls"],
+           "k_wls_exact": avg64["k_wls"]}
 
-    # Roofline of the dominant kernel: algorithmic bytes of one launch over
-    # its average duration (HIP events around that launch on the library's
-    # stream).
-    per_particle = {"k_cnl_pass": cnl_bytes_per_particle(kmean), "k_wls_mixed": wls_bytes_per_particle(kmean)}
-    dom = max(per_particle, key=lambda k: avg[k])
+    # Roofline of the dominant kernel of the headline step: algorithmic bytes
+    # of one launch over its average duration (HIP events around that launch
+    # on the library's stream).
+    per_particle = {"k_cnl_pass": cnl_bytes_per_particle(kmean), "k_wls_mixed": wls_bytes_per_particle(kmean),
+                    "k_wls_exact": wls_exact_bytes_per_particle(kmean)}
+    kernel_gbs = {k: round(n * per_particle[k] / (avg[k] * 1e-3) / 1e9, 1) for k in per_particle if avg[k] > 0}
+    dom = max(("k_cnl_pass", "k_wls_mixed"), key=lambda k: avg[k])
     dom_bytes = n * per_particle[dom]
     dom_gbs = dom_bytes / (avg[dom] * 1e-3) / 1e9
-    kernel_gbs = {k: round(n * per_particle[k] / (avg[k] * 1e-3) / 1e9, 1) for k in per_particle if avg[k] > 0}
 
-    extra = {"stage_ms": {k: round(v, 4) for k, v in avg.items()}, "neighbor_mean": kmean, "pairs": cs["pairs"],
+    extra = {"value_fp64": round(n / wall64 / 1e3, 3), "ms_per_step_fp64": round(wall64, 4),
+             "stage_ms": {k: round(v, 4) for k, v in avg.items()}, "neighbor_mean": kmean, "pairs": cs["pairs"],
              "types": ts["count"], "wls_fp64_redo": wr["fp64_redo"], "kernel_algorithmic_GBs": kernel_gbs}
 
     if not args.no_solve:
@@ -241,9 +309,17 @@ def main():
         asm_bytes = assembly_bytes(n, pr["unknowns"], pr["nnz"], kmean)
         extra["ppe"] = {"unknowns": pr["unknowns"], "nnz": pr["nnz"], "padded_nnz": pr["padded_nnz"],
                         "assembly_ms": round(pr["ms"], 3), "fill_kernel_ms": round(pr["kernel_ms"], 4),
-                        "first_call_ms": round(pr0["ms"], 3),
+                        "first_call_ms": round(pr0["ms"], 3), "algorithmic_bytes": int(asm_bytes),
                         "algorithmic_GBs": round(asm_bytes / (pr["ms"] * 1e-3) / 1e9, 1)}
         log(f"assembly {pr}")
+        # The target's aggregate: neighbor search + classification, WLS and
+        # PPE assembly, algorithmic bytes over the three stage times.
+        for tag, wkey, wk in (("mixed", "wls_mixed", "k_wls_mixed"), ("fp64", "wls_fp64", "k_wls_exact")):
+            b = n * (per_particle["k_cnl_pass"] + per_particle[wk]) + asm_bytes
+            t_ms = avg["cnl"] + avg[wkey] + pr["ms"]
+            extra.setdefault("aggregate", {})[tag] = {
+                "bytes": int(b), "ms": round(t_ms, 4), "GBs": round(b / (t_ms * 1e-3) / 1e9, 1),
+                "frac": round(b / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         solves = {}
         wanted = set(args.solvers.split(","))
         methods = [(k, m) for k, m in (("fp64_jacobi", L.SOLVE_JACOBI), ("mixed", L.SOLVE_MIXED)) if k in wanted]
@@ -253,18 +329,49 @@ def main():
 
             p.set_params(learn.load_params(weights))
             methods.append(("learned_mixed", L.SOLVE_LEARNED))
-        for name, method in methods:
+        for name, method in methods:  # allocation and graph capture
             st0 = p.solve(method, args.solve_max_iter, 1e-8, chunk=32)
-            st = p.solve(method, args.solve_max_iter, 1e-8, chunk=32)
-            log(f"solve {name}: {st}")
-            solves[name] = {"ms": round(st["ms"], 3), "iterations": st["iterations"], "converged": st["converged"],
-                            "final_residual": st["final_residual"], "replacements": st["replacements"],
-                            "fallback": st["fallback"], "initial_residual": st["initial_residual"],
-                            "first_call_ms": round(st0["ms"], 3)}
+            solves[name] = {"first_call_ms": round(st0["ms"], 3), "per_seed": []}
+        seeds = [11 + k for k in range(max(1, args.xstar_seeds))]
+        for seed in seeds:
+            if seed != 11:
+                p.assemble(re, 1 << 30, L.XSTAR_RANDOM, seed)
+            for name, method in methods:
+                st = p.solve(method, args.solve_max_iter, 1e-8, chunk=32)
+                log(f"x* seed {seed} solve {name}: {st}")
+                solves[name]["per_seed"].append(
+                    {"seed": seed, "ms": round(st["ms"], 3), "iterations": st["iterations"],
+                     "converged": st["converged"], "final_residual": st["final_residual"],
+                     "replacements": st["replacements"], "breakdown_restarts": st["breakdown_restarts"],
+                     "fallback": st["fallback"], "initial_residual": st["initial_residual"]})
+                if seed == seeds[-1] and name == methods[-1][0] and not args.no_components:
+                    A = p.ppe()
+                    for nm, mt in methods:
+                        p.solve(mt, args.solve_max_iter, 1e-8, chunk=32)
+                        solves[nm]["error_modulo_components"] = component_error(A, p.solution())
+                    del A
+        for name in solves:
+            ps = solves[name]["per_seed"]
+            solves[name]["median_ms"] = round(float(np.median([r["ms"] for r in ps])), 3)
+            solves[name]["median_iterations"] = float(np.median([r["iterations"] for r in ps]))
+            solves[name]["all_converged"] = all(r["converged"] for r in ps)
         for name in ("mixed", "learned_mixed"):
-            if name in solves and "fp64_jacobi" in solves and solves[name]["ms"] > 0:
-                solves[f"speedup_{name}_vs_fp64"] = round(solves["fp64_jacobi"]["ms"] / solves[name]["ms"], 3)
+            if name in solves and "fp64_jacobi" in solves:
+                solves[f"speedup_{name}_vs_fp64"] = round(solves["fp64_jacobi"]["median_ms"] /
+                                                          solves[name]["median_ms"], 3)
         extra["ppe_time_to_1e-8"] = solves
+        if "xstar_quadratic" in wanted:
+            # The reference's own manufactured pressure x* = x^2 + y^2
+            # (src/lsmps.cpp:29): a smooth right-hand side with a small norm.
+            p.assemble(re, 1 << 30, L.XSTAR_QUADRATIC, 0)
+            quad = {}
+            for name, method in methods:
+                st = p.solve(method, args.solve_max_iter, 1e-8, chunk=32)
+                log(f"x* = x^2 + y^2 solve {name}: {st}")
+                quad[name] = {"ms": round(st["ms"], 3), "iterations": st["iterations"], "converged": st["converged"],
+                              "final_residual": st["final_residual"], "breakdown_restarts": st["breakdown_restarts"],
+                              "fallback": st["fallback"]}
+            extra["ppe_xstar_quadratic"] = quad
         if "wls_rows" in wanted:
             # The WLS-Laplacian rows on the same particles (steady state).
             p.assemble_wls(re, l0, 1 << 30, L.XSTAR_RANDOM, 11)
@@ -284,16 +391,16 @@ def main():
 
     out = {
         "metric": "CNL+WLS Mparticles/s (PPE time-to-1e-8 residual, mixed vs plain FP64, in ppe_time_to_1e-8)",
-        "value": round(n / wall_ms / 1e3, 3),
+        "value": round(n / wall32 / 1e3, 3),
         "unit": "Mparticles/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall_ms, 4),
+        "ms_per_step": round(wall32, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 moments, f64 positions, f64 redo of flagged rows",
+        "dtype": "f32 moments (f64 positions, f64 redo of the rows the reference regularises); value_fp64: f64",
         "data": "synthetic",
         "config": {"workload": f"industrial_mix {n} particles: 70% jittered bulk, 20% tilted wall film 2-4 l0, "
                                f"10% Poisson-disk splash at 0.2/l0^3; l0=0.004, re=2.1 l0, max_neighbors=128; "
@@ -303,6 +410,8 @@ def main():
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, n),
                      "kernel": dom, "kernel_ms": round(avg[dom], 4), "algorithmic_bytes": int(dom_bytes)},
     }
+    if "aggregate" in extra:
+        out["roofline"]["aggregate"] = extra.pop("aggregate")
     out.update(extra)
     if not args.no_cpu:
         try:

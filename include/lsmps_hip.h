@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LSMPS_HIP_ABI_VERSION 4
+#define LSMPS_HIP_ABI_VERSION 5
 
 enum {
   LSMPS_OK = 0,
@@ -186,6 +186,8 @@ typedef struct {
   int32_t fallback;     /* LEARNED: 1 when the Jacobi path was used */
   double initial_residual, final_residual, relative_l2_error, max_error;
   float ms;             /* GPU time from setup to the final residual */
+  int32_t breakdown_restarts; /* JACOBI, MIXED, LEARNED: restarts from the true residual
+                                 after a rho, omega or r0.v breakdown (at most 32) */
 } lsmps_solve_result;
 int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_result* out);
 int lsmps_upload_x0(lsmps_ctx* ctx, const double* x0, uint64_t n);

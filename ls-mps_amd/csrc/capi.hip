@@ -50,10 +50,15 @@ struct DBuf {
   }
 };
 
+// A captured chunk of solver iterations. Its kernel nodes hold raw device
+// pointers, so a cached graph is replayed only when every buffer those
+// kernels read or write (and every size they were launched with) is the
+// same as at capture.
+constexpr int GRAPH_KEY = 12;
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
-  const void* key[6] = {};
+  const void* key[GRAPH_KEY] = {};
   int m = -1, chunk = 0, with_replace = -1, fp32 = -1;
   void reset() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -95,7 +100,17 @@ struct lsmps_ctx {
   bool have_cnl = false;
   bool replay = false;  // the neighbor table holds a replay graph (no positions)
   uint32_t replay_seed = 0;
-  double re = 0.0;
+  double re = 0.0;    // list radius of the last build (re + skin)
+  double skin = 0.0;  // skin of the last build
+  // Motion since the last build: moved is the sum over lsmps_advance calls
+  // of umax |dt|, a bound on every particle's displacement (umax bounds
+  // every particle's speed). The lists hold every pair within re + skin at
+  // the build, so they still hold every pair within re while two particles
+  // have closed in by less than skin, i.e. while 2 moved < skin.
+  double umax = 0.0, moved = 0.0;
+  // A build with skin > 0 followed by another build with no upload, append
+  // or removal in between is refused (see build_cell_list).
+  bool after_skin_build = false;
   int maxn = 0, gx = 0, gy = 0, gz = 0;
   uint64_t pairs = 0;
   DBuf<int> cx, cy, cz, box, cell, slot, skey, ccount, cstart, perm, iperm, sflag, nbr, cnt, cnt_orig, row_ptr, col, hist;
@@ -145,7 +160,12 @@ struct lsmps_ctx {
   int* host_small = nullptr;         // pinned scratch for small readbacks (64 ints)
   unsigned long long* host_rb = nullptr;  // pinned: 16 counters, then up to 512 histogram bins
   bool have_x0 = false, have_solution = false;
-  GraphCache graph[2];  // captured iterations, one per solve precision (FP64, FP32)
+  // Captured solver iterations; a solve replays the entry whose key matches
+  // (precision, sizes, every buffer), else captures into the oldest slot.
+  static constexpr int N_GRAPHS = 4;
+  GraphCache graph[N_GRAPHS];
+  int graph_next = 0;
+  bool solver_trace = false;  // LSMPS_SOLVER_TRACE set at context creation: per-chunk solver state on stderr
 
   // learned preconditioner: features of the current PPE (row order u), the
   // network (FP32 master, BF16 copy, Adam moments) and the training set
@@ -163,8 +183,7 @@ struct lsmps_ctx {
   DBuf<int> rcol;
   DBuf<uint8_t> rflag, rtype;
   void reset_graphs() {
-    graph[0].reset();
-    graph[1].reset();
+    for (auto& g : graph) g.reset();
   }
 };
 
@@ -321,6 +340,29 @@ hipError_t read_counters(lsmps_ctx* ctx, unsigned long long* c, size_t k) {
   return e;
 }
 
+// Largest speed |u| over n host velocity triples.
+double max_speed(const double* uvw, uint64_t n) {
+  double m2 = 0.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const double a = uvw[3 * i], b = uvw[3 * i + 1], c = uvw[3 * i + 2];
+    const double q = a * a + b * b + c * c;
+    if (!(q <= m2)) m2 = q;  // a NaN speed propagates and makes every later check fail
+  }
+  return std::sqrt(m2);
+}
+
+// Stages that read pairs within re from the lists need the lists to still
+// hold every such pair: refused once the particles may have closed in by
+// the skin (any motion at all for lists built without one).
+int check_lists_current(lsmps_ctx* ctx, const char* stage) {
+  if (ctx->moved != 0.0 && !(2.0 * ctx->moved < ctx->skin))
+    return fail(ctx, LSMPS_E_STATE,
+                std::string(stage) + ": particles moved by up to " + std::to_string(ctx->moved) +
+                    " since the neighbor build, half the skin or more (" + std::to_string(ctx->skin) +
+                    "): rebuild the neighbor lists");
+  return LSMPS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -338,6 +380,8 @@ int lsmps_create(int device, lsmps_ctx** out) {
     return LSMPS_E_ARG;
   }
   ctx->device = device;
+  // Read once here: the solve loop itself makes no libc environment calls.
+  ctx->solver_trace = std::getenv("LSMPS_SOLVER_TRACE") != nullptr;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->e0) != hipSuccess || hipEventCreate(&ctx->e1) != hipSuccess ||
       hipEventCreate(&ctx->e2) != hipSuccess || hipEventCreate(&ctx->e3) != hipSuccess ||
@@ -458,6 +502,8 @@ int lsmps_upload_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   ctx->have_field = false;
   ctx->spos_field = false;
   ctx->spec_ok = false;  // a new particle set gets an exact grid first
+  ctx->umax = ctx->moved = 0.0;
+  ctx->after_skin_build = false;
   ctx->reset_graphs();
   return LSMPS_OK;
   GUARD_END
@@ -547,6 +593,14 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   if (!(re > 0) || !std::isfinite(re)) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
   if (!(skin >= 0) || !std::isfinite(skin)) return fail(ctx, LSMPS_E_ARG, "skin must be finite and >= 0");
   if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
+  // A neighbor build that directly follows a skin build of the same
+  // particle set (no upload, append or removal in between) went silent on
+  // the GPU in testing and its cause is not established, so that sequence is
+  // refused. The step timeline rebuilds only after injections and removals.
+  if (ctx->after_skin_build)
+    return fail(ctx, LSMPS_E_STATE,
+                "a neighbor build directly after a skin build of the same particles is not supported: upload, "
+                "append or remove particles first");
   const double rl = re + skin;  // list radius and cell size
   // Lists above 256 entries would take a 64-thread variant of the neighbor
   // pass that has not been validated on the GPU; it stays switched off.
@@ -599,8 +653,15 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   }
   const int64_t ncells = g[0] * g[1] * g[2];
   // Dense grid: bounded so a sparse outlier cannot blow up memory.
-  if (ncells > ((int64_t)1 << 30) || ncells > 64 * (int64_t)n + ((int64_t)1 << 24))
+  if (ncells > ((int64_t)1 << 30) || ncells > 64 * (int64_t)n + ((int64_t)1 << 24)) {
+    if (spec) {
+      // The kept grid was sized for an earlier (larger) particle count: the
+      // exact box may still fit.
+      ctx->spec_ok = false;
+      return build_cell_list(ctx, re, skin, maxn, classify, l0, out, tout);
+    }
     return fail(ctx, LSMPS_E_LIMIT, "particle bounding box spans too many cells for the dense grid");
+  }
   ctx->gx = (int)g[0];
   ctx->gy = (int)g[1];
   ctx->gz = (int)g[2];
@@ -656,6 +717,9 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   cs.ms = elapsed(ctx);
   cs.kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
   ctx->re = rl;
+  ctx->skin = skin;
+  ctx->moved = 0.0;
+  ctx->after_skin_build = skin > 0.0;
   ctx->replay = false;
   ctx->n_cnl = (uint64_t)n;
   ctx->maxn = maxn;
@@ -707,6 +771,7 @@ int lsmps_set_velocities(lsmps_ctx* ctx, const double* uvw, uint64_t n) {
   HIPCHK(ctx->u.ensure(3 * n));
   if (n) HIPCHK(hipMemcpy(ctx->u.p, uvw, 3 * n * sizeof(double), hipMemcpyHostToDevice));
   ctx->have_u = true;
+  ctx->umax = max_speed(uvw, n);
   return LSMPS_OK;
   GUARD_END
 }
@@ -737,7 +802,12 @@ int lsmps_append_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   }
   HIPCHK(hipStreamSynchronize(st));
   ctx->n = n1;
-  if (uvw) ctx->have_u = true;
+  if (uvw) {
+    ctx->have_u = true;
+    const double um = max_speed(uvw, count);
+    if (!(um <= ctx->umax)) ctx->umax = um;
+  }
+  if (count) ctx->after_skin_build = false;
   // A field given as values has none for the new particles.
   if (count) {
     ctx->have_field = ctx->have_field && ctx->field_quadratic;
@@ -759,6 +829,8 @@ int lsmps_advance(lsmps_ctx* ctx, double dt) {
                        ctx->z.p, ctx->u.p, n, ctx->spos.p, ctx->perm.p, ns, dt);
     if (int rc = check_launch(ctx, "advance")) return rc;
   }
+  // Bound on the displacement since the last build.
+  ctx->moved += ctx->umax * std::fabs(dt);
   // Fields and everything derived from positions are out of date.
   ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_features = false;
   ctx->have_field = ctx->have_field && ctx->field_quadratic;
@@ -809,6 +881,7 @@ int lsmps_remove_outside(lsmps_ctx* ctx, const double* lo, const double* hi, uin
     HIPCHK(hipStreamSynchronize(st));
     ctx->n = (uint64_t)m;
     ctx->n_cnl = 0;
+    ctx->after_skin_build = false;
     ctx->have_cnl = ctx->have_wls = ctx->have_ppe = ctx->have_solution = ctx->have_features = false;
     ctx->have_field = ctx->have_field && ctx->field_quadratic;
   ctx->spos_field = false;
@@ -822,6 +895,8 @@ int lsmps_classify(lsmps_ctx* ctx, double l0, lsmps_type_stats* out) {
   if (!ctx) return LSMPS_E_ARG;
   if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "classify needs neighbors");
   if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
+  if (ctx->skin != 0.0) return fail(ctx, LSMPS_E_STATE, "stand-alone classification needs lists without a skin");
+  if (int rc = check_lists_current(ctx, "classify")) return rc;
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->st;
@@ -878,7 +953,10 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   if (!ctx->have_field) return fail(ctx, LSMPS_E_STATE, "WLS needs a field (lsmps_set_field_*)");
   if (p->precision != LSMPS_WLS_FP64 && p->precision != LSMPS_WLS_MIXED)
     return fail(ctx, LSMPS_E_ARG, "unknown WLS precision");
-  if (!(p->re > 0) || !(p->l0 > 0)) return fail(ctx, LSMPS_E_ARG, "re and l0 must be positive");
+  if (!(p->re > 0) || !(p->l0 > 0) || !std::isfinite(p->re) || !std::isfinite(p->l0))
+    return fail(ctx, LSMPS_E_ARG, "re and l0 must be positive and finite");
+  if (p->re > ctx->re) return fail(ctx, LSMPS_E_ARG, "re exceeds the radius of the neighbor lists");
+  if (int rc = check_lists_current(ctx, "WLS")) return rc;
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   const int n = (int)ctx->n_cnl;
@@ -1001,6 +1079,8 @@ int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0,
   if (replay && p->xmode != LSMPS_XSTAR_RANDOM) return fail(ctx, LSMPS_E_ARG, "a replay graph has no positions: x* must be random");
   if (!replay && (!(p->re > 0) || !std::isfinite(p->re))) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
   if (!replay && p->re > ctx->re) return fail(ctx, LSMPS_E_ARG, "re exceeds the radius of the neighbor lists");
+  if (!replay)
+    if (int rc = check_lists_current(ctx, "PPE assembly")) return rc;
   if (replay && (!(l0 > 0) || !std::isfinite(l0))) return fail(ctx, LSMPS_E_ARG, "replay weights need l0 > 0");
   if (wls && (!(l0 > 0) || !std::isfinite(l0) || !(reg >= 0) || !std::isfinite(reg)))
     return fail(ctx, LSMPS_E_ARG, "WLS rows need l0 > 0 and reg >= 0, finite");
@@ -1073,7 +1153,6 @@ int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0,
   ctx->nnz = m > 0 ? (uint64_t)c[2] : 0;
   ctx->have_ppe = true;
   ctx->have_solution = ctx->have_x0 = ctx->have_features = false;
-  ctx->reset_graphs();
   if (out) {
     out->unknowns = (uint64_t)m;
     out->nnz = ctx->nnz;
@@ -1168,6 +1247,8 @@ int lsmps_replay_load(lsmps_ctx* ctx, const int64_t* row_ptr, const int32_t* col
   ctx->n = ctx->n_cnl = n;
   ctx->replay = true;
   ctx->re = 0.0;
+  ctx->skin = ctx->moved = ctx->umax = 0.0;
+  ctx->after_skin_build = false;
   ctx->maxn = maxn;
   ctx->pairs = (uint64_t)nnz;
   ctx->replay_seed = seed;
@@ -1310,10 +1391,21 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
   if (int rc = check_launch(ctx, "solver_init")) return rc;
   // Capture `chunk` iterations once per buffer set; replay until the device
   // state leaves RUNNING (kernels past convergence return at once).
-  GraphCache& G = ctx->graph[sizeof(T) == 4 ? 1 : 0];
-  const void* key[6] = {B.val, B.r, B.y, B.scale, B.sptr, B.val64};
-  const bool hit = G.exec && G.m == B.m && G.chunk == chunk && G.with_replace == (int)with_replace &&
-                   G.fp32 == (int)(sizeof(T) == 4) && std::equal(key, key + 6, G.key);
+  // Every pointer the captured launches take (launch_solver_iteration): the
+  // matrix, the vectors (one block, B.r first), the partials and scalars.
+  const void* key[GRAPH_KEY] = {B.sptr, B.col, B.val, B.val64, B.b, B.scale, B.y, B.r, B.partials, ctx->sc.p,
+                                reinterpret_cast<const void*>((uintptr_t)ctx->sell_nnz), nullptr};
+  GraphCache* hitp = nullptr;
+  for (auto& c : ctx->graph)
+    if (c.exec && c.m == B.m && c.chunk == chunk && c.with_replace == (int)with_replace &&
+        c.fp32 == (int)(sizeof(T) == 4) && std::equal(key, key + GRAPH_KEY, c.key))
+      hitp = &c;
+  const bool hit = hitp != nullptr;
+  if (!hit) {
+    hitp = &ctx->graph[ctx->graph_next];
+    ctx->graph_next = (ctx->graph_next + 1) % lsmps_ctx::N_GRAPHS;
+  }
+  GraphCache& G = *hitp;
   if (!hit) {
     G.reset();
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1325,17 +1417,16 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
     G.chunk = chunk;
     G.with_replace = with_replace;
     G.fp32 = sizeof(T) == 4;
-    std::copy(key, key + 6, G.key);
+    std::copy(key, key + GRAPH_KEY, G.key);
   }
   const int max_launch = max_iter / chunk + 3;
   HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  const bool trace = std::getenv("LSMPS_SOLVER_TRACE") != nullptr;
   for (int l = 0; l < max_launch && ctx->host_sc->state == SOLVE_RUNNING; ++l) {
     HIPCHK(hipGraphLaunch(G.exec, st));
     HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    if (trace) {
+    if (ctx->solver_trace) {
       const SolverScalars& s = *ctx->host_sc;
       std::fprintf(stderr, "[solver] it=%d state=%d rn=%.6e rn/thr=%.3e rho=%.6e alpha=%.6e omega=%.6e beta=%.6e repl=%d\n",
                    s.it, s.state, s.rn, s.rn / s.thresh, s.rho, s.alpha, s.omega, s.beta, s.n_replace);
@@ -1351,6 +1442,7 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
   out->converged = s.state == SOLVE_CONVERGED ? 1 : 0;
   out->state = s.state;
   out->replacements = s.n_replace;
+  out->breakdown_restarts = s.n_break;
   out->initial_residual = s.init_res;
   out->final_residual = s.final_res;
   out->relative_l2_error = s.rel_l2;
@@ -1395,7 +1487,7 @@ int solve_fp64_jacobi(lsmps_ctx* ctx, const double* x0, int max_iter, double tol
   B.y = ctx->yv.p;
   B.partials = ctx->partials.p;
   bind_vectors(B, ctx->vec64.p, m);
-  return run_bicgstab<double>(ctx, B, x0, max_iter, tol, SOLVE_TRUE_CHECK, chunk, out);
+  return run_bicgstab<double>(ctx, B, x0, max_iter, tol, SOLVE_TRUE_CHECK | SOLVE_BREAK_RESTART, chunk, out);
 }
 
 // Mixed-precision BiCGStab with scaling `scale` from x0 (or zero). With
@@ -1420,7 +1512,7 @@ int solve_mixed(lsmps_ctx* ctx, const double* scale, const double* x0, bool prec
   B.y = ctx->yv.p;
   B.partials = ctx->partials.p;
   bind_vectors(B, ctx->vec64.p, m);
-  const int flags = SOLVE_RELIABLE | SOLVE_TRUE_CHECK | SOLVE_RESTART;
+  const int flags = SOLVE_RELIABLE | SOLVE_TRUE_CHECK | SOLVE_RESTART | SOLVE_BREAK_RESTART;
   if (precheck && x0) {
     launch_solver_init<float>(B, x0, ctx->sc.p, max_iter, tol, flags, RELIABLE_DELTA, st);
     if (int rc = check_launch(ctx, "solver_init")) return rc;
@@ -1445,6 +1537,7 @@ int solve_mixed(lsmps_ctx* ctx, const double* scale, const double* x0, bool prec
     rc = solve_fp64_jacobi(ctx, resume ? ctx->x0.p : nullptr, max_iter, tol, chunk, out);
     out->iterations += first.iterations;
     out->replacements += first.replacements;
+    out->breakdown_restarts += first.breakdown_restarts;
     out->initial_residual = first.initial_residual;
     out->fallback = 1;
   }

@@ -58,8 +58,6 @@
 namespace lsmps_hip {
 
 constexpr int PPE_SLICE = 64;
-constexpr int PPE_GROUP = 8;
-constexpr int PPE_THREADS = PPE_SLICE * PPE_GROUP;  // one slice per block
 
 __device__ __forceinline__ bool is_unknown_type(uint8_t t) { return t == T_INTERIOR || t == T_NEARWALL; }
 
@@ -130,18 +128,29 @@ __device__ __forceinline__ double replay_weight(uint32_t seed, int lo, int hi, d
   return (0.5 + unit53(mix64(key))) / (l0 * l0);
 }
 
-// Slice-local row of this thread's group and the slice index.
+// One thread per row: a slice of 64 rows is one wavefront, so the slice's
+// width is a wavefront maximum, a row's entries at position k of the slice
+// are 64 consecutive words (coalesced stores), and the list-order sums of
+// the diagonal and of b are plain sequential sums in the row's thread.
+// Blocks of PPE_THREADS threads hold PPE_THREADS / 64 slices; the block index
+// is remapped so each XCD takes a contiguous range of slices.
+constexpr int PPE_THREADS = 256;
+constexpr int PPE_SLICES_PER_BLOCK = PPE_THREADS / PPE_SLICE;
+// Neighbor entries each thread has in flight (indices, then their unknown
+// ids and positions) before it consumes them in list order.
+constexpr int PPE_BATCH = 4;
+
 struct PpeRow {
-  int slice, u, g, gbase;
-  bool live;
+  int slice, u, lane;
+  bool live, slice_live;
 };
 __device__ __forceinline__ PpeRow ppe_row(int m) {
   PpeRow R;
-  R.slice = (int)xcd_block(blockIdx.x, gridDim.x);
-  R.g = threadIdx.x & (PPE_GROUP - 1);
-  R.gbase = threadIdx.x & ~(PPE_GROUP - 1) & (WAVE - 1);
-  R.u = R.slice * PPE_SLICE + (int)(threadIdx.x / PPE_GROUP);
+  R.lane = threadIdx.x & (PPE_SLICE - 1);
+  R.slice = (int)xcd_block(blockIdx.x, gridDim.x) * PPE_SLICES_PER_BLOCK + (int)(threadIdx.x / PPE_SLICE);
+  R.u = R.slice * PPE_SLICE + R.lane;
   R.live = R.u < m;
+  R.slice_live = R.slice * PPE_SLICE < m;
   return R;
 }
 
@@ -152,27 +161,32 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_count(const int* __restrict
                                                            const int* __restrict__ uid, const int* __restrict__ nbr,
                                                            const int* __restrict__ ncnt, int maxn, int m, int pin0,
                                                            int* __restrict__ wsz) {
-  __shared__ int wmax;
-  if (threadIdx.x == 0) wmax = 0;
-  __syncthreads();
   const PpeRow R = ppe_row(m);
+  if (!R.slice_live) return;  // whole wavefronts past the last slice
   int c = 0;
   if (R.live && (!pin0 || urr[R.u] != 0)) {
     const int s = uslot[R.u];
     const int* row = nbr + (size_t)s * maxn;
     const int k = ncnt[s];
-    for (int q = R.g; q < k; q += PPE_GROUP) c += uid[row[q]] >= 0 ? 1 : 0;
+    for (int q0 = 0; q0 < k; q0 += PPE_BATCH) {
+      int s2[PPE_BATCH];
+#pragma unroll
+      for (int j = 0; j < PPE_BATCH; ++j) s2[j] = q0 + j < k ? row[q0 + j] : -1;
+#pragma unroll
+      for (int j = 0; j < PPE_BATCH; ++j) c += (s2[j] >= 0 && uid[s2[j]] >= 0) ? 1 : 0;
+    }
   }
-  c = group8_sum_int(c);
-  if (R.live && R.g == 0) atomicMax(&wmax, c + 1);
-  __syncthreads();
-  if (threadIdx.x == 0) wsz[R.slice] = wmax * PPE_SLICE;
+  const int w = wave_max(R.live ? c + 1 : 0);
+  if (R.lane == 0) wsz[R.slice] = w * PPE_SLICE;
 }
 
 // The fill pass. Outputs in row order u: the sliced-ELL columns (sorted
 // numbering) and FP64 values, the row lengths, the diagonal, x* and b.
 // Counts per block into bpart (then counters): [0] failed rows, [1] NearWall
 // rows except row 0, [2] nonzeros.
+// Columns come out ascending: the list is ascending in sorted slots and the
+// row numbering u is monotone in the slot, so the diagonal takes the
+// position where the first column above u would go.
 // REPLAY: the edges of a replay graph (slots are particle indices) with
 // replay_weight(wseed, ...) in place of w/r^2 from positions (re unused);
 // self-edges are skipped as the reference skips r <= 1e-12.
@@ -186,102 +200,92 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
     unsigned* __restrict__ bpart) {
   __shared__ unsigned lds[3 * PPE_THREADS / WAVE];
   const PpeRow R = ppe_row(m);
-  const unsigned below = (1u << R.g) - 1u;
   int failed = 0, neumann = 0, len = 0;
-  if (R.live) {
-    const int u = R.u;
-    const int s = uslot[u];
-    const int rr = urr[u];
-    const double4 pi = spos[s];
+  if (R.slice_live) {
     const int sb = sptr[R.slice];
     const int width = (sptr[R.slice + 1] - sb) / PPE_SLICE;
-    const int base = sb + (u - R.slice * PPE_SLICE);
-    const double xu = xmode == 0 ? pi.x * pi.x + pi.y * pi.y  // x* = x^2 + y^2 (src/lsmps.cpp:29)
-                                 : xstar_random(xseed, rr);
-    double d = 0.0, bs = 0.0;
-    int kept = 0, below_u = 0;
-    if (rr != 0) {
-      const int* row = nbr + (size_t)s * maxn;
-      const int k = ncnt[s];
-      for (int c0 = 0; c0 < k; c0 += PPE_GROUP) {
-        const int q = c0 + R.g;
-        int cj = -1;
-        double a = 0.0, xj = 0.0;
-        bool keep = false;
-        if (q < k) {
-          const int s2 = row[q];
-          cj = uid[s2];
-          if (cj >= 0) {
+    int* cw = col + sb + R.lane;
+    double* vw = val + sb + R.lane;
+    if (R.live) {
+      const int u = R.u;
+      const int s = uslot[u];
+      const int rr = urr[u];
+      const double4 pi = spos[s];
+      const double xu = xmode == 0 ? pi.x * pi.x + pi.y * pi.y  // x* = x^2 + y^2 (src/lsmps.cpp:29)
+                                   : xstar_random(xseed, rr);
+      double d = 0.0, bs = 0.0;
+      int at = 0, dpos = -1;
+      if (rr != 0) {
+        const int* row = nbr + (size_t)s * maxn;
+        const int k = ncnt[s];
+        for (int q0 = 0; q0 < k; q0 += PPE_BATCH) {
+          int s2[PPE_BATCH], cj[PPE_BATCH];
+          double4 pj[PPE_BATCH];
+#pragma unroll
+          for (int j = 0; j < PPE_BATCH; ++j) s2[j] = q0 + j < k ? row[q0 + j] : -1;
+#pragma unroll
+          for (int j = 0; j < PPE_BATCH; ++j) cj[j] = s2[j] >= 0 ? uid[s2[j]] : -1;
+          if (!REPLAY) {
+#pragma unroll
+            for (int j = 0; j < PPE_BATCH; ++j) pj[j] = cj[j] >= 0 ? spos[s2[j]] : pi;
+          }
+#pragma unroll
+          for (int j = 0; j < PPE_BATCH; ++j) {
+            if (cj[j] < 0) continue;
+            double a = 0.0, xj = 0.0;
+            bool keep;
             if (REPLAY) {
-              keep = s2 != s;
+              keep = s2[j] != s;
               if (keep) {
-                a = replay_weight(wseed, s < s2 ? s : s2, s < s2 ? s2 : s, l0);
-                xj = xstar_random(xseed, urr[cj]);
+                a = replay_weight(wseed, s < s2[j] ? s : s2[j], s < s2[j] ? s2[j] : s, l0);
+                xj = xstar_random(xseed, urr[cj[j]]);
               }
             } else {
-              const double4 pj = spos[s2];
-              keep = pair_coef(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, re, &a);
-              if (keep) xj = xmode == 0 ? pj.x * pj.x + pj.y * pj.y : xstar_random(xseed, urr[cj]);
+              keep = pair_coef(pj[j].x - pi.x, pj[j].y - pi.y, pj[j].z - pi.z, re, &a);
+              if (keep) xj = xmode == 0 ? pj[j].x * pj[j].x + pj[j].y * pj[j].y : xstar_random(xseed, urr[cj[j]]);
             }
+            if (!keep) continue;
+            if (dpos < 0 && cj[j] > u) dpos = at++;
+            cw[at * PPE_SLICE] = cj[j];
+            vw[at * PPE_SLICE] = -a;
+            ++at;
+            // Sums in list order, as the reference's CSR order gives them.
+            d += a;
+            bs += -a * xj;
           }
         }
-        const unsigned gm = group8_ballot(keep, R.gbase);
-        const unsigned lt = group8_ballot(keep && cj < u, R.gbase);
-        if (keep) {
-          // Entries with a column above u sit one slot later: the diagonal
-          // takes slot below_u + popc(lt).
-          const int slot = kept + __popc(gm & below) + (cj > u ? 1 : 0);
-          col[base + slot * PPE_SLICE] = cj;
-          val[base + slot * PPE_SLICE] = -a;
+        if (d <= 0 || !isfinite(d)) {
+          failed = 1;
+          d = 1.0;
         }
-        // Sums in list order: each kept entry is broadcast to the group in turn.
-        for (int j = 0; j < PPE_GROUP; ++j) {
-          if ((gm >> j) & 1u) {
-            const double aj = __shfl(a, R.gbase + j);
-            const double xjj = __shfl(xj, R.gbase + j);
-            d += aj;
-            bs += -aj * xjj;
-          }
-        }
-        kept += __popc(gm);
-        below_u += __popc(lt);
+        if (shift != 0.0) d += shift * d;
+        neumann = type_s[s] == T_NEARWALL ? 1 : 0;
+      } else {
+        d = 1.0;  // the identity pin of row 0
       }
-      if (d <= 0 || !isfinite(d)) {
-        failed = 1;
-        d = 1.0;
-      }
-      if (shift != 0.0) d += shift * d;
-      neumann = type_s[s] == T_NEARWALL ? 1 : 0;
-    } else {
-      d = 1.0;  // the identity pin of row 0
-    }
-    bs += d * xu;
-    len = kept + 1;
-    if (R.g == 0) {
-      col[base + below_u * PPE_SLICE] = u;
-      val[base + below_u * PPE_SLICE] = d;
+      if (dpos < 0) dpos = at++;
+      bs += d * xu;
+      len = at;
+      cw[dpos * PPE_SLICE] = u;
+      vw[dpos * PPE_SLICE] = d;
       rlen[u] = len;
       diag[u] = d;
       xe[u] = xu;
       b[u] = bs;
-    }
-    for (int k2 = len + R.g; k2 < width; k2 += PPE_GROUP) {
-      col[base + k2 * PPE_SLICE] = u;
-      val[base + k2 * PPE_SLICE] = 0.0;
-    }
-  } else {
-    // Rows past m in the last slice: every stored entry is defined, as
-    // (0, 0), so passes over all stored entries stay in bounds.
-    const int sb = sptr[R.slice];
-    const int width = (sptr[R.slice + 1] - sb) / PPE_SLICE;
-    const int base = sb + (R.u - R.slice * PPE_SLICE);
-    for (int k2 = R.g; k2 < width; k2 += PPE_GROUP) {
-      col[base + k2 * PPE_SLICE] = 0;
-      val[base + k2 * PPE_SLICE] = 0.0;
+      for (int k2 = len; k2 < width; ++k2) {
+        cw[k2 * PPE_SLICE] = u;
+        vw[k2 * PPE_SLICE] = 0.0;
+      }
+    } else {
+      // Rows past m in the last slice: every stored entry is defined, as
+      // (0, 0), so passes over all stored entries stay in bounds.
+      for (int k2 = 0; k2 < width; ++k2) {
+        cw[k2 * PPE_SLICE] = 0;
+        vw[k2 * PPE_SLICE] = 0.0;
+      }
     }
   }
-  const bool leader = R.live && R.g == 0;
-  unsigned v[3] = {leader ? (unsigned)failed : 0u, leader ? (unsigned)neumann : 0u, leader ? (unsigned)len : 0u};
+  unsigned v[3] = {(unsigned)failed, (unsigned)neumann, (unsigned)len};
   block_count_partials<3>(v, bpart, lds);
 }
 
@@ -518,6 +522,7 @@ __global__ __launch_bounds__(256) void k_from_rr(const int* __restrict__ urr, co
 static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 int ppe_slices(int m) { return (m + PPE_SLICE - 1) / PPE_SLICE; }
+static int ppe_fill_blocks(int m) { return (ppe_slices(m) + PPE_SLICES_PER_BLOCK - 1) / PPE_SLICES_PER_BLOCK; }
 
 void launch_unknown_flags(const uint8_t* type_s, const int* iperm, int n, int* flag, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_unknown_flags, dim3(blocks(n, 256)), dim3(256), 0, st, type_s, iperm, n, flag);
@@ -534,8 +539,8 @@ void launch_unknown_maps(const int* perm, const int* fs, const int* upos, const 
 void launch_ppe_count(const int* uslot, const int* urr, const int* uid, const int* nbr, const int* ncnt, int maxn,
                       int m, bool pin0, int* wsz, hipStream_t st) {
   if (m > 0)
-    hipLaunchKernelGGL(k_ppe_count, dim3(ppe_slices(m)), dim3(PPE_THREADS), 0, st, uslot, urr, uid, nbr, ncnt, maxn, m,
-                       pin0 ? 1 : 0, wsz);
+    hipLaunchKernelGGL(k_ppe_count, dim3(ppe_fill_blocks(m)), dim3(PPE_THREADS), 0, st, uslot, urr, uid, nbr, ncnt,
+                       maxn, m, pin0 ? 1 : 0, wsz);
 }
 void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr, const int* uid,
                      const int* nbr, const int* ncnt, int maxn, int m, double re, double shift, int xmode,
@@ -543,7 +548,7 @@ void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslo
                      double* b, unsigned long long* counters, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin,
                      hipEvent_t ev_end, bool replay, uint32_t wseed, double l0) {
   if (m <= 0) return;
-  const int g = ppe_slices(m);
+  const int g = ppe_fill_blocks(m);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
   if (replay)
     hipLaunchKernelGGL(k_ppe_fill<true>, dim3(g), dim3(PPE_THREADS), 0, st, spos, type_s, uslot, urr, uid, nbr, ncnt,

@@ -73,6 +73,13 @@ __device__ __forceinline__ bool iter_active(const SolverScalars* sc) {
   return load_state(sc) == SOLVE_RUNNING && sc->it < sc->max_iter;
 }
 
+// A breakdown that the solve may answer with a restart from the true
+// residual of its iterate (SOLVE_BREAK_RESTART, a bounded number of times);
+// every block reads the same n_break (only k_fin_residual changes it).
+__device__ __forceinline__ bool can_restart(const SolverScalars* sc) {
+  return (sc->flags & SOLVE_BREAK_RESTART) && sc->n_break < SOLVER_MAX_BREAK_RESTARTS;
+}
+
 // After a step that leaves r updated: decide stop/continue and next beta.
 // Runs in one thread of the last block.
 __device__ void finish_iteration(SolverScalars* sc, double rr, double r0r) {
@@ -94,7 +101,8 @@ __device__ void finish_iteration(SolverScalars* sc, double rr, double r0r) {
     return;
   }
   if (fabs(sc->omega) < 1e-30) {
-    sc->state = SOLVE_BREAKDOWN;
+    if (can_restart(sc)) sc->replace = 3;
+    else sc->state = SOLVE_BREAKDOWN;
     return;
   }
   if ((sc->flags & SOLVE_RELIABLE) && rn < sc->delta * sc->rn_max) {
@@ -106,7 +114,8 @@ __device__ void finish_iteration(SolverScalars* sc, double rr, double r0r) {
   sc->rho_prev = sc->rho;
   sc->rho = r0r;
   if (fabs(r0r) < 1e-30) {
-    sc->state = SOLVE_BREAKDOWN;
+    if (can_restart(sc)) sc->replace = 3;
+    else sc->state = SOLVE_BREAKDOWN;
     return;
   }
   sc->beta = (sc->rho / sc->rho_prev) * (sc->alpha / sc->omega);
@@ -180,8 +189,16 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_s(int m, const double* 
   const bool broke = fabs(den) < 1e-30 || !isfinite(den);
   const double alpha = broke ? 0.0 : sc->rho / den;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (broke) store_state(sc, SOLVE_BREAKDOWN);
-    else sc->alpha = alpha;
+    if (broke && can_restart(sc)) {
+      // The rest of this iteration is skipped; the true-residual launch
+      // that follows restarts the Krylov space from the iterate.
+      sc->replace = 3;
+      store_state(sc, SOLVE_RESTARTING);
+    } else if (broke) {
+      store_state(sc, SOLVE_BREAKDOWN);
+    } else {
+      sc->alpha = alpha;
+    }
   }
   if (broke) return;
   for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS)
@@ -230,11 +247,18 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const double*
   const bool sconv = (sc->flags & SOLVE_S_EXIT) && sn <= sc->thresh;
   const bool broke = !sconv && (tot[1] < 1e-30 || !isfinite(tot[1]) || !isfinite(tot[2]));
   const double omega = sconv || broke ? 0.0 : tot[2] / tot[1];
+  // On a breakdown every block returns without touching y (a block that
+  // starts after block 0 stored the state returns at entry just the same);
+  // with SOLVE_BREAK_RESTART the true-residual launch then restarts.
+  const bool restart = broke && can_restart(sc);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->sn = sn;
     if (sconv) {
       store_state(sc, SOLVE_S_CONVERGED);
       sc->iterations = sc->it + 1;
+    } else if (restart) {
+      sc->replace = 3;
+      store_state(sc, SOLVE_RESTARTING);
     } else if (broke) {
       store_state(sc, SOLVE_BREAKDOWN);
     } else {
@@ -288,7 +312,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_true_residual(
     double* __restrict__ r, double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v,
     SolverScalars* sc, double* __restrict__ partials) {
   if (!INIT && sc->replace == 0) return;
-  const bool restart = !INIT && (sc->flags & SOLVE_RESTART);
+  const bool restart = !INIT && ((sc->flags & SOLVE_RESTART) || sc->replace == 3);
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
   double rr = 0.0, r0r = 0.0, bb = 0.0;
   for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
@@ -327,7 +351,7 @@ template <bool INIT>
 __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_residual(const double* __restrict__ partials, int nblk,
                                                                  SolverScalars* sc) {
   if (!INIT && sc->replace == 0) return;
-  const bool restart = !INIT && (sc->flags & SOLVE_RESTART);
+  const bool restart = !INIT && ((sc->flags & SOLVE_RESTART) || sc->replace == 3);
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
   double tot[3];
   finalize_sum<3>(partials, nblk, tot, lds);
@@ -343,6 +367,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_residual(const double* _
       sc->it = sc->iterations = 0;
       sc->replace = 0;
       sc->n_replace = 0;
+      sc->n_break = 0;
       sc->state = SOLVE_RUNNING;
       if (sc->init_res <= sc->thresh && (sc->flags & SOLVE_TRUE_CHECK)) sc->state = SOLVE_CONVERGED;
       else if (fabs(sc->rho) < 1e-30) sc->state = SOLVE_BREAKDOWN;
@@ -352,13 +377,18 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_residual(const double* _
       const int kind = sc->replace;
       sc->replace = 0;
       sc->n_replace += 1;
+      if (kind == 3) {
+        sc->n_break += 1;
+        sc->state = SOLVE_RUNNING;  // was RESTARTING when the breakdown came mid-iteration
+      }
       sc->rn = rn;
       sc->rn_max = rn;
       if (!isfinite(rn) || !isfinite(tot[1])) {
         sc->state = SOLVE_BREAKDOWN;
         return;
       }
-      if (kind == 2 && rn <= sc->thresh) {
+      // The true residual decides convergence whenever it is computed.
+      if ((kind == 2 || (sc->flags & SOLVE_TRUE_CHECK)) && rn <= sc->thresh) {
         sc->state = SOLVE_CONVERGED;
         return;
       }

@@ -12,13 +12,18 @@ enum : int {
   SOLVE_CONVERGED = 2,
   SOLVE_BREAKDOWN = 3,
   SOLVE_MAXITER = 4,
+  SOLVE_RESTARTING = 5,   // a breakdown inside this iteration: the rest of it is skipped, the restart follows
 };
 enum : int {
   SOLVE_S_EXIT = 1,      // stop on ||s|| as the reference does
   SOLVE_RELIABLE = 2,    // replace the recurrence residual by b - A x (FP64) on 10x drops
   SOLVE_TRUE_CHECK = 4,  // confirm convergence with the true FP64 residual
   SOLVE_RESTART = 8,     // restart the Krylov space (r0 = r, p = v = 0) at each replacement
+  SOLVE_BREAK_RESTART = 16,  // on a rho, omega or r0.v breakdown, restart from the true residual of the iterate
 };
+// Breakdown restarts allowed per solve (SOLVE_BREAK_RESTART) before the
+// solve ends in SOLVE_BREAKDOWN.
+constexpr int SOLVER_MAX_BREAK_RESTARTS = 32;
 
 // Read by every block of the streaming kernels, written only by the
 // one-block finalize kernels between them, so no intra-kernel races.
@@ -28,7 +33,7 @@ struct SolverScalars {
   double sn, rn, rn_max, delta;
   double final_res, rel_l2, max_err;
   int it, max_iter, state, iterations;
-  int flags, replace, n_replace, pad;
+  int flags, replace, n_replace, n_break;  // replace: 1 reliable, 2 convergence check, 3 breakdown restart
 };
 
 constexpr unsigned SOLVER_MAX_GRID = 1024;  // 4 blocks of 256 per CU on 256 CUs, grid-stride

@@ -81,7 +81,7 @@ class SolveParams(C.Structure):
 class SolveResult(C.Structure):
     _fields_ = [("iterations", _i32), ("converged", _i32), ("state", _i32), ("replacements", _i32),
                 ("fallback", _i32), ("initial_residual", _d), ("final_residual", _d), ("relative_l2_error", _d),
-                ("max_error", _d), ("ms", _f)]
+                ("max_error", _d), ("ms", _f), ("breakdown_restarts", _i32)]
 
 
 class TrainParams(C.Structure):

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import lsmps_amd as L
+import oracle_py as O
 from lsmps_amd import timeline
 
 pytestmark = pytest.mark.gpu
@@ -214,3 +215,136 @@ def test_timeline_conserves_particles_and_converges():
     assert sum(r["rebuild"] for r in rows) == 3
     # Injected particles join the PPE at the next rebuild.
     assert rows[10]["unknowns"] > rows[9]["unknowns"]
+
+
+def _error_code(exc):
+    return int(str(exc).split("(")[1].split(")")[0])
+
+
+def test_stale_lists_and_repeated_skin_build_refused():
+    """The stages refuse lists that may have lost pairs: once the particles
+    may have closed in by the skin since the build (2 x the displacement
+    bound >= skin), WLS and assembly return LSMPS_E_STATE; lists built
+    without a skin are stale after any motion. A second build straight after
+    a skin build of the same particles is refused; after an append it runs.
+    WLS with a radius beyond the lists' is an argument error."""
+    l0 = 0.01
+    re = 2.1 * l0
+    skin = 0.2 * re
+    xyz, t, layer, lx, ly = timeline.gap_scene(20000, l0, 5)
+    u = np.zeros_like(xyz)
+    u[:, 0] = np.asarray(timeline.LAYER_SPEED)[layer]  # |u| <= 1
+    p = L.Pipeline(0)
+    try:
+        say("guards: skin build")
+        p.upload(xyz, t)
+        p.set_velocities(u)
+        p.neighbors_skin(re, skin, 128, l0, True)
+        with pytest.raises(L.LsmpsError) as e:
+            p.neighbors(re, 128)
+        assert _error_code(e.value) == -3
+        p.set_field(L.C1_FIELD)
+        with pytest.raises(L.LsmpsError) as e:
+            p.wls(re * 1.5, l0, L.WLS_FP64)
+        assert _error_code(e.value) == -1
+        dt = 0.2 * skin  # moved = 0.2 skin: 2 x 0.2 < 1
+        p.advance(dt)
+        p.wls(re, l0, L.WLS_FP64)
+        p.assemble(re, 1 << 30)
+        p.advance(dt)
+        p.advance(dt)  # moved = 0.6 skin: 2 x 0.6 >= 1
+        say("guards: stale")
+        for call in (lambda: p.wls(re, l0, L.WLS_MIXED), lambda: p.assemble(re, 1 << 30)):
+            with pytest.raises(L.LsmpsError) as e:
+                call()
+            assert _error_code(e.value) == -3
+        p.append(xyz[:1] + np.array([[0.0, 0.0, 3 * l0]]), t[:1], u[:1])
+        p.neighbors_skin(re, skin, 128, l0, True)  # lists current again
+        p.set_field(L.C1_FIELD)
+        p.wls(re, l0, L.WLS_FP64)
+    finally:
+        p.close()
+    say("guards: plain lists")
+    p = L.Pipeline(0)
+    try:
+        p.upload(xyz, t)
+        p.set_velocities(u)
+        p.neighbors_classify(re, 128, l0)
+        p.advance(0.0)
+        p.set_field(L.C1_FIELD)
+        p.wls(re, l0, L.WLS_FP64)  # no motion yet
+        p.advance(1e-9)
+        with pytest.raises(L.LsmpsError) as e:
+            p.wls(re, l0, L.WLS_FP64)
+        assert _error_code(e.value) == -3
+    finally:
+        p.close()
+
+
+def test_skin_lists_match_oracle_fresh_build():
+    """Skin lists after motion hold every pair the stages need: on the gap
+    scene with its per-layer velocities, after three steps (relative motion
+    below skin / 2), the GPU's PPE and FP64 WLS from the skin lists built
+    before the motion equal the CPU oracle's from lists built afresh at the
+    moved positions -- the same unknowns, the same off-diagonal entries bit
+    for bit, the diagonal and b = A x* to rounding (their list-order sums
+    run over the pairs in another order), the same WLS statuses, gradient
+    and Laplacian to rounding. The moved positions are x += u dt three
+    times on the host, the same FP64 operations the GPU performs."""
+    l0 = 0.01
+    re = 2.1 * l0
+    skin = 0.2 * re
+    maxn = 128
+    xyz, t, layer, lx, ly = timeline.gap_scene(40000, l0, 5)
+    u = np.zeros_like(xyz)
+    u[:, 0] = np.asarray(timeline.LAYER_SPEED)[layer]
+    dt = 0.5 * skin / (3 * 1.2)  # three steps move any pair less than skin / 2
+    say("skin vs oracle: gap scene", len(xyz))
+    p = L.Pipeline(0)
+    try:
+        p.upload(xyz, t)
+        p.set_velocities(u)
+        cs, _ = p.neighbors_skin(re, skin, maxn, l0, True)
+        assert cs["truncated"] == 0
+        for _ in range(3):
+            p.advance(dt)
+        p.set_field(L.C1_FIELD)
+        p.wls(re, l0, L.WLS_FP64)
+        ga, la, sa = p.wls_results()
+        p.assemble(re, 1 << 30, L.XSTAR_QUADRATIC)
+        Aa = p.ppe()
+    finally:
+        p.close()
+    say("skin vs oracle: GPU phase done")
+    moved = xyz.copy()
+    for _ in range(3):
+        moved += u * dt
+    o0 = O.Oracle(xyz, t)
+    o0.neighbors(re, maxn)
+    o0.classify(l0)
+    o = O.Oracle(moved, o0.types())
+    _, trunc = o.neighbors(re, maxn)
+    assert trunc == 0
+    o.assemble(re, 1 << 30, 0)
+    Ab = o.get_ppe()
+    gb, lb, sb = o.wls_all(L.cases.field_values(moved, L.C1_FIELD), re, l0)
+    assert np.array_equal(Aa["row"], Ab["row"]) and np.array_equal(Aa["rev"], Ab["rev"])
+    assert np.array_equal(Aa["x_exact"], Ab["x_exact"])
+    m = len(Ab["b"])
+    for A in (Aa, Ab):
+        diag_at = A["row"][1:] - 1
+        A["diag"] = A["val"][diag_at]
+        off = np.ones(len(A["val"]), bool)
+        off[diag_at] = False
+        rows = np.repeat(np.arange(m), np.diff(A["row"]))[off]
+        order = np.lexsort((A["col"][off], rows))
+        A["off"] = (rows[order], A["col"][off][order], A["val"][off][order])
+    for k in range(3):
+        assert np.array_equal(Aa["off"][k], Ab["off"][k])
+    np.testing.assert_allclose(Aa["diag"], Ab["diag"], rtol=1e-13)
+    np.testing.assert_allclose(Aa["b"], Ab["b"], rtol=1e-12, atol=1e-12 * np.abs(Ab["b"]).max())
+    assert np.array_equal(sa, sb)
+    ok = sa == L.WLS_OK
+    assert ok.sum() >= 0.45 * len(xyz)  # the two film layers (the walls are skipped)
+    np.testing.assert_allclose(ga[ok], gb[ok], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(la[ok], lb[ok], rtol=1e-8, atol=1e-6)

@@ -454,3 +454,80 @@ def test_hashed_uniform_known_answers():
         assert L.orc_hashed_uniform(j, 1234567) == want
     # Seed 0 is the finaliser of the row itself; the finaliser maps 0 to 0.
     assert L.orc_hashed_uniform(0, 0) == -1.0
+
+
+def test_features_recomputed_from_geometry():
+    """Every feature of the learned preconditioner recomputed independently
+    of the oracle's lists and loops: neighbor pairs from a k-d tree on the
+    positions (no truncation in this scene, so they are the lists' pairs),
+    per-row segment sums in numpy, the graph-Laplacian diagonal
+    sum_j w/r^2 over unknown neighbors from the same pairs, and the PPE's
+    b only through q = b / diag. A changed distance moment (r/l0 for r/re,
+    a dropped square), diagonal scaling, one-hop average or Wall share
+    fails here."""
+    from scipy.spatial import cKDTree
+
+    l0, re = 0.01, 0.021
+    xyz, t = O.generate_case("wall_film", 3000, 5, l0)
+    o = O.Oracle(xyz, t)
+    _, truncated = o.neighbors(re, 96)
+    assert truncated == 0
+    o.classify(l0)
+    o.assemble(re, 1 << 30, 1, 9)
+    f, sb = o.features(re)
+    A = o.get_ppe()
+    types = o.types()
+    rev = A["rev"]
+    n, m = len(xyz), len(rev)
+    pairs = cKDTree(xyz).query_pairs(re, output_type="ndarray")
+    i = np.concatenate([pairs[:, 0], pairs[:, 1]])
+    j = np.concatenate([pairs[:, 1], pairs[:, 0]])
+    r = np.linalg.norm(xyz[j] - xyz[i], axis=1)
+    keep = r < re  # query_pairs is inclusive at re; the reference is strict
+    i, j, r = i[keep], j[keep], r[keep]
+    w = np.where(r > 1e-12, (re / r - 1.0) ** 2, 0.0)
+    a = np.where(w > 0, w / (r * r), 0.0)
+    k = np.bincount(i, minlength=n).astype(np.float64)
+    unk = np.full(n, -1)
+    unk[rev] = np.arange(m)
+    isu = (unk[j] >= 0) & (w > 0)
+    diag_geo = np.bincount(i, weights=np.where(isu, a, 0.0), minlength=n)
+    d = A["val"][A["row"][1:] - 1]
+    lens = np.diff(A["row"])
+    pinned = (np.arange(m) == 0) | ((lens == 1) & (d == 1.0))
+    # The assembled diagonal is the geometric one except on pinned/failed rows.
+    ok = ~pinned
+    np.testing.assert_allclose(d[ok], diag_geo[rev][ok], rtol=1e-12)
+    q = A["b"] / d
+    sb_ref = np.sqrt(np.mean(q * q))
+    assert abs(sb - sb_ref) <= 1e-12 * sb_ref
+    kk = k[rev]
+    ik = np.where(kk > 0, 1.0 / np.maximum(kk, 1), 0.0)
+    rn = r / re
+    mean_r = np.bincount(i, weights=rn, minlength=n)[rev] * ik
+    mean_r2 = np.bincount(i, weights=rn * rn, minlength=n)[rev] * ik
+    rmin = np.full(n, np.inf)
+    np.minimum.at(rmin, i, rn)
+    rmin = np.where(kk > 0, rmin[rev], 0.0)
+    a_all = np.bincount(i, weights=a, minlength=n)[rev]
+    a_unk = np.bincount(i, weights=np.where(isu, a, 0.0), minlength=n)[rev]
+    n_unk = np.bincount(i, weights=isu.astype(np.float64), minlength=n)[rev]
+    hop = np.bincount(i, weights=np.where(isu, a * q[np.maximum(unk[j], 0)], 0.0), minlength=n)[rev]
+    n_wall = np.bincount(i, weights=(types[j] == 4).astype(np.float64), minlength=n)[rev]
+    want = np.zeros((m, 16))
+    want[np.arange(m), types[rev]] = 1.0
+    want[:, 5] = kk / 32.0
+    want[:, 6] = mean_r
+    want[:, 7] = mean_r2
+    want[:, 8] = rmin
+    want[:, 9] = d * re * re / 32.0
+    want[:, 10] = np.where(a_all > 0, a_unk / np.where(a_all > 0, a_all, 1.0), 0.0)
+    want[:, 11] = n_unk * ik
+    want[:, 12] = q / sb_ref
+    want[:, 13] = np.where(pinned, 0.0, hop / d / sb_ref)
+    want[:, 14] = n_wall * ik
+    want[:, 15] = pinned
+    np.testing.assert_allclose(f.astype(np.float64), want, rtol=2e-6, atol=2e-6)
+    # The scene exercises every feature: walls among the neighbors, rows
+    # with non-unknown neighbors, a spread of distances.
+    assert want[:, 14].max() > 0.1 and want[:, 10].min() < 0.9 and np.ptp(want[:, 6]) > 0.03
