@@ -964,7 +964,7 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   HIPCHK(ctx->grad_s.ensure(3 * (size_t)n));
   HIPCHK(ctx->lap_s.ensure(n));
   HIPCHK(ctx->wstatus_s.ensure(n));
-  if (p->precision == LSMPS_WLS_MIXED) HIPCHK(ctx->redo.ensure(n));
+  HIPCHK(ctx->redo.ensure(std::max(n, 1)));
   HIPCHK(ctx->bpart.ensure(6 * ((size_t)n / 64 + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
   // Field into the sorted records: evaluated there for a quadratic,
@@ -979,11 +979,12 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   zero_counters(ctx, 16);
   const int* nb = ctx->nbr.p;
   const int* nc = ctx->cnt.p;
+  launch_fill_int(ctx->nredo.p, 1, 0, st);
   if (p->precision == LSMPS_WLS_FP64) {
     launch_wls_exact(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
-                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->counters.p, ctx->bpart.p, st, ctx->e2, ctx->e3);
+                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, ctx->bpart.p, st,
+                     ctx->e2, ctx->e3);
   } else {
-    launch_fill_int(ctx->nredo.p, 1, 0, st);
     launch_wls_mixed(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
                      ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, ctx->bpart.p, st,
                      ctx->e2, ctx->e3);

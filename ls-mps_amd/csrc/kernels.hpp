@@ -97,12 +97,14 @@ void launch_ell_to_csr(const CnlBufs& B, int n, int maxn, const int* row_ptr, in
 // outputs in sorted order)
 void launch_field_quadratic(double4* spos, int n, const QuadField& q, hipStream_t st);
 void launch_gather_field(const int* perm, const double* v, int n, double4* spos, hipStream_t st);
-// bpart: per-block count partials, 6 * ceil(n / 128) (exact) or
+// bpart: per-block count partials, 5 * ceil(n / 64) (exact) or
 // 2 * ceil(n / 64) (mixed) words.
+// FP64: rows whose first factorisation fails take the reference's ladder
+// in k_wls_redo through the work list redo / nredo (*nredo zeroed first).
 void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
-                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status,
-                      unsigned long long* counts, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin = nullptr,
-                      hipEvent_t ev_end = nullptr);
+                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
+                      int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,
+                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
                       int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,

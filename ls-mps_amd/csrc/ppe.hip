@@ -128,29 +128,29 @@ __device__ __forceinline__ double replay_weight(uint32_t seed, int lo, int hi, d
   return (0.5 + unit53(mix64(key))) / (l0 * l0);
 }
 
-// One thread per row: a slice of 64 rows is one wavefront, so the slice's
-// width is a wavefront maximum, a row's entries at position k of the slice
-// are 64 consecutive words (coalesced stores), and the list-order sums of
-// the diagonal and of b are plain sequential sums in the row's thread.
-// Blocks of PPE_THREADS threads hold PPE_THREADS / 64 slices; the block index
-// is remapped so each XCD takes a contiguous range of slices.
-constexpr int PPE_THREADS = 256;
-constexpr int PPE_SLICES_PER_BLOCK = PPE_THREADS / PPE_SLICE;
-// Neighbor entries each thread has in flight (indices, then their unknown
-// ids and positions) before it consumes them in list order.
-constexpr int PPE_BATCH = 4;
+// Eight lanes per row, eight rows per wavefront, one 64-row slice per
+// block. The lanes of a row walk its neighbor list in batches of four
+// chunks of eight entries: all of a batch's list entries are loaded first,
+// then their unknown ids and positions, so a lane has four gathers in
+// flight instead of one.
+constexpr int PPE_GROUP = 8;
+constexpr int PPE_THREADS = PPE_SLICE * PPE_GROUP;  // one slice per block
+constexpr int PPE_BATCH = 4;                        // chunks of PPE_GROUP entries per batch
+constexpr int PPE_BATCH_ENTRIES = PPE_BATCH * PPE_GROUP;
 
+// Slice-local row of this thread's group and the slice index.
 struct PpeRow {
-  int slice, u, lane;
-  bool live, slice_live;
+  int slice, u, g, gbase, local;
+  bool live;
 };
 __device__ __forceinline__ PpeRow ppe_row(int m) {
   PpeRow R;
-  R.lane = threadIdx.x & (PPE_SLICE - 1);
-  R.slice = (int)xcd_block(blockIdx.x, gridDim.x) * PPE_SLICES_PER_BLOCK + (int)(threadIdx.x / PPE_SLICE);
-  R.u = R.slice * PPE_SLICE + R.lane;
+  R.slice = (int)xcd_block(blockIdx.x, gridDim.x);
+  R.g = threadIdx.x & (PPE_GROUP - 1);
+  R.gbase = threadIdx.x & ~(PPE_GROUP - 1) & (WAVE - 1);
+  R.local = (int)(threadIdx.x / PPE_GROUP);
+  R.u = R.slice * PPE_SLICE + R.local;
   R.live = R.u < m;
-  R.slice_live = R.slice * PPE_SLICE < m;
   return R;
 }
 
@@ -161,32 +161,41 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_count(const int* __restrict
                                                            const int* __restrict__ uid, const int* __restrict__ nbr,
                                                            const int* __restrict__ ncnt, int maxn, int m, int pin0,
                                                            int* __restrict__ wsz) {
+  __shared__ int wmax;
+  if (threadIdx.x == 0) wmax = 0;
+  __syncthreads();
   const PpeRow R = ppe_row(m);
-  if (!R.slice_live) return;  // whole wavefronts past the last slice
   int c = 0;
   if (R.live && (!pin0 || urr[R.u] != 0)) {
     const int s = uslot[R.u];
     const int* row = nbr + (size_t)s * maxn;
     const int k = ncnt[s];
-    for (int q0 = 0; q0 < k; q0 += PPE_BATCH) {
+    for (int c0 = 0; c0 < k; c0 += PPE_BATCH_ENTRIES) {
       int s2[PPE_BATCH];
 #pragma unroll
-      for (int j = 0; j < PPE_BATCH; ++j) s2[j] = q0 + j < k ? row[q0 + j] : -1;
+      for (int j = 0; j < PPE_BATCH; ++j) {
+        const int q = c0 + j * PPE_GROUP + R.g;
+        s2[j] = q < k ? row[q] : -1;
+      }
 #pragma unroll
       for (int j = 0; j < PPE_BATCH; ++j) c += (s2[j] >= 0 && uid[s2[j]] >= 0) ? 1 : 0;
     }
   }
-  const int w = wave_max(R.live ? c + 1 : 0);
-  if (R.lane == 0) wsz[R.slice] = w * PPE_SLICE;
+  c = group8_sum_int(c);
+  if (R.live && R.g == 0) atomicMax(&wmax, c + 1);
+  __syncthreads();
+  if (threadIdx.x == 0) wsz[R.slice] = wmax * PPE_SLICE;
 }
 
 // The fill pass. Outputs in row order u: the sliced-ELL columns (sorted
 // numbering) and FP64 values, the row lengths, the diagonal, x* and b.
 // Counts per block into bpart (then counters): [0] failed rows, [1] NearWall
 // rows except row 0, [2] nonzeros.
-// Columns come out ascending: the list is ascending in sorted slots and the
-// row numbering u is monotone in the slot, so the diagonal takes the
-// position where the first column above u would go.
+// Each lane computes its entries' a = w / r^2 and -a x*_j; ordered ballots
+// place the kept entries; the batch's terms go to LDS in list order (an
+// entry that is not kept contributes exact zeros, which leave the sums
+// unchanged), and lanes 0 and 1 of the row add them to the diagonal and to
+// b in that order, as the reference's sequential loop does.
 // REPLAY: the edges of a replay graph (slots are particle indices) with
 // replay_weight(wseed, ...) in place of w/r^2 from positions (re unused);
 // self-edges are skipped as the reference skips r <= 1e-12.
@@ -199,41 +208,45 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
     int* __restrict__ rlen, double* __restrict__ diag, double* __restrict__ xe, double* __restrict__ b,
     unsigned* __restrict__ bpart) {
   __shared__ unsigned lds[3 * PPE_THREADS / WAVE];
+  __shared__ double terms[PPE_SLICE][2][PPE_BATCH_ENTRIES];  // per row: a, then -a x*_j
   const PpeRow R = ppe_row(m);
+  const unsigned below = (1u << R.g) - 1u;
   int failed = 0, neumann = 0, len = 0;
-  if (R.slice_live) {
+  if (R.live) {
+    const int u = R.u;
+    const int s = uslot[u];
+    const int rr = urr[u];
+    const double4 pi = spos[s];
     const int sb = sptr[R.slice];
     const int width = (sptr[R.slice + 1] - sb) / PPE_SLICE;
-    int* cw = col + sb + R.lane;
-    double* vw = val + sb + R.lane;
-    if (R.live) {
-      const int u = R.u;
-      const int s = uslot[u];
-      const int rr = urr[u];
-      const double4 pi = spos[s];
-      const double xu = xmode == 0 ? pi.x * pi.x + pi.y * pi.y  // x* = x^2 + y^2 (src/lsmps.cpp:29)
-                                   : xstar_random(xseed, rr);
-      double d = 0.0, bs = 0.0;
-      int at = 0, dpos = -1;
-      if (rr != 0) {
-        const int* row = nbr + (size_t)s * maxn;
-        const int k = ncnt[s];
-        for (int q0 = 0; q0 < k; q0 += PPE_BATCH) {
-          int s2[PPE_BATCH], cj[PPE_BATCH];
-          double4 pj[PPE_BATCH];
+    const int base = sb + (u - R.slice * PPE_SLICE);
+    const double xu = xmode == 0 ? pi.x * pi.x + pi.y * pi.y  // x* = x^2 + y^2 (src/lsmps.cpp:29)
+                                 : xstar_random(xseed, rr);
+    double acc = 0.0;  // lane 0: the diagonal; lane 1: b (both in list order)
+    int kept = 0, below_u = 0;
+    if (rr != 0) {
+      const int* row = nbr + (size_t)s * maxn;
+      const int k = ncnt[s];
+      double* tl = &terms[R.local][0][0];
+      for (int c0 = 0; c0 < k; c0 += PPE_BATCH_ENTRIES) {
+        int s2[PPE_BATCH], cj[PPE_BATCH];
+        double4 pj[PPE_BATCH];
 #pragma unroll
-          for (int j = 0; j < PPE_BATCH; ++j) s2[j] = q0 + j < k ? row[q0 + j] : -1;
+        for (int j = 0; j < PPE_BATCH; ++j) {
+          const int q = c0 + j * PPE_GROUP + R.g;
+          s2[j] = q < k ? row[q] : -1;
+        }
 #pragma unroll
-          for (int j = 0; j < PPE_BATCH; ++j) cj[j] = s2[j] >= 0 ? uid[s2[j]] : -1;
-          if (!REPLAY) {
+        for (int j = 0; j < PPE_BATCH; ++j) cj[j] = s2[j] >= 0 ? uid[s2[j]] : -1;
+        if (!REPLAY) {
 #pragma unroll
-            for (int j = 0; j < PPE_BATCH; ++j) pj[j] = cj[j] >= 0 ? spos[s2[j]] : pi;
-          }
+          for (int j = 0; j < PPE_BATCH; ++j) pj[j] = cj[j] >= 0 ? spos[s2[j]] : pi;
+        }
 #pragma unroll
-          for (int j = 0; j < PPE_BATCH; ++j) {
-            if (cj[j] < 0) continue;
-            double a = 0.0, xj = 0.0;
-            bool keep;
+        for (int j = 0; j < PPE_BATCH; ++j) {
+          double a = 0.0, xj = 0.0;
+          bool keep = false;
+          if (cj[j] >= 0) {
             if (REPLAY) {
               keep = s2[j] != s;
               if (keep) {
@@ -244,48 +257,69 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
               keep = pair_coef(pj[j].x - pi.x, pj[j].y - pi.y, pj[j].z - pi.z, re, &a);
               if (keep) xj = xmode == 0 ? pj[j].x * pj[j].x + pj[j].y * pj[j].y : xstar_random(xseed, urr[cj[j]]);
             }
-            if (!keep) continue;
-            if (dpos < 0 && cj[j] > u) dpos = at++;
-            cw[at * PPE_SLICE] = cj[j];
-            vw[at * PPE_SLICE] = -a;
-            ++at;
-            // Sums in list order, as the reference's CSR order gives them.
-            d += a;
-            bs += -a * xj;
           }
+          const unsigned gm = group8_ballot(keep, R.gbase);
+          const unsigned lt = group8_ballot(keep && cj[j] < u, R.gbase);
+          if (keep) {
+            // Entries with a column above u sit one slot later: the diagonal
+            // takes slot below_u.
+            const int slot = kept + __popc(gm & below) + (cj[j] > u ? 1 : 0);
+            col[base + slot * PPE_SLICE] = cj[j];
+            val[base + slot * PPE_SLICE] = -a;
+          }
+          kept += __popc(gm);
+          below_u += __popc(lt);
+          const int t = j * PPE_GROUP + R.g;
+          tl[t] = keep ? a : 0.0;
+          tl[PPE_BATCH_ENTRIES + t] = keep ? -a * xj : 0.0;
         }
-        if (d <= 0 || !isfinite(d)) {
-          failed = 1;
-          d = 1.0;
+        wave_sync();  // a row's terms are written and read by its own wavefront
+        if (R.g < 2) {
+          const double* src = tl + R.g * PPE_BATCH_ENTRIES;
+          const int nt = k - c0 < PPE_BATCH_ENTRIES ? k - c0 : PPE_BATCH_ENTRIES;
+          for (int t = 0; t < nt; ++t) acc += src[t];
         }
-        if (shift != 0.0) d += shift * d;
-        neumann = type_s[s] == T_NEARWALL ? 1 : 0;
-      } else {
-        d = 1.0;  // the identity pin of row 0
+        wave_sync();  // read before the next batch overwrites them
       }
-      if (dpos < 0) dpos = at++;
-      bs += d * xu;
-      len = at;
-      cw[dpos * PPE_SLICE] = u;
-      vw[dpos * PPE_SLICE] = d;
+    }
+    double d = __shfl(acc, R.gbase), bs = __shfl(acc, R.gbase + 1);
+    if (rr != 0) {
+      if (d <= 0 || !isfinite(d)) {
+        failed = 1;
+        d = 1.0;
+      }
+      if (shift != 0.0) d += shift * d;
+      neumann = type_s[s] == T_NEARWALL ? 1 : 0;
+    } else {
+      d = 1.0;  // the identity pin of row 0
+    }
+    bs += d * xu;
+    len = kept + 1;
+    if (R.g == 0) {
+      col[base + below_u * PPE_SLICE] = u;
+      val[base + below_u * PPE_SLICE] = d;
       rlen[u] = len;
       diag[u] = d;
       xe[u] = xu;
       b[u] = bs;
-      for (int k2 = len; k2 < width; ++k2) {
-        cw[k2 * PPE_SLICE] = u;
-        vw[k2 * PPE_SLICE] = 0.0;
-      }
-    } else {
-      // Rows past m in the last slice: every stored entry is defined, as
-      // (0, 0), so passes over all stored entries stay in bounds.
-      for (int k2 = 0; k2 < width; ++k2) {
-        cw[k2 * PPE_SLICE] = 0;
-        vw[k2 * PPE_SLICE] = 0.0;
-      }
+    }
+    for (int k2 = len + R.g; k2 < width; k2 += PPE_GROUP) {
+      col[base + k2 * PPE_SLICE] = u;
+      val[base + k2 * PPE_SLICE] = 0.0;
+    }
+  } else {
+    // Rows past m in the last slice: every stored entry is defined, as
+    // (0, 0), so passes over all stored entries stay in bounds.
+    const int sb = sptr[R.slice];
+    const int width = (sptr[R.slice + 1] - sb) / PPE_SLICE;
+    const int base = sb + (R.u - R.slice * PPE_SLICE);
+    for (int k2 = R.g; k2 < width; k2 += PPE_GROUP) {
+      col[base + k2 * PPE_SLICE] = 0;
+      val[base + k2 * PPE_SLICE] = 0.0;
     }
   }
-  unsigned v[3] = {(unsigned)failed, (unsigned)neumann, (unsigned)len};
+  const bool leader = R.live && R.g == 0;
+  unsigned v[3] = {leader ? (unsigned)failed : 0u, leader ? (unsigned)neumann : 0u, leader ? (unsigned)len : 0u};
   block_count_partials<3>(v, bpart, lds);
 }
 
@@ -522,7 +556,7 @@ __global__ __launch_bounds__(256) void k_from_rr(const int* __restrict__ urr, co
 static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 int ppe_slices(int m) { return (m + PPE_SLICE - 1) / PPE_SLICE; }
-static int ppe_fill_blocks(int m) { return (ppe_slices(m) + PPE_SLICES_PER_BLOCK - 1) / PPE_SLICES_PER_BLOCK; }
+static int ppe_fill_blocks(int m) { return ppe_slices(m); }
 
 void launch_unknown_flags(const uint8_t* type_s, const int* iperm, int n, int* flag, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_unknown_flags, dim3(blocks(n, 256)), dim3(256), 0, st, type_s, iperm, n, flag);

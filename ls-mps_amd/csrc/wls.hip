@@ -12,9 +12,10 @@
 //    reference, so the results are bit-identical to the reference's. Only
 //    the lower triangle is accumulated: it is the part the factorisation
 //    reads, and each entry is formed as (w q_a) q_c with a >= c, exactly as
-//    the reference forms it. One thread per particle; the ladder re-runs
-//    the (deterministic) accumulation instead of keeping a copy of M, which
-//    keeps the thread at ~70 live doubles.
+//    the reference forms it. One thread per particle and the first
+//    factorisation only; a row that needs the regularisation ladder goes to
+//    the work list of k_wls_redo (below), which runs the whole ladder with
+//    the same results.
 //  * k_wls_mixed  FP32 moments on offsets normalised by re, FMA
 //    accumulation, FP32 Cholesky. A row whose FP32 pivots lose more than
 //    three digits (pivot <= 1e-3 * diagonal), or that would need the
@@ -24,76 +25,6 @@
 #include "wls_core.hpp"
 
 namespace lsmps_hip {
-
-// FP64 solve of particle i with the reference's operation order. Returns
-// the status; writes grad/lap.
-__device__ __noinline__ uint8_t wls_exact_one(int i, const double4* __restrict__ spos, const int* __restrict__ nbr,
-                                              const int* __restrict__ cnt, int maxn, double re, double l0, double reg,
-                                              double* g3, double* lap) {
-  const double4 pi = spos[i];
-  const double xi = pi.x, yi = pi.y, zi = pi.z, fi = pi.w;
-  const int* col = nbr + (size_t)i * maxn;
-  const int rb = 0, rend = cnt[i];
-  const double h2 = 2 * l0;
-  for (int attempt = 0; attempt < 5; ++attempt) {
-    double M[45], B[9], ws = 0.0;
-#pragma unroll
-    for (int k = 0; k < 45; ++k) M[k] = 0.0;
-#pragma unroll
-    for (int a = 0; a < 9; ++a) B[a] = 0.0;
-    for (int k = rb; k < rend; ++k) {
-      const double4 pj = spos[col[k]];
-      const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-      const double r = sqrt(dx * dx + dy * dy + dz * dz);
-      if (r <= 1e-12 || r >= re) continue;
-      const double tq = re / r - 1.0;
-      const double w = tq * tq;
-      if (w == 0.0) continue;
-      const double inv = 1.0 / r, h = h2 * r, g = l0 * r;
-      double q[9];
-      q[0] = dx * inv;
-      q[1] = dy * inv;
-      q[2] = dz * inv;
-      q[3] = dx * dx / h;
-      q[4] = dy * dy / h;
-      q[5] = dz * dz / h;
-      q[6] = dx * dy / g;
-      q[7] = dx * dz / g;
-      q[8] = dy * dz / g;
-      const double df = pj.w - fi;
-      ws += w;
-#pragma unroll
-      for (int a = 0; a < 9; ++a) {
-        const double wq = w * q[a];
-        B[a] += wq * df / r;
-#pragma unroll
-        for (int c = 0; c <= a; ++c) M[tri(a, c)] += wq * q[c];
-      }
-    }
-    if (ws <= 0) return WLS_NO_WEIGHT;
-#pragma unroll
-    for (int k = 0; k < 45; ++k) M[k] /= ws;
-#pragma unroll
-    for (int a = 0; a < 9; ++a) B[a] /= ws;
-    const double add = attempt == 0 ? 0.0 : attempt == 1 ? reg * 1.0 : attempt == 2 ? reg * 10.0
-                                                            : attempt == 3 ? reg * 100.0 : reg * 1000.0;
-    if (attempt > 0 && add == 0.0) return WLS_FAILED;  // zero reg: every retry equals attempt 0
-    if (add != 0.0) {
-#pragma unroll
-      for (int a = 0; a < 9; ++a) M[tri(a, a)] += add;
-    }
-    const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (!chol9<double>(M, 1e-14, 0.0, zero9)) continue;
-    double sol[9];
-    chol9_solve<double>(M, B, sol);
-    g3[0] = sol[0];
-    g3[1] = sol[1];
-    g3[2] = sol[2];
-    *lap = (sol[3] + sol[4] + sol[5]) / l0;
-    return attempt == 0 ? WLS_OK : WLS_REGULARIZED;
-  }
-  return WLS_FAILED;
-}
 
 // Block-aggregated status histogram (statuses 0..4 and the redo slot);
 // every thread of the block must call it, with slot < 0 for "nothing".
@@ -116,27 +47,106 @@ __device__ __forceinline__ void store_result(int i, uint8_t st, const double* g3
   status[i] = st;
 }
 
-__global__ __launch_bounds__(128) void k_wls_exact(const double4* __restrict__ spos,
-                                                   const uint8_t* __restrict__ type, const int* __restrict__ nbr,
-                                                   const int* __restrict__ cnt, int maxn, int n, double re, double l0,
-                                                   double reg, double* __restrict__ grad, double* __restrict__ lap,
-                                                   uint8_t* __restrict__ status, unsigned* __restrict__ bpart) {
-  const int i = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+// FP64 WLS of one particle per lane, in the reference's operation order:
+// the moments and right-hand sides are accumulated neighbor by neighbor
+// (each neighbor's record is loaded one neighbor ahead and its index two
+// ahead, so the loads overlap the previous neighbor's arithmetic), then the
+// first Cholesky (no diagonal shift) and the solve. A row whose first
+// factorisation fails goes to the work list of k_wls_redo, which runs the
+// reference's whole ladder with the same results bit for bit; keeping the
+// ladder out of this kernel takes its register count from 248 to 200.
+// Measured at 3M (industrial_mix): 1.67 ms; forcing three wavefronts per
+// SIMD (168 VGPRs) 1.75 ms, four (128 VGPRs, spills) 7.2 ms. Counters put
+// it at ~500 VALU instructions per neighbor pair, most of them the
+// correctly rounded FP64 divisions the reference's expressions need.
+__global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ spos,
+                                                    const uint8_t* __restrict__ type, const int* __restrict__ nbr,
+                                                    const int* __restrict__ cnt, int maxn, int n, double re, double l0,
+                                                    double* __restrict__ grad, double* __restrict__ lap,
+                                                    uint8_t* __restrict__ status, int* __restrict__ redo,
+                                                    int* __restrict__ nredo, unsigned* __restrict__ bpart) {
+  const int i = (int)xcd_block(blockIdx.x, gridDim.x) * WAVE + threadIdx.x;
   int slot = -1;
   if (i < n) {
     const uint8_t t = type[i];
     double g3[3] = {0.0, 0.0, 0.0}, L = 0.0;
-    uint8_t st = WLS_SKIPPED;
-    if (t != T_WALL && t != T_SPLASH) st = wls_exact_one(i, spos, nbr, cnt, maxn, re, l0, reg, g3, &L);
-    store_result(i, st, g3, L, grad, lap, status);
-    slot = st;
+    int st = WLS_SKIPPED;
+    if (t != T_WALL && t != T_SPLASH) {
+      const double4 pi = spos[i];
+      const int* col = nbr + (size_t)i * maxn;
+      const int kn = cnt[i];
+      const double h2 = 2 * l0;
+      double M[45], B[9], ws = 0.0;
+#pragma unroll
+      for (int k = 0; k < 45; ++k) M[k] = 0.0;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) B[a] = 0.0;
+      double4 pn = kn > 0 ? spos[col[0]] : pi;
+      int jn = kn > 1 ? col[1] : 0;
+      for (int k = 0; k < kn; ++k) {
+        const double4 pj = pn;
+        if (k + 1 < kn) pn = spos[jn];
+        if (k + 2 < kn) jn = col[k + 2];
+        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (r <= 1e-12 || r >= re) continue;
+        const double tq = re / r - 1.0;
+        const double w = tq * tq;
+        if (w == 0.0) continue;
+        const double inv = 1.0 / r, h = h2 * r, g = l0 * r;
+        double q[9];
+        q[0] = dx * inv;
+        q[1] = dy * inv;
+        q[2] = dz * inv;
+        q[3] = dx * dx / h;
+        q[4] = dy * dy / h;
+        q[5] = dz * dz / h;
+        q[6] = dx * dy / g;
+        q[7] = dx * dz / g;
+        q[8] = dy * dz / g;
+        const double df = pj.w - pi.w;
+        ws += w;
+#pragma unroll
+        for (int a = 0; a < 9; ++a) {
+          const double wq = w * q[a];
+          B[a] += wq * df / r;
+#pragma unroll
+          for (int c = 0; c <= a; ++c) M[tri(a, c)] += wq * q[c];
+        }
+      }
+      if (ws <= 0) {
+        st = WLS_NO_WEIGHT;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 45; ++k) M[k] /= ws;
+#pragma unroll
+        for (int a = 0; a < 9; ++a) B[a] /= ws;
+        const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (chol9<double>(M, 1e-14, 0.0, zero9)) {
+          double sol[9];
+          chol9_solve<double>(M, B, sol);
+          g3[0] = sol[0];
+          g3[1] = sol[1];
+          g3[2] = sol[2];
+          L = (sol[3] + sol[4] + sol[5]) / l0;
+          st = WLS_OK;
+        } else {
+          st = -1;  // the ladder: k_wls_redo writes this row
+        }
+      }
+    }
+    if (st < 0) {
+      redo[atomicAdd(nredo, 1)] = i;
+    } else {
+      store_result(i, (uint8_t)st, g3, L, grad, lap, status);
+      slot = st;
+    }
   }
   count_status(slot, bpart);
 }
 
-
 // The FP64 redo of the rows the FP32 kernel could not trust, eight lanes
-// per row, with the same results as wls_exact_one bit for bit:
+// per row, with the same results as the reference's sequential ladder bit for bit:
 //  - the per-neighbor values (weight, the nine scaled monomials, the field
 //    difference and r) are computed by the row's lanes in parallel, a chunk
 //    of RX_CHUNK neighbors at a time, into LDS, with the same expressions;
@@ -544,17 +554,21 @@ void launch_field_quadratic(double4* spos, int n, const QuadField& q, hipStream_
 }
 
 void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
-                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status,
-                      unsigned long long* counts, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin,
+                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
+                      int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin,
                       hipEvent_t ev_end) {
   if (n <= 0) return;
-  const unsigned g = blocks(n, 128);
+  const unsigned g = blocks(n, WAVE);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
-  hipLaunchKernelGGL(k_wls_exact, dim3(g), dim3(128), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, reg, grad, lap,
-                     status, bpart);
+  hipLaunchKernelGGL(k_wls_exact, dim3(g), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, grad, lap, status,
+                     redo, nredo, bpart);
   if (ev_end) (void)hipEventRecord(ev_end, st);
-  CountSlots cs{WLS_REDO_SLOT + 1, {0, 1, 2, 3, 4, 5, 0, 0}};
+  CountSlots cs{WLS_REDO_SLOT, {0, 1, 2, 3, 4, 0, 0, 0}};
   launch_sum_counts(bpart, (int)g, cs, counts, st);
+  // Rows whose first factorisation failed: the reference's ladder.
+  const unsigned gr = blocks(n, RX_ROWS) < 1024u ? blocks(n, RX_ROWS) : 1024u;
+  hipLaunchKernelGGL(k_wls_redo, dim3(gr), dim3(WAVE), 0, st, spos, nbr, cnt, maxn, redo, nredo, re, l0, reg, grad, lap,
+                     status, counts);
 }
 
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
