@@ -122,6 +122,23 @@ typedef struct {
 /* Gradient and Laplacian for every particle except Wall and Splash. */
 int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out);
 
+/* lsmps_wls without allocating device memory. No counterpart in the
+ * reference: for callers that cannot allocate (work captured into a HIP
+ * graph, tight serving loops, harnesses that forbid hipMalloc). Runs the
+ * same kernels as lsmps_wls on the context's current neighbor lists and
+ * writes the gradient (3 per particle), Laplacian and status into the
+ * caller's device arrays, in particle order (particles appended since the
+ * last neighbor build are not written). The stage's work list, block
+ * partials and counters live in a fixed device array of about 9 MB per GPU,
+ * held until the process ends; a context with more than
+ * LSMPS_NOALLOC_MAX_PARTICLES particles is refused with LSMPS_E_LIMIT.
+ * Calls on one GPU share that array: calls that overlap in time on one GPU
+ * (other threads, other streams) race. The context's own WLS results
+ * (lsmps_download_wls, lsmps_wls_sampled_stats) are not changed. */
+#define LSMPS_NOALLOC_MAX_PARTICLES 2097152
+int lsmps_wls_noalloc(lsmps_ctx* ctx, const lsmps_wls_params* p, double* grad3_dev, double* lap_dev,
+                      uint8_t* status_dev, lsmps_wls_result* out);
+
 /* The reference's sampled statistics (WlsStats, include/lsmps.hpp:20):
  * stride max(1, n / wls_samples) from particle 0 until wls_samples are
  * counted, Wall/Splash skipped (src/lsmps.cpp:28). Needs a quadratic field. */

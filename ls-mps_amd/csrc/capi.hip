@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -994,6 +995,109 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   unsigned long long c[6];
   HIPCHK(read_counters(ctx, c, sizeof(c) / sizeof(c[0])));
   ctx->have_wls = true;
+  if (out) {
+    out->ok = c[WLS_OK];
+    out->regularized = c[WLS_REGULARIZED];
+    out->failed = c[WLS_FAILED] + c[WLS_NO_WEIGHT];
+    out->skipped = c[WLS_SKIPPED];
+    out->fp64_redo = c[WLS_REDO_SLOT];
+    out->ms = elapsed(ctx);
+    out->kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
+  }
+  return LSMPS_OK;
+  GUARD_END
+}
+
+}  // extern "C"
+
+namespace {
+
+// Scratch memory of lsmps_wls_noalloc, one copy per GPU, held until the
+// process ends: the stage counters, the FP64 work-list length and list, and
+// the per-block count partials. Host code never uses the array's name as a
+// device address: each device's copy is found with hipGetSymbolAddress
+// (noalloc_scratch) and only that pointer is handed to kernels and copies.
+constexpr size_t NA_COUNTERS = 0;                      // 16 x u64
+constexpr size_t NA_NREDO = 128;                       // 1 x int
+constexpr size_t NA_BPART = 256;                       // 6 x ceil(max / 64) x u32
+constexpr size_t NA_BPART_WORDS = 6 * (size_t)(LSMPS_NOALLOC_MAX_PARTICLES / 64);
+constexpr size_t NA_REDO = NA_BPART + ((NA_BPART_WORDS * 4 + 255) / 256) * 256;  // max x int
+constexpr size_t NA_BYTES = NA_REDO + (size_t)LSMPS_NOALLOC_MAX_PARTICLES * 4;
+static_assert(LSMPS_NOALLOC_MAX_PARTICLES % 64 == 0, "block partials sized per 64 particles");
+__device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mutex;
+void* na_ptr[NA_MAX_DEVICES] = {};
+
+// This device's copy of scratch (the device must be current).
+int noalloc_scratch(lsmps_ctx* ctx, unsigned char** out) {
+  if (ctx->device < 0 || ctx->device >= NA_MAX_DEVICES) return fail(ctx, LSMPS_E_LIMIT, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(na_mutex);
+  if (!na_ptr[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ctx, LSMPS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    na_ptr[ctx->device] = p;
+  }
+  *out = static_cast<unsigned char*>(na_ptr[ctx->device]);
+  return LSMPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsmps_wls_noalloc(lsmps_ctx* ctx, const lsmps_wls_params* p, double* grad3, double* lap, uint8_t* status,
+                      lsmps_wls_result* out) {
+  if (!ctx || !p) return LSMPS_E_ARG;
+  if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "WLS needs neighbors");
+  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph: use lsmps_replay_wls");
+  if (!ctx->have_field) return fail(ctx, LSMPS_E_STATE, "WLS needs a field (lsmps_set_field_*)");
+  if (p->precision != LSMPS_WLS_FP64 && p->precision != LSMPS_WLS_MIXED)
+    return fail(ctx, LSMPS_E_ARG, "unknown WLS precision");
+  if (!(p->re > 0) || !(p->l0 > 0) || !std::isfinite(p->re) || !std::isfinite(p->l0))
+    return fail(ctx, LSMPS_E_ARG, "re and l0 must be positive and finite");
+  if (p->re > ctx->re) return fail(ctx, LSMPS_E_ARG, "re exceeds the radius of the neighbor lists");
+  if (int rc = check_lists_current(ctx, "WLS")) return rc;
+  const int n = (int)ctx->n_cnl;
+  if (n > LSMPS_NOALLOC_MAX_PARTICLES)
+    return fail(ctx, LSMPS_E_LIMIT, "lsmps_wls_noalloc: more particles than LSMPS_NOALLOC_MAX_PARTICLES");
+  if (n && (!grad3 || !lap || !status)) return fail(ctx, LSMPS_E_ARG, "null output arrays");
+  GUARD_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  unsigned char* sp = nullptr;
+  if (int rc = noalloc_scratch(ctx, &sp)) return rc;
+  unsigned long long* counters = reinterpret_cast<unsigned long long*>(sp + NA_COUNTERS);
+  int* nredo = reinterpret_cast<int*>(sp + NA_NREDO);
+  unsigned* bpart = reinterpret_cast<unsigned*>(sp + NA_BPART);
+  int* redo = reinterpret_cast<int*>(sp + NA_REDO);
+  hipStream_t st = ctx->st;
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  if (!ctx->spos_field) {
+    if (ctx->field_quadratic)
+      launch_field_quadratic(ctx->spos.p, n, ctx->qf, st);
+    else
+      launch_gather_field(ctx->perm.p, ctx->f.p, n, ctx->spos.p, st);
+    ctx->spos_field = true;
+  }
+  // Counters and the work-list length start at zero (fill kernels, no
+  // memset nodes, so the call can sit in a captured graph).
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counters, 16);
+  launch_fill_int(nredo, 1, 0, st);
+  if (p->precision == LSMPS_WLS_FP64)
+    launch_wls_exact(ctx->spos.p, ctx->type_s.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, n, p->re, p->l0, p->reg, grad3, lap,
+                     status, redo, nredo, counters, bpart, st, ctx->e2, ctx->e3, ctx->perm.p);
+  else
+    launch_wls_mixed(ctx->spos.p, ctx->type_s.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, n, p->re, p->l0, p->reg, grad3, lap,
+                     status, redo, nredo, counters, bpart, st, ctx->e2, ctx->e3, ctx->perm.p);
+  if (int rc = check_launch(ctx, "wls_noalloc")) return rc;
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  unsigned long long c[6];
+  HIPCHK(hipMemcpyAsync(ctx->host_rb, counters, sizeof(c), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::memcpy(c, ctx->host_rb, sizeof(c));
   if (out) {
     out->ok = c[WLS_OK];
     out->regularized = c[WLS_REGULARIZED];

@@ -101,14 +101,16 @@ void launch_gather_field(const int* perm, const double* v, int n, double4* spos,
 // 2 * ceil(n / 64) (mixed) words.
 // FP64: rows whose first factorisation fails take the reference's ladder
 // in k_wls_redo through the work list redo / nredo (*nredo zeroed first).
+// Results go to grad / lap / status at the sorted slot, or at out_perm[slot]
+// (the particle index) when out_perm is given.
 void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
                       int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,
-                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, const int* out_perm = nullptr);
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
                       int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,
-                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, const int* out_perm = nullptr);
 void launch_wls_sample_errors(const double4* spos, const uint8_t* type, const double* grad, const double* lap,
                               const uint8_t* status, const int* iperm, int64_t stride, int ns, const QuadField& q,
                               double* ge, double* le, uint8_t* st_out, uint8_t* ty_out, hipStream_t st);

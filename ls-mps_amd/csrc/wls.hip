@@ -64,7 +64,8 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
                                                     const int* __restrict__ cnt, int maxn, int n, double re, double l0,
                                                     double* __restrict__ grad, double* __restrict__ lap,
                                                     uint8_t* __restrict__ status, int* __restrict__ redo,
-                                                    int* __restrict__ nredo, unsigned* __restrict__ bpart) {
+                                                    int* __restrict__ nredo, unsigned* __restrict__ bpart,
+                                                    const int* __restrict__ out_perm) {
   const int i = (int)xcd_block(blockIdx.x, gridDim.x) * WAVE + threadIdx.x;
   int slot = -1;
   if (i < n) {
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
     if (st < 0) {
       redo[atomicAdd(nredo, 1)] = i;
     } else {
-      store_result(i, (uint8_t)st, g3, L, grad, lap, status);
+      store_result(out_perm ? out_perm[i] : i, (uint8_t)st, g3, L, grad, lap, status);
       slot = st;
     }
   }
@@ -165,7 +166,8 @@ __global__ __launch_bounds__(WAVE) void k_wls_redo(const double4* __restrict__ s
                                                    const int* __restrict__ nlist, double re, double l0, double reg,
                                                    double* __restrict__ grad, double* __restrict__ lap,
                                                    uint8_t* __restrict__ status,
-                                                   unsigned long long* __restrict__ counts) {
+                                                   unsigned long long* __restrict__ counts,
+                                                   const int* __restrict__ out_perm) {
   __shared__ double rec[RX_ROWS][RX_CHUNK][RX_REC];  // w, q[0..8], df, r
   __shared__ int okn[RX_ROWS][RX_CHUNK];
   __shared__ double acc[RX_ROWS][56];
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_redo(const double4* __restrict__ s
     const unsigned mk = (unsigned)(__builtin_amdgcn_ballot_w64(live && solved) >> (sub * RX_LANES)) & 0xFFu;
     const int winner = mk ? __builtin_ctz(mk) : 0;
     if (live && g == winner) {
-      store_result(i, st, g3, L, grad, lap, status);
+      store_result(out_perm ? out_perm[i] : i, st, g3, L, grad, lap, status);
       atomicAdd(&counts[st], 1ull);
       atomicAdd(&counts[WLS_REDO_SLOT], 1ull);
     }
@@ -399,7 +401,8 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
                                                     const int* __restrict__ cnt, int maxn, int n, double re, double l0,
                                                     double* __restrict__ grad, double* __restrict__ lap,
                                                     uint8_t* __restrict__ status, int* __restrict__ redo,
-                                                    int* __restrict__ nredo, unsigned* __restrict__ bpart) {
+                                                    int* __restrict__ nredo, unsigned* __restrict__ bpart,
+                                                    const int* __restrict__ out_perm) {
   __shared__ float sums[WLS_HALF * WLS_STRIDE];
   const int lane = threadIdx.x;
   const double inv_re = 1.0 / re;
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
         const uint8_t t = type[s];
         if (t == T_WALL || t == T_SPLASH) {
           const double g0[3] = {0.0, 0.0, 0.0};
-          store_result(s, WLS_SKIPPED, g0, 0.0, grad, lap, status);
+          store_result(out_perm ? out_perm[s] : s, WLS_SKIPPED, g0, 0.0, grad, lap, status);
           ++n_skip;
         } else {
           float sol[9];
@@ -463,7 +466,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
             // unscaled monomials times 2 l0 / re.
             const double g3[3] = {(double)sol[0], (double)sol[1], (double)sol[2]};
             const float quad = (sol[3] + sol[4] + sol[5]) * (2.0f * l0s);
-            store_result(s, WLS_OK, g3, (double)quad / l0, grad, lap, status);
+            store_result(out_perm ? out_perm[s] : s, WLS_OK, g3, (double)quad / l0, grad, lap, status);
             ++n_ok;
           } else {
             const int at = atomicAdd(nredo, 1);
@@ -556,30 +559,30 @@ void launch_field_quadratic(double4* spos, int n, const QuadField& q, hipStream_
 void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
                       int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin,
-                      hipEvent_t ev_end) {
+                      hipEvent_t ev_end, const int* out_perm) {
   if (n <= 0) return;
   const unsigned g = blocks(n, WAVE);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
   hipLaunchKernelGGL(k_wls_exact, dim3(g), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, grad, lap, status,
-                     redo, nredo, bpart);
+                     redo, nredo, bpart, out_perm);
   if (ev_end) (void)hipEventRecord(ev_end, st);
   CountSlots cs{WLS_REDO_SLOT, {0, 1, 2, 3, 4, 0, 0, 0}};
   launch_sum_counts(bpart, (int)g, cs, counts, st);
   // Rows whose first factorisation failed: the reference's ladder.
   const unsigned gr = blocks(n, RX_ROWS) < 1024u ? blocks(n, RX_ROWS) : 1024u;
   hipLaunchKernelGGL(k_wls_redo, dim3(gr), dim3(WAVE), 0, st, spos, nbr, cnt, maxn, redo, nredo, re, l0, reg, grad, lap,
-                     status, counts);
+                     status, counts, out_perm);
 }
 
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
                       int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,
-                      hipEvent_t ev_begin, hipEvent_t ev_end) {
+                      hipEvent_t ev_begin, hipEvent_t ev_end, const int* out_perm) {
   if (n <= 0) return;
   const unsigned gm = blocks(n, WAVE);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
   hipLaunchKernelGGL(k_wls_mixed, dim3(gm), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, grad, lap, status,
-                     redo, nredo, bpart);
+                     redo, nredo, bpart, out_perm);
   if (ev_end) (void)hipEventRecord(ev_end, st);
   CountSlots cs{2, {WLS_OK, WLS_SKIPPED, 0, 0, 0, 0, 0, 0}};
   launch_sum_counts(bpart, (int)gm, cs, counts, st);
@@ -587,7 +590,7 @@ void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, 
   // strides over it (typical lists are empty or a few thousand entries).
   const unsigned gr = blocks(n, RX_ROWS) < 1024u ? blocks(n, RX_ROWS) : 1024u;
   hipLaunchKernelGGL(k_wls_redo, dim3(gr), dim3(WAVE), 0, st, spos, nbr, cnt, maxn, redo, nredo, re, l0, reg, grad, lap,
-                     status, counts);
+                     status, counts, out_perm);
 }
 
 void launch_wls_sample_errors(const double4* spos, const uint8_t* type, const double* grad, const double* lap,

@@ -118,6 +118,7 @@ ABI = {
     "lsmps_set_field_quadratic": (C.c_int, [_p, C.POINTER(QuadField)]),
     "lsmps_set_field_values": (C.c_int, [_p, _p, _u64]),
     "lsmps_wls": (C.c_int, [_p, C.POINTER(WlsParams), C.POINTER(WlsResult)]),
+    "lsmps_wls_noalloc": (C.c_int, [_p, C.POINTER(WlsParams), _p, _p, _p, C.POINTER(WlsResult)]),
     "lsmps_wls_sampled_stats": (C.c_int, [_p, _i32, C.POINTER(WlsStats)]),
     "lsmps_assemble_ppe": (C.c_int, [_p, C.POINTER(PpeParams), C.POINTER(PpeResult)]),
     "lsmps_assemble_ppe_wls": (C.c_int, [_p, C.POINTER(PpeParams), C.c_double, C.c_double, C.POINTER(PpeResult)]),
