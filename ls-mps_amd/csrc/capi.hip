@@ -586,6 +586,42 @@ void sync_types(lsmps_ctx* ctx) {
   ctx->types_stale = false;
 }
 
+// Coordinate compression of a sparse grid. On each axis the distinct cell
+// coordinates of the particles, in order, get new coordinates: +1 after a
+// value whose successor is the next integer, +2 (one empty cell) after a
+// gap. Two particles' cells are neighbors (coordinates differing by at most
+// one on every axis) before exactly when they are after, and the order of
+// cells is kept, so the lists built on the compressed grid are the lists of
+// the full grid, in the same order. One far outlier then costs two cells per
+// axis instead of the whole box between it and the rest. The coordinates are
+// remapped on the host (a fallback path: one round trip of the cell
+// coordinates).
+int compress_grid(lsmps_ctx* ctx, int n, int64_t g[3]) {
+  std::vector<int> c[3];
+  DBuf<int>* dev[3] = {&ctx->cx, &ctx->cy, &ctx->cz};
+  for (int a = 0; a < 3; ++a) {
+    c[a].resize(n);
+    if (n) HIPCHK(hipMemcpyAsync(c[a].data(), dev[a]->p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  for (int a = 0; a < 3; ++a) {
+    std::vector<int> v(c[a]);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    std::vector<int> to(v.size());
+    int64_t at = 0;
+    for (size_t k = 0; k < v.size(); ++k) {
+      if (k) at += (int64_t)v[k] - v[k - 1] == 1 ? 1 : 2;
+      to[k] = (int)at;
+    }
+    g[a] = at + 1;
+    for (int& x : c[a]) x = to[std::lower_bound(v.begin(), v.end(), x) - v.begin()];
+    if (n) HIPCHK(hipMemcpyAsync(dev[a]->p, c[a].data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return LSMPS_OK;
+}
+
 // Cell sort + one neighbor pass (+ classification when classify).
 // re: classification radius; lists hold every pair within re + skin.
 int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool classify, double l0,
@@ -652,27 +688,44 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
     else
       bx[0] = bx[1] = bx[2] = 0;
   }
-  const int64_t ncells = g[0] * g[1] * g[2];
+  // Saturated at 2^62: the limit test below rejects it either way.
+  auto cells_of = [](const int64_t* e) {
+    const double c = (double)e[0] * (double)e[1] * (double)e[2];
+    return c >= 0x1p62 ? ((int64_t)1 << 62) : e[0] * e[1] * e[2];
+  };
+  int64_t ncells = cells_of(g);
+  // FP32 offsets are taken from the box's first cell; their error bound
+  // grows with the box's extent (in cells of size rl).
+  const int64_t ext_cells = std::max(g[0], std::max(g[1], g[2]));
+  ctx->ox = (double)bx[0] * rl;
+  ctx->oy = (double)bx[1] * rl;
+  ctx->oz = (double)bx[2] * rl;
   // Dense grid: bounded so a sparse outlier cannot blow up memory.
-  if (ncells > ((int64_t)1 << 30) || ncells > 64 * (int64_t)n + ((int64_t)1 << 24)) {
+  auto too_many = [n](int64_t c) { return c > ((int64_t)1 << 30) || c > 64 * (int64_t)n + ((int64_t)1 << 24); };
+  bool compressed = false;
+  if (too_many(ncells)) {
     if (spec) {
       // The kept grid was sized for an earlier (larger) particle count: the
       // exact box may still fit.
       ctx->spec_ok = false;
       return build_cell_list(ctx, re, skin, maxn, classify, l0, out, tout);
     }
-    return fail(ctx, LSMPS_E_LIMIT, "particle bounding box spans too many cells for the dense grid");
+    // A sparse box (outliers): compress the occupied coordinates per axis.
+    if (int rc = compress_grid(ctx, n, g)) return rc;
+    ncells = cells_of(g);
+    if (too_many(ncells))
+      return fail(ctx, LSMPS_E_LIMIT, "particle cells too scattered for the dense grid, even compressed per axis");
+    bx[0] = bx[1] = bx[2] = 0;
+    compressed = true;
   }
   ctx->gx = (int)g[0];
   ctx->gy = (int)g[1];
   ctx->gz = (int)g[2];
-  ctx->ox = (double)bx[0] * rl;
-  ctx->oy = (double)bx[1] * rl;
-  ctx->oz = (double)bx[2] * rl;
   HIPCHK(ctx->ccount.ensure((size_t)ncells));
   HIPCHK(ctx->cstart.ensure((size_t)ncells + 1));
   HIPCHK(ctx->scan_scr.ensure(std::max(scan_scratch_elems(ncells), scan_scratch_elems((int64_t)n + 1))));
-  const CnlBufs B = cnl_bufs(ctx);
+  CnlBufs B = cnl_bufs(ctx);
+  B.extent = rl * (double)(ext_cells + 1);
   const int fmode = !ctx->have_field ? FIELD_NONE : ctx->field_quadratic ? FIELD_QUADRATIC : FIELD_VALUES;
   launch_cell_sort(B, n, bx, fmode, ctx->qf, fmode == FIELD_VALUES ? ctx->f.p : nullptr, ctx->box.p + 6, st);
   ctx->spos_field = fmode != FIELD_NONE;
@@ -702,7 +755,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
       ctx->spec[3 + k] = (int)std::min<int64_t>(hi - ctx->spec[k] + 1, INT_MAX / 2);
     }
     const int64_t sc = (int64_t)ctx->spec[3] * ctx->spec[4] * ctx->spec[5];
-    ctx->spec_ok = sc <= ((int64_t)1 << 30) && sc <= 64 * (int64_t)n + ((int64_t)1 << 24);
+    ctx->spec_ok = !compressed && !too_many(sc);
     ctx->spec_rl = rl;
   } else {
     ctx->spec_ok = false;

@@ -631,7 +631,8 @@ void launch_cnl_pass(const CnlBufs& B, int n, double re2, double re2c, int maxn,
   const size_t lds = (size_t)(threads / CNL_GROUP) * (size_t)(maxn > 0 ? maxn : 1) * sizeof(int) +
                      (size_t)(threads + WAVE) * sizeof(float4);
   const unsigned grid = blocks(n, threads / CNL_GROUP);
-  const double extent = std::sqrt(re2) * (double)(std::max(B.gx, std::max(B.gy, B.gz)) + 1);
+  const double extent =
+      B.extent > 0.0 ? B.extent : std::sqrt(re2) * (double)(std::max(B.gx, std::max(B.gy, B.gz)) + 1);
   float lo, mid, hw, loc, midc, hwc;
   cnl_band_test(re2, extent, &lo, &mid, &hw);
   cnl_band_test(std::min(re2c, re2), extent, &loc, &midc, &hwc);

@@ -65,6 +65,7 @@ struct CnlBufs {
   double4* spos = nullptr;                                  // sorted {x, y, z, field}
   float4* sposf = nullptr;  // sorted {FP32 offsets from (ox, oy, oz), cell id as int bits}, n + CNL_PAD
   double ox = 0.0, oy = 0.0, oz = 0.0;                      // grid origin
+  double extent = 0.0;  // span of the particles' cells (FP32 error bound); 0: the grid's own
   int *nbr = nullptr, *cnt = nullptr, *cnt_orig = nullptr;  // ELL table
 };
 // Also type[i] = type_s_old[iperm_old[i]] for i < n_unsort (the previous

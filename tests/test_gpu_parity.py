@@ -377,3 +377,48 @@ def test_deterministic(pipe):
         outs.append((g.copy(), lap.copy(), st["iterations"], pipe.solution().copy()))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
     assert outs[0][2] == outs[1][2] and np.array_equal(outs[0][3], outs[1][3])
+
+
+def test_far_outliers_take_the_compressed_grid(pipe):
+    """Particles 10^6 re and 3-4 x 10^5 re away from the rest (a box far too
+    large for the dense grid): the build compresses the occupied cell
+    coordinates per axis and still gives the oracle's lists and types (the
+    reference hashes cells, so any extent works), and the FP64 WLS and the
+    PPE stay bit-exact; a second build on the same particles agrees. A cloud
+    whose cells are scattered on every axis is refused with LSMPS_E_LIMIT."""
+    l0 = 0.01
+    re = 2.1 * l0
+    xyz, t = L.generate_case("wall_film", 6000, 3, l0)
+    far = np.array([[1e6 * re, 0.0, 0.0], [-3e5 * re, 2e5 * re, 4e5 * re], [-3e5 * re, 2e5 * re, 4e5 * re + 0.5 * re]])
+    pts = np.concatenate([xyz, xyz[:1] + far])
+    ty = np.concatenate([t, np.zeros(3, np.uint8)])
+    say("far outliers", len(pts))
+    o = O.Oracle(pts, ty)
+    o.neighbors(re, 96)
+    orow, ocol = o.get_neighbors()
+    o.classify(l0)
+    og, ol, ost = o.wls_all(L.cases.field_values(pts, L.C1_FIELD), re, l0)
+    o.assemble(re, 1 << 30, 1, 3)
+    B = o.get_ppe()
+    pipe.upload(pts, ty)
+    for _ in range(2):
+        cs, _ = pipe.neighbors_classify(re, 96, l0)
+        row, col = pipe.neighbor_lists()
+        assert np.array_equal(row, orow) and np.array_equal(col, ocol)
+        assert np.array_equal(pipe.types(), o.types())
+    # The last two outliers are neighbors of each other only.
+    assert sorted(col[row[-2]:row[-1]].tolist()) == [len(pts) - 2]
+    pipe.set_field(L.C1_FIELD)
+    pipe.wls(re, l0, L.WLS_FP64)
+    g, lap, st = pipe.wls_results()
+    assert np.array_equal(st, ost) and np.array_equal(g, og) and np.array_equal(lap, ol)
+    pipe.assemble(re, 1 << 30, L.XSTAR_RANDOM, 3)
+    A = pipe.ppe()
+    for k in ("row", "col", "val", "b", "x_exact", "rev"):
+        assert np.array_equal(A[k], B[k]), k
+    rng = np.random.default_rng(2)
+    cloud = rng.uniform(0.0, 5e4 * re, (3000, 3))
+    pipe.upload(cloud, np.zeros(len(cloud), np.uint8))
+    with pytest.raises(L.LsmpsError) as e:
+        pipe.neighbors(re, 96)
+    assert "(-4)" in str(e.value)
