@@ -92,7 +92,18 @@ def assembly_bytes(n, m, nnz, k_mean):
     return n * 1 + m * (4 * k_mean + 4 + 24 + 4) + nnz * 12 + m * 24
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_cnl_pass_k_wls_mixed_industrial_mix_3M.txt")
+COUNTER_FILE = os.path.join(ROOT, "profiles", "counters_industrial_mix_3M.txt")
+# VALU issue rate of MI355X: 256 CUs, one wavefront instruction per CU and
+# cycle, 2.4 GHz.
+VALU_PEAK_PER_S = 256 * 2.4e9
+
+
+def _counter_lines(tag):
+    try:
+        with open(COUNTER_FILE) as f:
+            return [ln.split() for ln in f if ln.startswith(tag + " ")]
+    except OSError:
+        return []
 
 
 def measured_traffic(kernel, n):
@@ -100,15 +111,19 @@ def measured_traffic(kernel, n):
     (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter
     reads for a device copy of known size), or None when the summary is
     absent or was taken on another particle count."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            lines = [ln.split() for ln in f if ln.startswith("traffic ")]
-    except OSError:
-        return None
-    for fields in lines:
+    for fields in _counter_lines("traffic"):
         # traffic <kernel> <particles> <fetch_bytes> <write_bytes>
         if len(fields) == 5 and fields[1] == kernel and int(fields[2]) == n:
             return int(float(fields[3]) + float(fields[4]))
+    return None
+
+
+def measured_valu(kernel, n):
+    """VALU wavefront instructions per launch of `kernel` (SQ_INSTS_VALU)
+    from the committed counter summary, or None."""
+    for fields in _counter_lines("valu"):
+        if len(fields) == 4 and fields[1] == kernel and int(fields[2]) == n:
+            return int(fields[3])
     return None
 
 
@@ -410,6 +425,16 @@ def main():
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, n),
                      "kernel": dom, "kernel_ms": round(avg[dom], 4), "algorithmic_bytes": int(dom_bytes)},
     }
+    # The same kernels against the VALU issue rate (the neighbor pass and the
+    # WLS kernels are VALU-bound: their HBM fraction stays low by design).
+    valu = {}
+    for k in per_particle:
+        inst = measured_valu(k, n)
+        if inst and avg[k] > 0:
+            valu[k] = {"instructions": inst, "per_particle": round(inst / n, 1),
+                       "issue_frac": round(inst / (VALU_PEAK_PER_S * avg[k] * 1e-3), 4)}
+    if valu:
+        out["roofline"]["valu"] = valu
     if "aggregate" in extra:
         out["roofline"]["aggregate"] = extra.pop("aggregate")
     out.update(extra)
