@@ -15,8 +15,10 @@ unknowns, and the PPE solve to a true relative residual of 1e-8 with plain
 FP64 right-Jacobi BiCGStab, with mixed-precision BiCGStab (FP32 Krylov
 vectors and matrix, FP64 reliable updates), and with the learned start and
 diagonal inside the mixed solver (network weights from
-ls-mps_amd/weights/, trained by python -m lsmps_amd.learn), all on the same
-matrix and right-hand side (seeded x* ~ U(-1, 1)).
+ls-mps_amd/weights/mlp_industrial_mix.npy, trained by python -m
+lsmps_amd.learn --case industrial_mix), all on the same matrix, over five
+seeded right-hand sides x* ~ U(-1, 1) (medians reported), then once on the
+reference's x* = x^2 + y^2.
 
 The CPU baseline is the reference's own src/lsmps.cpp (built into
 oracle/_ref where the reference tree was present) running neighbors +
@@ -338,7 +340,12 @@ def main():
         solves = {}
         wanted = set(args.solvers.split(","))
         methods = [(k, m) for k, m in (("fp64_jacobi", L.SOLVE_JACOBI), ("mixed", L.SOLVE_MIXED)) if k in wanted]
-        weights = os.path.join(ROOT, "ls-mps_amd", "weights", "mlp_bulk_lattice.npy")
+        # Weights trained on solves of this scene family (industrial_mix,
+        # 100k-300k particles); the bulk-lattice weights otherwise.
+        weights = os.path.join(ROOT, "ls-mps_amd", "weights", "mlp_industrial_mix.npy")
+        if not os.path.exists(weights):
+            weights = os.path.join(ROOT, "ls-mps_amd", "weights", "mlp_bulk_lattice.npy")
+        extra["learned_weights"] = os.path.relpath(weights, ROOT)
         if os.path.exists(weights) and "learned_mixed" in wanted:
             from lsmps_amd import learn
 

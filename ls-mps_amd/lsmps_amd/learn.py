@@ -33,6 +33,9 @@ DEFAULT_WEIGHTS = os.path.join(WEIGHTS_DIR, "mlp_bulk_lattice.npy")
 L0 = 0.01
 RE = 2.1 * L0
 MAXN = 96
+# Scene families the network is trained and evaluated on: particle spacing
+# l0 and max_neighbors of each (re = 2.1 l0).
+SCENES = {"bulk_lattice": (0.01, 96), "industrial_mix": (0.004, 128)}
 
 
 def log(*a):
@@ -72,7 +75,9 @@ def scene_size(seed: int, lo: int = 100_000, hi: int = 300_000) -> int:
     return lo + (seed * 7919) % (hi - lo + 1)
 
 
-def prepare(pipe: Pipeline, case: str, n: int, seed: int, l0: float = L0, maxn: int = MAXN):
+def prepare(pipe: Pipeline, case: str, n: int, seed: int, l0: float = None, maxn: int = None):
+    if l0 is None or maxn is None:
+        l0, maxn = SCENES.get(case, (L0, MAXN))
     xyz, t = generate_case(case, n, seed, l0)
     pipe.upload(xyz, t)
     pipe.neighbors_classify(2.1 * l0, maxn, l0)
@@ -145,6 +150,10 @@ def summarize(rows):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--case", default="bulk_lattice", choices=sorted(SCENES),
+                    help="scene family of the training solves (and of the evaluation unless --eval-case)")
+    ap.add_argument("--eval-case", default=None, choices=sorted(SCENES))
+    ap.add_argument("--init", default=None, help="start from these weights instead of a fresh initialisation")
     ap.add_argument("--train-seeds", type=int, default=200)
     ap.add_argument("--eval-seeds", type=int, default=20)
     ap.add_argument("--eval-particles", type=int, default=1_000_000)
@@ -164,34 +173,44 @@ def main(argv=None):
             ap.error("--eval-only needs --report")
         params = load_params(args.out)
         pipe.set_params(params)
+        eval_case = args.eval_case or args.case
         eval_seeds = list(range(1001, 1001 + args.eval_seeds))
-        rows = evaluate(pipe, eval_seeds, args.eval_particles)
+        rows = evaluate(pipe, eval_seeds, args.eval_particles, case=eval_case)
         summary = summarize(rows)
         meta_path = os.path.splitext(args.out)[0] + ".json"
         meta = {}
         if os.path.exists(meta_path):
             with open(meta_path) as f:
                 meta = json.load(f)
-        meta["eval"] = {"seeds": eval_seeds, "particles": args.eval_particles, "tol": 1e-8, "summary": summary}
+        meta["eval"] = {"case": eval_case, "seeds": eval_seeds, "particles": args.eval_particles, "tol": 1e-8,
+                        "summary": summary}
+        # The weights' metadata carries the latest evaluation, so the two
+        # files never disagree.
+        with open(meta_path, "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
         with open(args.report, "w") as f:
             json.dump({"meta": meta, "rows": rows}, f, indent=1)
         log(f"evaluated {args.out}; summary {json.dumps(summary)}; {time.time() - t0:.0f} s")
         pipe.close()
         return
-    pipe.set_params(init_params(0))
+    pipe.set_params(load_params(args.init) if args.init else init_params(0))
     train_seeds = list(range(1, args.train_seeds + 1))
-    n_samples = collect(pipe, train_seeds, stride=args.stride)
+    n_samples = collect(pipe, train_seeds, case=args.case, stride=args.stride)
     log(f"training set: {n_samples} rows from {len(train_seeds)} scenes ({time.time() - t0:.0f} s)")
     curve = train(pipe, args.steps, args.batch, args.lr)
     params = pipe.get_params()
+    eval_case = args.eval_case or args.case
     eval_seeds = list(range(1001, 1001 + args.eval_seeds))
-    rows = evaluate(pipe, eval_seeds, args.eval_particles)
+    rows = evaluate(pipe, eval_seeds, args.eval_particles, case=eval_case)
     summary = summarize(rows)
-    meta = {"train": {"case": "bulk_lattice", "l0": L0, "re": RE, "max_neighbors": MAXN, "seeds": [1, args.train_seeds],
+    l0, maxn = SCENES[args.case]
+    meta = {"train": {"case": args.case, "l0": l0, "re": 2.1 * l0, "max_neighbors": maxn, "seeds": [1, args.train_seeds],
                       "particles": [100_000, 300_000], "stride": args.stride, "samples": n_samples,
                       "steps": args.steps, "batch": args.batch, "lr": args.lr, "solve": "FP64 Jacobi to 1e-10",
                       "loss_curve": curve},
-            "eval": {"seeds": eval_seeds, "particles": args.eval_particles, "tol": 1e-8, "summary": summary}}
+            "init": args.init,
+            "eval": {"case": eval_case, "seeds": eval_seeds, "particles": args.eval_particles, "tol": 1e-8,
+                     "summary": summary}}
     save_params(args.out, params, meta)
     if args.report:
         with open(args.report, "w") as f:
