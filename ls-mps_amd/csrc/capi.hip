@@ -55,7 +55,7 @@ struct DBuf {
 // pointers, so a cached graph is replayed only when every buffer those
 // kernels read or write (and every size they were launched with) is the
 // same as at capture.
-constexpr int GRAPH_KEY = 12;
+constexpr int GRAPH_KEY = 13;
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -149,6 +149,11 @@ struct lsmps_ctx {
   DBuf<int> sell_w, sell_ptr, sell_col;
   DBuf<double> sell_v64;
   uint64_t sell_nnz = 0;
+  // The columns packed to 16 bits for the solver's products, made on the
+  // first solve after an assembly.
+  DBuf<uint16_t> sell_pcol;
+  DBuf<int> sell_ptab;
+  bool packed_valid = false;
   // reference-order CSR and vectors, made on download / upload
   DBuf<int> rcnt, prow, pcol, rev;
   DBuf<double> pval, vrr;
@@ -446,6 +451,8 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->counters.release();
   ctx->bpart.release();
   ctx->sell_v32.release();
+  ctx->sell_pcol.release();
+  ctx->sell_ptab.release();
   ctx->sc.release();
   if (ctx->host_sc) (void)hipHostFree(ctx->host_sc);
   if (ctx->host_small) (void)hipHostFree(ctx->host_small);
@@ -1310,6 +1317,7 @@ int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0,
   ctx->ppe_re = p->re;
   ctx->nnz = m > 0 ? (uint64_t)c[2] : 0;
   ctx->have_ppe = true;
+  ctx->packed_valid = false;
   ctx->have_solution = ctx->have_x0 = ctx->have_features = false;
   if (out) {
     out->unknowns = (uint64_t)m;
@@ -1552,7 +1560,7 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
   // Every pointer the captured launches take (launch_solver_iteration): the
   // matrix, the vectors (one block, B.r first), the partials and scalars.
   const void* key[GRAPH_KEY] = {B.sptr, B.col, B.val, B.val64, B.b, B.scale, B.y, B.r, B.partials, ctx->sc.p,
-                                reinterpret_cast<const void*>((uintptr_t)ctx->sell_nnz), nullptr};
+                                reinterpret_cast<const void*>((uintptr_t)ctx->sell_nnz), B.pcol, B.ptab};
   GraphCache* hitp = nullptr;
   for (auto& c : ctx->graph)
     if (c.exec && c.m == B.m && c.chunk == chunk && c.with_replace == (int)with_replace &&
@@ -1626,6 +1634,25 @@ void bind_vectors(SolverBufs<T>& B, double* v, int m) {
   B.t = v + 5 * (size_t)m;
 }
 
+// The 16-bit columns of the current matrix (k_sell_pack), made once per
+// assembly; both solver precisions stream them in their products.
+int ensure_packed(lsmps_ctx* ctx) {
+  if (ctx->packed_valid) return LSMPS_OK;
+  const int ns = ppe_slices((int)ctx->m);
+  HIPCHK(ctx->sell_pcol.ensure(std::max<uint64_t>(ctx->sell_nnz, 1)));
+  HIPCHK(ctx->sell_ptab.ensure((size_t)std::max(ns, 1) * SELL_PACK_IDS));
+  launch_sell_pack(ctx->sell_ptr.p, ctx->sell_col.p, ns, ctx->sell_pcol.p, ctx->sell_ptab.p, ctx->st);
+  if (int rc = check_launch(ctx, "sell_pack")) return rc;
+  ctx->packed_valid = true;
+  return LSMPS_OK;
+}
+
+template <typename T>
+void bind_packed(lsmps_ctx* ctx, SolverBufs<T>& B) {
+  B.pcol = ctx->sell_pcol.p;
+  B.ptab = ctx->sell_ptab.p;
+}
+
 // FP64 right-Jacobi solve (the judge and fallback), optionally from x0.
 int solve_fp64_jacobi(lsmps_ctx* ctx, const double* x0, int max_iter, double tol, int chunk, lsmps_solve_result* out) {
   const int m = (int)ctx->m;
@@ -1645,6 +1672,8 @@ int solve_fp64_jacobi(lsmps_ctx* ctx, const double* x0, int max_iter, double tol
   B.y = ctx->yv.p;
   B.partials = ctx->partials.p;
   bind_vectors(B, ctx->vec64.p, m);
+  if (int rc = ensure_packed(ctx)) return rc;
+  bind_packed(ctx, B);
   return run_bicgstab<double>(ctx, B, x0, max_iter, tol, SOLVE_TRUE_CHECK | SOLVE_BREAK_RESTART, chunk, out);
 }
 
@@ -1670,6 +1699,8 @@ int solve_mixed(lsmps_ctx* ctx, const double* scale, const double* x0, bool prec
   B.y = ctx->yv.p;
   B.partials = ctx->partials.p;
   bind_vectors(B, ctx->vec64.p, m);
+  if (int rc = ensure_packed(ctx)) return rc;
+  bind_packed(ctx, B);
   const int flags = SOLVE_RELIABLE | SOLVE_TRUE_CHECK | SOLVE_RESTART | SOLVE_BREAK_RESTART;
   if (precheck && x0) {
     launch_solver_init<float>(B, x0, ctx->sc.p, max_iter, tol, flags, RELIABLE_DELTA, st);
@@ -1759,6 +1790,8 @@ int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_res
     B.y = ctx->yv.p;
     B.partials = ctx->partials.p;
     bind_vectors(B, ctx->vec64.p, m);
+    if (int r2 = ensure_packed(ctx)) return r2;
+    bind_packed(ctx, B);
     rc = run_bicgstab<double>(ctx, B, x0, p->max_iter, p->tol, SOLVE_S_EXIT, chunk, out);
   }
   if (rc != LSMPS_OK) return rc;

@@ -156,19 +156,80 @@ __device__ __forceinline__ double sell_dot(const int* __restrict__ sptr, const i
   return acc;
 }
 
+// The same product with 16-bit packed columns: entry p of the slice is
+// table[p >> 11] + (p & 2047), where the slice's table of up to 32 column
+// bases sits in lanes 0..31 of `tab` (k_sell_pack). Same products, same
+// order as sell_dot, so the same result; 2 bytes per entry instead of 4.
+template <typename TM>
+__device__ __forceinline__ double sell_dot_packed(const int* __restrict__ sptr, const uint16_t* __restrict__ pcol,
+                                                  int tab, const TM* __restrict__ val, const double* __restrict__ x,
+                                                  int row) {
+  const SellRow R = sell_row(sptr, row);
+  double acc = 0.0;
+  int k = 0;
+  // Four entries per step, loads first (the table exchange is a wavefront
+  // operation, so the compiler does not unroll this loop by itself).
+  for (; k + 4 <= R.width; k += 4) {
+    unsigned pk[4];
+    TM v[4];
+    double xv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = R.base + (k + j) * SLICE;
+      pk[j] = pcol[e];
+      v[j] = val[e];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      xv[j] = x[__shfl(tab, (int)(pk[j] >> SELL_PACK_OFF_BITS)) + (int)(pk[j] & SELL_PACK_OFF_MASK)];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc = fma((double)v[j], xv[j], acc);
+  }
+  for (; k < R.width; ++k) {
+    const int e = R.base + k * SLICE;
+    const unsigned pk = pcol[e];
+    const int c = __shfl(tab, (int)(pk >> SELL_PACK_OFF_BITS)) + (int)(pk & SELL_PACK_OFF_MASK);
+    acc = fma((double)val[e], x[c], acc);
+  }
+  return acc;
+}
+
+// One slice (64 rows) per wavefront and loop step, every lane active (the
+// last slice's rows past m are padding entries with value 0), so the packed
+// columns' table exchange sees the whole wavefront. Each lane still owns the
+// rows it owned row by row, so the partial sums keep their order.
+template <typename TM>
+__device__ __forceinline__ double sell_row_product(const int* __restrict__ sptr, const int* __restrict__ col,
+                                                   const uint16_t* __restrict__ pcol, const int* __restrict__ ptab,
+                                                   const TM* __restrict__ val, const double* __restrict__ x,
+                                                   int row) {
+  if (pcol) {
+    const int slice = row / SLICE, lane = threadIdx.x & (WAVE - 1);
+    const int tab = lane < SELL_PACK_IDS ? ptab[(size_t)slice * SELL_PACK_IDS + lane] : 0;
+    if (__shfl(tab, 0) >= 0) return sell_dot_packed(sptr, pcol, tab, val, x, row);
+  }
+  return sell_dot(sptr, col, val, x, row);  // unpacked, or a slice too scattered to pack
+}
+
 template <typename TM>
 __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_v(const int* __restrict__ sptr, const int* __restrict__ col,
-                                                           const TM* __restrict__ val, int m,
-                                                           const double* __restrict__ p, const double* __restrict__ r0,
-                                                           double* __restrict__ v, SolverScalars* sc,
-                                                           double* __restrict__ partials) {
+                                                           const uint16_t* __restrict__ pcol,
+                                                           const int* __restrict__ ptab, const TM* __restrict__ val,
+                                                           int m, const double* __restrict__ p,
+                                                           const double* __restrict__ r0, double* __restrict__ v,
+                                                           SolverScalars* sc, double* __restrict__ partials) {
   if (!iter_active(sc)) return;
   __shared__ double lds[SOLVER_THREADS / WAVE];
   double dot = 0.0;
-  for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
-    const double acc = sell_dot(sptr, col, val, p, row);
-    v[row] = acc;
-    dot += r0[row] * acc;
+  const int mpad = (m + SLICE - 1) / SLICE * SLICE;
+  for (int r0w = blockIdx.x * SOLVER_THREADS + (threadIdx.x & ~(WAVE - 1)); r0w < mpad;
+       r0w += gridDim.x * SOLVER_THREADS) {
+    const int row = r0w + (threadIdx.x & (WAVE - 1));
+    const double acc = sell_row_product(sptr, col, pcol, ptab, val, p, row);
+    if (row < m) {
+      v[row] = acc;
+      dot += r0[row] * acc;
+    }
   }
   double vv[1] = {dot};
   block_partials<1>(vv, partials, lds);
@@ -207,19 +268,26 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_s(int m, const double* 
 
 template <typename TM>
 __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_t(const int* __restrict__ sptr, const int* __restrict__ col,
-                                                           const TM* __restrict__ val, int m,
-                                                           const double* __restrict__ s, double* __restrict__ t,
-                                                           SolverScalars* sc, double* __restrict__ partials) {
+                                                           const uint16_t* __restrict__ pcol,
+                                                           const int* __restrict__ ptab, const TM* __restrict__ val,
+                                                           int m, const double* __restrict__ s,
+                                                           double* __restrict__ t, SolverScalars* sc,
+                                                           double* __restrict__ partials) {
   if (!iter_active(sc)) return;
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
   double ss = 0.0, tt = 0.0, ts = 0.0;
-  for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
-    const double acc = sell_dot(sptr, col, val, s, row);
-    const double si = s[row];
-    t[row] = acc;
-    ss += si * si;
-    tt += acc * acc;
-    ts += acc * si;
+  const int mpad = (m + SLICE - 1) / SLICE * SLICE;
+  for (int r0w = blockIdx.x * SOLVER_THREADS + (threadIdx.x & ~(WAVE - 1)); r0w < mpad;
+       r0w += gridDim.x * SOLVER_THREADS) {
+    const int row = r0w + (threadIdx.x & (WAVE - 1));
+    const double acc = sell_row_product(sptr, col, pcol, ptab, val, s, row);
+    if (row < m) {
+      const double si = s[row];
+      t[row] = acc;
+      ss += si * si;
+      tt += acc * acc;
+      ts += acc * si;
+    }
   }
   double vv[3] = {ss, tt, ts};
   block_partials<3>(vv, partials, lds);
@@ -485,6 +553,67 @@ __global__ void k_solver_setup(SolverScalars* sc, int max_iter, double tol, int 
   }
 }
 
+// ---- 16-bit packed columns of the iteration matrix ---------------------
+// Per slice (one wavefront): the distinct column blocks (col >> 11) of its
+// stored entries go into a 64-slot LDS hash table; when there are at most
+// 32, each gets an index and the slice's table holds their first columns,
+// and every entry becomes (index << 11) | (col & 2047). Cell-sorted rows
+// make a slice's columns fall into a few runs of neighboring cells (about
+// 10 blocks at 3M particles). A slice with more than 32 blocks keeps its
+// 32-bit columns: its table starts with -1.
+__device__ __forceinline__ unsigned pack_hash(int key) { return ((unsigned)key * 2654435761u) >> 26; }
+
+__global__ __launch_bounds__(256) void k_sell_pack(const int* __restrict__ sptr, const int* __restrict__ col,
+                                                   int nslices, uint16_t* __restrict__ pcol, int* __restrict__ ptab) {
+  __shared__ int keys[256 / WAVE][WAVE];
+  __shared__ int ids[256 / WAVE][WAVE];
+  const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const int slice = blockIdx.x * (256 / WAVE) + w;
+  if (slice >= nslices) return;  // whole wavefronts; no block barriers below
+  keys[w][lane] = -1;
+  wave_sync();
+  const int sb = sptr[slice], width = (sptr[slice + 1] - sb) / SLICE;
+  bool full = false;
+  for (int k = 0; k < width; ++k) {
+    const int key = col[sb + k * SLICE + lane] >> SELL_PACK_OFF_BITS;
+    unsigned h = pack_hash(key);
+    bool placed = false;
+    for (int probe = 0; probe < WAVE && !placed; ++probe) {
+      const int old = atomicCAS(&keys[w][h], -1, key);
+      placed = old == -1 || old == key;
+      h = (h + 1) & (WAVE - 1);
+    }
+    full = full || !placed;
+  }
+  wave_sync();
+  const bool occ = keys[w][lane] != -1;
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(occ);
+  const int id = __popcll(mask & ((1ull << lane) - 1ull));
+  const bool packable = !__builtin_amdgcn_ballot_w64(full) && __popcll(mask) <= SELL_PACK_IDS;
+  int* tab = ptab + (size_t)slice * SELL_PACK_IDS;
+  if (!packable) {
+    if (lane == 0) tab[0] = -1;
+    return;
+  }
+  ids[w][lane] = id;
+  if (occ) tab[id] = keys[w][lane] << SELL_PACK_OFF_BITS;
+  if (lane >= __popcll(mask) && lane < SELL_PACK_IDS) tab[lane] = 0;
+  wave_sync();
+  for (int k = 0; k < width; ++k) {
+    const int e = sb + k * SLICE + lane;
+    const int c = col[e], key = c >> SELL_PACK_OFF_BITS;
+    unsigned h = pack_hash(key);
+    for (int probe = 0; probe < WAVE && keys[w][h] != key; ++probe) h = (h + 1) & (WAVE - 1);
+    pcol[e] = (uint16_t)((ids[w][h] << SELL_PACK_OFF_BITS) | (c & SELL_PACK_OFF_MASK));
+  }
+}
+
+void launch_sell_pack(const int* sptr, const int* col, int nslices, uint16_t* pcol, int* ptab, hipStream_t st) {
+  if (nslices > 0)
+    hipLaunchKernelGGL(k_sell_pack, dim3((unsigned)((nslices + 3) / 4)), dim3(256), 0, st, sptr, col, nslices, pcol,
+                       ptab);
+}
+
 static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 unsigned solver_grid(int m) {
@@ -525,9 +654,11 @@ void launch_solver_iteration(const SolverBufs<TM>& B, bool with_replace, SolverS
   const unsigned g = solver_grid(m);
   const dim3 one(1), blk(SOLVER_THREADS), grid(g);
   hipLaunchKernelGGL(k_bicg_p, grid, blk, 0, st, m, B.r, B.v, B.p, sc);
-  hipLaunchKernelGGL((k_bicg_v<TM>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.p, B.r0, B.v, sc, B.partials);
+  hipLaunchKernelGGL((k_bicg_v<TM>), grid, blk, 0, st, B.sptr, B.col, B.pcol, B.ptab, B.val, m, B.p, B.r0, B.v, sc,
+                     B.partials);
   hipLaunchKernelGGL(k_bicg_s, grid, blk, 0, st, m, B.r, B.v, B.s, sc, (const double*)B.partials, (int)g);
-  hipLaunchKernelGGL((k_bicg_t<TM>), grid, blk, 0, st, B.sptr, B.col, B.val, m, B.s, B.t, sc, B.partials);
+  hipLaunchKernelGGL((k_bicg_t<TM>), grid, blk, 0, st, B.sptr, B.col, B.pcol, B.ptab, B.val, m, B.s, B.t, sc,
+                     B.partials);
   hipLaunchKernelGGL(k_bicg_xr, grid, blk, 0, st, m, B.p, B.s, B.t, B.r0, B.r, B.y, sc, B.partials, (int)g);
   hipLaunchKernelGGL(k_fin_iter, one, blk, 0, st, B.partials + 3 * (size_t)g, (int)g, sc);
   if (with_replace) {

@@ -48,6 +48,8 @@ struct SolverBufs {
   int m = 0;
   const int* sptr = nullptr;       // slice offsets, ceil(m/64) + 1 entries
   const int* col = nullptr;        // padded columns
+  const uint16_t* pcol = nullptr;  // the same columns packed to 16 bits (nullptr: use col)
+  const int* ptab = nullptr;       // per slice, SELL_PACK_IDS column bases (first -1: slice not packed)
   const TM* val = nullptr;         // A S in precision TM
   const double* val64 = nullptr;   // A in FP64 (true residuals), same layout
   const double* b = nullptr;
@@ -61,6 +63,10 @@ struct SolverBufs {
   double* t = nullptr;
   double* partials = nullptr;      // >= 5 * SOLVER_MAX_GRID doubles
 };
+
+// 16-bit packed columns: 5 bits of table index, 11 bits of offset.
+constexpr int SELL_PACK_OFF_BITS = 11, SELL_PACK_OFF_MASK = (1 << SELL_PACK_OFF_BITS) - 1, SELL_PACK_IDS = 32;
+void launch_sell_pack(const int* sptr, const int* col, int nslices, uint16_t* pcol, int* ptab, hipStream_t st);
 
 unsigned solver_grid(int m);
 // val = A S over all nnz stored entries (padding included), in precision T.
