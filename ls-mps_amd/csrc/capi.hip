@@ -145,7 +145,7 @@ struct lsmps_ctx {
   // Rows u are the unknowns in sorted-slot order, assembled straight into
   // sliced-ELL (sell_*); rr numbers them as the reference does.
   DBuf<int> flag, pos, fs, upos, uid, uslot, urr, rru, rlen;
-  DBuf<double> b, xe, diag;  // row order u
+  DBuf<double> b, xe, diag, xs;  // row order u
   DBuf<int> sell_w, sell_ptr, sell_col;
   DBuf<double> sell_v64;
   uint64_t sell_nnz = 0;
@@ -413,7 +413,7 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->sposf.release();
   DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x,   &ctx->y,     &ctx->z,      &ctx->f,        &ctx->grad_s,
                        &ctx->lap_s,  &ctx->grad, &ctx->lap, &ctx->ge,    &ctx->le,     &ctx->pval,     &ctx->vrr,
-                       &ctx->b,      &ctx->xe,   &ctx->diag, &ctx->scale, &ctx->yv,    &ctx->xsol,     &ctx->x0,
+                       &ctx->b,      &ctx->xe,   &ctx->diag, &ctx->xs, &ctx->scale, &ctx->yv,    &ctx->xsol,     &ctx->x0,
                        &ctx->vec64,  &ctx->partials, &ctx->sell_v64, &ctx->sell_vs64};
   for (auto* p : d) p->release();
   DBuf<int>* i[] = {&ctx->cx,    &ctx->cy,     &ctx->cz,    &ctx->box,      &ctx->cell,     &ctx->skey,
@@ -1257,6 +1257,7 @@ int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0,
   HIPCHK(ctx->fs.ensure(n));
   HIPCHK(ctx->upos.ensure((size_t)n + 1));
   HIPCHK(ctx->uid.ensure(n));
+  HIPCHK(ctx->xs.ensure(n));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
   // Reference numbering: unknowns by particle index, the first cap of them.
@@ -1282,7 +1283,7 @@ int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0,
   HIPCHK(ctx->sell_w.ensure(ns));
   HIPCHK(ctx->sell_ptr.ensure((size_t)ns + 1));
   launch_unknown_maps(ctx->perm.p, ctx->fs.p, ctx->upos.p, ctx->pos.p, n, ctx->uid.p, ctx->uslot.p, ctx->urr.p,
-                      ctx->rru.p, st);
+                      ctx->rru.p, ctx->spos.p, p->xmode, p->xseed, ctx->xs.p, st);
   launch_ppe_count(ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, m, !wls, ctx->sell_w.p,
                    st);
   HIPCHK(ctx->scan_scr.ensure(std::max(scan_scratch_elems((int64_t)n + 1), scan_scratch_elems((int64_t)ns + 1))));
@@ -1305,8 +1306,8 @@ int assemble_ppe(lsmps_ctx* ctx, const lsmps_ppe_params* p, bool wls, double l0,
                         ctx->sell_v64.p, ctx->rlen.p, ctx->diag.p, ctx->xe.p, ctx->b.p, ctx->counters.p, ctx->bpart.p,
                         st, ctx->e2, ctx->e3);
   else
-    launch_ppe_fill(ctx->spos.p, ctx->type_s.p, ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->nbr.p, ctx->cnt.p,
-                    ctx->maxn, m, p->re, p->shift, p->xmode, p->xseed, ctx->sell_ptr.p, ctx->sell_col.p,
+    launch_ppe_fill(ctx->spos.p, ctx->type_s.p, ctx->uslot.p, ctx->urr.p, ctx->uid.p, ctx->xs.p, ctx->nbr.p,
+                    ctx->cnt.p, ctx->maxn, m, p->re, p->shift, ctx->sell_ptr.p, ctx->sell_col.p,
                     ctx->sell_v64.p, ctx->rlen.p, ctx->diag.p, ctx->xe.p, ctx->b.p, ctx->counters.p, ctx->bpart.p, st,
                     ctx->e2, ctx->e3, replay, wseed, l0);
   if (int rc = check_launch(ctx, "ppe_fill")) return rc;

@@ -124,14 +124,15 @@ int ppe_slices(int m);
 void launch_unknown_flags(const uint8_t* type_s, const int* iperm, int n, int* flag, hipStream_t st);
 void launch_unknown_slots(const int* perm, const int* flag, const int* pos, int n, int cap, int* fs, hipStream_t st);
 void launch_unknown_maps(const int* perm, const int* fs, const int* upos, const int* pos, int n, int* uid, int* uslot,
-                         int* urr, int* rru, hipStream_t st);
+                         int* urr, int* rru, const double4* spos, int xmode, uint32_t xseed, double* xs,
+                         hipStream_t st);
 // pin0: row 0 is the identity (graph-Laplacian rows), so its width is 1.
 void launch_ppe_count(const int* uslot, const int* urr, const int* uid, const int* nbr, const int* ncnt, int maxn,
                       int m, bool pin0, int* wsz, hipStream_t st);
 // replay: edges of a replay graph with synthetic weights (replay.hip).
 void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr, const int* uid,
-                     const int* nbr, const int* ncnt, int maxn, int m, double re, double shift, int xmode,
-                     uint32_t xseed, const int* sptr, int* col, double* val, int* rlen, double* diag, double* xe,
+                     const double* xs, const int* nbr, const int* ncnt, int maxn, int m, double re, double shift,
+                     const int* sptr, int* col, double* val, int* rlen, double* diag, double* xe,
                      double* b, unsigned long long* counters, unsigned* bpart, hipStream_t st,
                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, bool replay = false,
                      uint32_t wseed = 0, double l0 = 0.0);

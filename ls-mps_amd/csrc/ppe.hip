@@ -21,13 +21,13 @@
 //
 // Kernels: unknown flags over particle index, a scan (reference numbering
 // rr), flags over sorted slots, a scan (sorted numbering u), the maps
-// between them; then a count pass (unknown neighbors per row -> slice
-// widths) and the fill pass. Count and fill run one 512-thread block per
-// slice, eight lanes per row: the lanes walk the row's ELL neighbor list
-// eight entries at a time, and ordered ballots give each kept entry its
-// position. The diagonal and b are sums in list order, accumulated by
-// broadcasting each kept entry to the row's lanes in order, so they are
-// bit-identical to the reference (this file is built with
+// between them and x* per unknown slot; then a count pass (unknown
+// neighbors per row -> slice widths) and the fill pass. Count and fill run
+// one 512-thread block per slice, eight lanes per row: the lanes walk the
+// row's ELL neighbor list eight entries at a time, and ordered ballots give
+// each kept entry its position. The diagonal and b are sums in list order
+// (each batch's terms staged in LDS and added by one lane per sum), so they
+// are bit-identical to the reference (this file is built with
 // -ffp-contract=off).
 //
 // WLS-Laplacian rows (k_ppe_wls_fill). The reference's C++ has only the
@@ -61,6 +61,18 @@ constexpr int PPE_SLICE = 64;
 
 __device__ __forceinline__ bool is_unknown_type(uint8_t t) { return t == T_INTERIOR || t == T_NEARWALL; }
 
+// Seeded manufactured solution x*_k ~ U(-1, 1): the top 53 bits of the
+// splitmix64 finaliser of seed * 2^64/phi + k, scaled to [0, 1) and mapped
+// to [-1, 1). Counter-based on the reference row k, so every row draws its
+// value independently of the layout.
+__device__ __forceinline__ double xstar_random(uint32_t seed, int k) {
+  uint64_t z = (uint64_t)seed * 0x9E3779B97F4A7C15ull + (uint64_t)k;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return -1.0 + 2.0 * ((double)(z >> 11) * 0x1.0p-53);
+}
+
 // flag[i] over particle index i, from the sorted-order types.
 __global__ __launch_bounds__(256) void k_unknown_flags(const uint8_t* __restrict__ type_s,
                                                        const int* __restrict__ iperm, int n, int* __restrict__ flag) {
@@ -79,11 +91,15 @@ __global__ __launch_bounds__(256) void k_unknown_slots(const int* __restrict__ p
 }
 
 // uid[s] = row u of slot s or -1; uslot[u] = s; urr[u] = reference row;
-// rru[rr] = u.
+// rru[rr] = u; xs[s] = x* of the unknown at slot s (unset for other slots:
+// the fill reads it for kept entries only), so the fill gathers one value
+// per entry instead of hashing the reference row of every neighbor.
 __global__ __launch_bounds__(256) void k_unknown_maps(const int* __restrict__ perm, const int* __restrict__ fs,
                                                       const int* __restrict__ upos, const int* __restrict__ pos, int n,
                                                       int* __restrict__ uid, int* __restrict__ uslot,
-                                                      int* __restrict__ urr, int* __restrict__ rru) {
+                                                      int* __restrict__ urr, int* __restrict__ rru,
+                                                      const double4* __restrict__ spos, int xmode, uint32_t xseed,
+                                                      double* __restrict__ xs) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   if (!fs[s]) {
@@ -91,6 +107,12 @@ __global__ __launch_bounds__(256) void k_unknown_maps(const int* __restrict__ pe
     return;
   }
   const int u = upos[s], rr = pos[perm[s]];
+  if (xmode == 0) {
+    const double4 p = spos[s];
+    xs[s] = p.x * p.x + p.y * p.y;  // x* = x^2 + y^2 (src/lsmps.cpp:29)
+  } else {
+    xs[s] = xstar_random(xseed, rr);
+  }
   uid[s] = u;
   uslot[u] = s;
   urr[u] = rr;
@@ -106,18 +128,6 @@ __device__ __forceinline__ bool pair_coef(double dx, double dy, double dz, doubl
   if (w <= 0.0) return false;
   *a = w / (r * r);
   return true;
-}
-
-// Seeded manufactured solution x*_k ~ U(-1, 1): the top 53 bits of the
-// splitmix64 finaliser of seed * 2^64/phi + k, scaled to [0, 1) and mapped
-// to [-1, 1). Counter-based on the reference row k, so every row draws its
-// value independently of the layout.
-__device__ __forceinline__ double xstar_random(uint32_t seed, int k) {
-  uint64_t z = (uint64_t)seed * 0x9E3779B97F4A7C15ull + (uint64_t)k;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z ^= z >> 31;
-  return -1.0 + 2.0 * ((double)(z >> 11) * 0x1.0p-53);
 }
 
 // Synthetic weight of the replay edge {lo, hi} (lo <= hi, particle indices):
@@ -202,8 +212,8 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_count(const int* __restrict
 template <bool REPLAY>
 __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
     const double4* __restrict__ spos, const uint8_t* __restrict__ type_s, const int* __restrict__ uslot,
-    const int* __restrict__ urr, const int* __restrict__ uid, const int* __restrict__ nbr,
-    const int* __restrict__ ncnt, int maxn, int m, double re, double shift, int xmode, uint32_t xseed,
+    const int* __restrict__ urr, const int* __restrict__ uid, const double* __restrict__ xs,
+    const int* __restrict__ nbr, const int* __restrict__ ncnt, int maxn, int m, double re, double shift,
     uint32_t wseed, double l0, const int* __restrict__ sptr, int* __restrict__ col, double* __restrict__ val,
     int* __restrict__ rlen, double* __restrict__ diag, double* __restrict__ xe, double* __restrict__ b,
     unsigned* __restrict__ bpart) {
@@ -220,8 +230,7 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
     const int sb = sptr[R.slice];
     const int width = (sptr[R.slice + 1] - sb) / PPE_SLICE;
     const int base = sb + (u - R.slice * PPE_SLICE);
-    const double xu = xmode == 0 ? pi.x * pi.x + pi.y * pi.y  // x* = x^2 + y^2 (src/lsmps.cpp:29)
-                                 : xstar_random(xseed, rr);
+    const double xu = xs[s];
     double acc = 0.0;  // lane 0: the diagonal; lane 1: b (both in list order)
     int kept = 0, below_u = 0;
     if (rr != 0) {
@@ -231,6 +240,7 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
       for (int c0 = 0; c0 < k; c0 += PPE_BATCH_ENTRIES) {
         int s2[PPE_BATCH], cj[PPE_BATCH];
         double4 pj[PPE_BATCH];
+        double xjs[PPE_BATCH];
 #pragma unroll
         for (int j = 0; j < PPE_BATCH; ++j) {
           const int q = c0 + j * PPE_GROUP + R.g;
@@ -238,6 +248,8 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
         }
 #pragma unroll
         for (int j = 0; j < PPE_BATCH; ++j) cj[j] = s2[j] >= 0 ? uid[s2[j]] : -1;
+#pragma unroll
+        for (int j = 0; j < PPE_BATCH; ++j) xjs[j] = s2[j] >= 0 ? xs[s2[j]] : 0.0;
         if (!REPLAY) {
 #pragma unroll
           for (int j = 0; j < PPE_BATCH; ++j) pj[j] = cj[j] >= 0 ? spos[s2[j]] : pi;
@@ -251,11 +263,11 @@ __global__ __launch_bounds__(PPE_THREADS) void k_ppe_fill(
               keep = s2[j] != s;
               if (keep) {
                 a = replay_weight(wseed, s < s2[j] ? s : s2[j], s < s2[j] ? s2[j] : s, l0);
-                xj = xstar_random(xseed, urr[cj[j]]);
+                xj = xjs[j];
               }
             } else {
               keep = pair_coef(pj[j].x - pi.x, pj[j].y - pi.y, pj[j].z - pi.z, re, &a);
-              if (keep) xj = xmode == 0 ? pj[j].x * pj[j].x + pj[j].y * pj[j].y : xstar_random(xseed, urr[cj[j]]);
+              if (keep) xj = xjs[j];
             }
           }
           const unsigned gm = group8_ballot(keep, R.gbase);
@@ -565,10 +577,11 @@ void launch_unknown_slots(const int* perm, const int* flag, const int* pos, int 
   if (n > 0) hipLaunchKernelGGL(k_unknown_slots, dim3(blocks(n, 256)), dim3(256), 0, st, perm, flag, pos, n, cap, fs);
 }
 void launch_unknown_maps(const int* perm, const int* fs, const int* upos, const int* pos, int n, int* uid, int* uslot,
-                         int* urr, int* rru, hipStream_t st) {
+                         int* urr, int* rru, const double4* spos, int xmode, uint32_t xseed, double* xs,
+                         hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_unknown_maps, dim3(blocks(n, 256)), dim3(256), 0, st, perm, fs, upos, pos, n, uid, uslot, urr,
-                       rru);
+                       rru, spos, xmode, xseed, xs);
 }
 void launch_ppe_count(const int* uslot, const int* urr, const int* uid, const int* nbr, const int* ncnt, int maxn,
                       int m, bool pin0, int* wsz, hipStream_t st) {
@@ -577,19 +590,19 @@ void launch_ppe_count(const int* uslot, const int* urr, const int* uid, const in
                        maxn, m, pin0 ? 1 : 0, wsz);
 }
 void launch_ppe_fill(const double4* spos, const uint8_t* type_s, const int* uslot, const int* urr, const int* uid,
-                     const int* nbr, const int* ncnt, int maxn, int m, double re, double shift, int xmode,
-                     uint32_t xseed, const int* sptr, int* col, double* val, int* rlen, double* diag, double* xe,
+                     const double* xs, const int* nbr, const int* ncnt, int maxn, int m, double re, double shift,
+                     const int* sptr, int* col, double* val, int* rlen, double* diag, double* xe,
                      double* b, unsigned long long* counters, unsigned* bpart, hipStream_t st, hipEvent_t ev_begin,
                      hipEvent_t ev_end, bool replay, uint32_t wseed, double l0) {
   if (m <= 0) return;
   const int g = ppe_fill_blocks(m);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
   if (replay)
-    hipLaunchKernelGGL(k_ppe_fill<true>, dim3(g), dim3(PPE_THREADS), 0, st, spos, type_s, uslot, urr, uid, nbr, ncnt,
-                       maxn, m, re, shift, xmode, xseed, wseed, l0, sptr, col, val, rlen, diag, xe, b, bpart);
+    hipLaunchKernelGGL(k_ppe_fill<true>, dim3(g), dim3(PPE_THREADS), 0, st, spos, type_s, uslot, urr, uid, xs, nbr,
+                       ncnt, maxn, m, re, shift, wseed, l0, sptr, col, val, rlen, diag, xe, b, bpart);
   else
-    hipLaunchKernelGGL(k_ppe_fill<false>, dim3(g), dim3(PPE_THREADS), 0, st, spos, type_s, uslot, urr, uid, nbr, ncnt,
-                       maxn, m, re, shift, xmode, xseed, 0u, 0.0, sptr, col, val, rlen, diag, xe, b, bpart);
+    hipLaunchKernelGGL(k_ppe_fill<false>, dim3(g), dim3(PPE_THREADS), 0, st, spos, type_s, uslot, urr, uid, xs, nbr,
+                       ncnt, maxn, m, re, shift, 0u, 0.0, sptr, col, val, rlen, diag, xe, b, bpart);
   if (ev_end) (void)hipEventRecord(ev_end, st);
   launch_sum_counts(bpart, g, CountSlots{3, {0, 1, 2, 0, 0, 0, 0, 0}}, counters, st);
 }
