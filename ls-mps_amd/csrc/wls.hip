@@ -47,6 +47,21 @@ __device__ __forceinline__ void store_result(int i, uint8_t st, const double* g3
   status[i] = st;
 }
 
+// RN(a / b) from y = RN(1 / b) (Markstein's theorem): q0 = RN(a y) is
+// within one ulp of a/b, the remainder b q0 - a is exact in one FMA, and
+// RN(q0 - (b q0 - a) y) is the correctly rounded quotient; negating the
+// remainder keeps the sign of a zero quotient. It holds while the remainder
+// is clear of underflow and the quotient of overflow: `off` is set when
+// |q0| leaves [2^-900, 2^900] for a nonzero a (NaN included), and the
+// caller then must not use the result.
+__device__ __forceinline__ double div_rn(double a, double b, double y, bool& off) {
+  const double q0 = a * y;
+  const double e = fma(b, q0, -a);
+  const double aq = fabs(q0);
+  off = off || (!(aq >= 0x1p-900 && aq <= 0x1p900) && a != 0.0);
+  return fma(-e, y, q0);
+}
+
 // FP64 WLS of one particle per lane, in the reference's operation order:
 // the moments and right-hand sides are accumulated neighbor by neighbor
 // (each neighbor's record is loaded one neighbor ahead and its index two
@@ -55,10 +70,13 @@ __device__ __forceinline__ void store_result(int i, uint8_t st, const double* g3
 // factorisation fails goes to the work list of k_wls_redo, which runs the
 // reference's whole ladder with the same results bit for bit; keeping the
 // ladder out of this kernel takes its register count from 248 to 200.
-// Measured at 3M (industrial_mix): 1.67 ms; forcing three wavefronts per
-// SIMD (168 VGPRs) 1.75 ms, four (128 VGPRs, spills) 7.2 ms. Counters put
-// it at ~500 VALU instructions per neighbor pair, most of them the
-// correctly rounded FP64 divisions the reference's expressions need.
+// The reference's divisions by r, 2 l0 r, l0 r and the weight sum are
+// correctly rounded quotients from one reciprocal each (div_rn): 14 of the
+// 17 divisions per neighbor pair and the 54 per particle become three
+// instructions each, 1.70 -> 1.41 ms at 3M (industrial_mix), the same
+// results bit for bit; a row with a quotient div_rn cannot vouch for goes
+// to k_wls_redo, which divides. Forcing three wavefronts per SIMD (168
+// VGPRs) measured 1.75 ms against 1.67, four (spills) 7.2 ms.
 __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ spos,
                                                     const uint8_t* __restrict__ type, const int* __restrict__ nbr,
                                                     const int* __restrict__ cnt, int maxn, int n, double re, double l0,
@@ -78,6 +96,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
       const int kn = cnt[i];
       const double h2 = 2 * l0;
       double M[45], B[9], ws = 0.0;
+      bool off = false;  // a div_rn quotient out of its range: divide in k_wls_redo
 #pragma unroll
       for (int k = 0; k < 45; ++k) M[k] = 0.0;
 #pragma unroll
@@ -91,26 +110,27 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
         const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         if (r <= 1e-12 || r >= re) continue;
-        const double tq = re / r - 1.0;
+        const double inv = 1.0 / r, h = h2 * r, g = l0 * r;
+        const double tq = div_rn(re, r, inv, off) - 1.0;  // re / r >= 1: in range
         const double w = tq * tq;
         if (w == 0.0) continue;
-        const double inv = 1.0 / r, h = h2 * r, g = l0 * r;
+        const double yh = 1.0 / h, yg = 1.0 / g;
         double q[9];
         q[0] = dx * inv;
         q[1] = dy * inv;
         q[2] = dz * inv;
-        q[3] = dx * dx / h;
-        q[4] = dy * dy / h;
-        q[5] = dz * dz / h;
-        q[6] = dx * dy / g;
-        q[7] = dx * dz / g;
-        q[8] = dy * dz / g;
+        q[3] = div_rn(dx * dx, h, yh, off);
+        q[4] = div_rn(dy * dy, h, yh, off);
+        q[5] = div_rn(dz * dz, h, yh, off);
+        q[6] = div_rn(dx * dy, g, yg, off);
+        q[7] = div_rn(dx * dz, g, yg, off);
+        q[8] = div_rn(dy * dz, g, yg, off);
         const double df = pj.w - pi.w;
         ws += w;
 #pragma unroll
         for (int a = 0; a < 9; ++a) {
           const double wq = w * q[a];
-          B[a] += wq * df / r;
+          B[a] += div_rn(wq * df, r, inv, off);
 #pragma unroll
           for (int c = 0; c <= a; ++c) M[tri(a, c)] += wq * q[c];
         }
@@ -118,12 +138,13 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
       if (ws <= 0) {
         st = WLS_NO_WEIGHT;
       } else {
+        const double yw = 1.0 / ws;
 #pragma unroll
-        for (int k = 0; k < 45; ++k) M[k] /= ws;
+        for (int k = 0; k < 45; ++k) M[k] = div_rn(M[k], ws, yw, off);
 #pragma unroll
-        for (int a = 0; a < 9; ++a) B[a] /= ws;
+        for (int a = 0; a < 9; ++a) B[a] = div_rn(B[a], ws, yw, off);
         const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (chol9<double>(M, 1e-14, 0.0, zero9)) {
+        if (!off && chol9<double>(M, 1e-14, 0.0, zero9)) {
           double sol[9];
           chol9_solve<double>(M, B, sol);
           g3[0] = sol[0];
@@ -132,7 +153,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
           L = (sol[3] + sol[4] + sol[5]) / l0;
           st = WLS_OK;
         } else {
-          st = -1;  // the ladder: k_wls_redo writes this row
+          st = -1;  // the ladder (or plain divisions): k_wls_redo writes this row
         }
       }
     }

@@ -194,6 +194,38 @@ def test_wls_other_field_and_mixed(pipe):
     assert np.sqrt(np.mean((l32[okm] - 4.0) ** 2)) < 5e-2
 
 
+def test_fp64_wls_tiny_quotients_take_plain_divisions(pipe):
+    """The FP64 WLS forms its quotients from one reciprocal per divisor and
+    sends a row whose quotient is too small for that (|q| < 2^-900) to the
+    FP64 redo kernel, which divides. A field of 1e-300 times the C1 values
+    on half the block (x below the median) puts every right-hand-side
+    quotient of those rows there: those rows are redone, the others are
+    not, and every row equals the oracle's bit for bit."""
+    xyz, t = L.generate_case("bulk_lattice", 20000, 3, 0.01)
+    vals = L.cases.field_values(xyz, L.C1_FIELD)
+    tiny = xyz[:, 0] < np.median(xyz[:, 0])
+    vals = np.where(tiny, vals * 1e-300, vals)
+    say("tiny quotients", len(xyz), int(tiny.sum()))
+    pipe.upload(xyz, t)
+    pipe.neighbors(0.021, 96)
+    pipe.classify(0.01)
+    pipe.set_field(vals)
+    r64 = pipe.wls(0.021, 0.01, L.WLS_FP64)
+    g64, l64, s64 = pipe.wls_results()
+    o = O.Oracle(xyz, t)
+    o.neighbors(0.021, 96)
+    o.classify(0.01)
+    og, ol, ost = o.wls_all(vals, 0.021, 0.01)
+    say("redone", r64["fp64_redo"])
+    assert np.array_equal(ost, s64)
+    assert np.array_equal(og, g64) and np.array_equal(ol, l64)
+    solved = s64 <= L.WLS_REGULARIZED
+    assert int((solved & tiny).sum()) > 1000
+    assert int((solved & tiny).sum()) <= r64["fp64_redo"] < int(solved.sum())
+    deep = xyz[:, 0] < np.median(xyz[:, 0]) - 0.05  # neighbors all in the scaled half
+    assert np.all(np.abs(l64[solved & deep]) < 1e-290)
+
+
 def test_mixed_wls_redo_rows_equal_fp64(pipe):
     """Rows the mixed WLS redoes in FP64 (eight lanes per row, the five
     regularisation steps side by side) carry exactly the FP64 kernel's
