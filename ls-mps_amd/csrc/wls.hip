@@ -432,7 +432,8 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
   const int s0 = (int)xcd_block(blockIdx.x, gridDim.x) * WAVE;
   int n_ok = 0, n_skip = 0;
   for (int half = 0; half < WAVE / WLS_HALF; ++half) {
-    // phase 1: 8 lanes per particle, 8 particles per pass, 4 passes.
+    // phase 1: WLS_GROUP lanes per particle, WAVE / WLS_GROUP particles per
+    // pass, WLS_HALF / (WAVE / WLS_GROUP) passes (one with two lanes).
     const int g = lane % WLS_GROUP, sub = lane / WLS_GROUP;
     for (int pass = 0; pass < WLS_HALF / (WAVE / WLS_GROUP); ++pass) {
       const int local = pass * (WAVE / WLS_GROUP) + sub;  // 0..31 within the half
