@@ -671,13 +671,18 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   HIPCHK(ctx->hist.ensure((size_t)maxn + 1 + (size_t)(maxn + 6) * CNL_STATS_BLOCKS));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
-  launch_cell_coords(ctx->x.p, ctx->y.p, ctx->z.p, n, rl, ctx->cx.p, ctx->cy.p, ctx->cz.p, ctx->box.p, ctx->iperm.p,
-                     ctx->type_s.p, n_unsort, ctx->type.p, st);
-  ctx->types_stale = false;
-  if (int rc = check_launch(ctx, "cell_coords")) return rc;
+  // With the last build's grid the key pass makes the cell coordinates and
+  // the box itself (launch_cell_sort with CellKeyDirect); otherwise the box
+  // comes first.
+  const bool spec = n > 0 && ctx->spec_ok && ctx->spec_rl == rl;
+  if (!spec) {
+    launch_cell_coords(ctx->x.p, ctx->y.p, ctx->z.p, n, rl, ctx->cx.p, ctx->cy.p, ctx->cz.p, ctx->box.p,
+                       ctx->iperm.p, ctx->type_s.p, n_unsort, ctx->type.p, st);
+    ctx->types_stale = false;
+    if (int rc = check_launch(ctx, "cell_coords")) return rc;
+  }
   int bx[6];
   int64_t g[3] = {1, 1, 1};
-  const bool spec = n > 0 && ctx->spec_ok && ctx->spec_rl == rl;
   if (spec) {
     // The grid of the last build (its box plus a margin), checked on the
     // device by k_cell_key.
@@ -734,7 +739,17 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   CnlBufs B = cnl_bufs(ctx);
   B.extent = rl * (double)(ext_cells + 1);
   const int fmode = !ctx->have_field ? FIELD_NONE : ctx->field_quadratic ? FIELD_QUADRATIC : FIELD_VALUES;
-  launch_cell_sort(B, n, bx, fmode, ctx->qf, fmode == FIELD_VALUES ? ctx->f.p : nullptr, ctx->box.p + 6, st);
+  CellKeyDirect dk;
+  if (spec) {
+    dk.rl = rl;
+    dk.iperm_old = ctx->iperm.p;
+    dk.type_s_old = ctx->type_s.p;
+    dk.n_unsort = n_unsort;
+    dk.type = ctx->type.p;
+  }
+  launch_cell_sort(B, n, bx, fmode, ctx->qf, fmode == FIELD_VALUES ? ctx->f.p : nullptr, ctx->box.p,
+                   spec ? &dk : nullptr, st);
+  if (spec) ctx->types_stale = false;
   ctx->spos_field = fmode != FIELD_NONE;
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
   zero_counters(ctx, 16);

@@ -118,10 +118,14 @@ __global__ void k_fill_int(int* __restrict__ p, int64_t n, int v) {
 //                lattice order, so consecutive particles share a cell: a run
 //                of equal cells among consecutive lanes takes its slots with
 //                one atomic from its first lane (about half the atomics).
+//                With the grid known in advance (the last build's box plus a
+//                margin), k_cell_key_direct computes the cell coordinates
+//                and the box in the same pass (no k_cell_coords).
 //  (scan)        exclusive scan of the counts: the cells' first slots.
 //  k_scatter     particle and cell id to start + slot (arrival order).
-//  k_cell_rank   each slot's rank among its cell's particle indices, so
-//                perm holds every cell's particles in ascending index order.
+//  k_rank_gather each slot's rank among its cell's particle indices, so
+//                perm holds every cell's particles in ascending index order,
+//                and the sorted copies of the particle at its final slot.
 __global__ __launch_bounds__(256) void k_cell_key(const int* __restrict__ cx, const int* __restrict__ cy,
                                                   const int* __restrict__ cz, int n, int bx, int by, int bz,
                                                   int gx, int gy, int gz, int* __restrict__ cell,
@@ -161,6 +165,80 @@ __global__ __launch_bounds__(256) void k_cell_key(const int* __restrict__ cx, co
   }
 }
 
+// k_cell_coords and k_cell_key in one pass for a grid known in advance
+// (first cell bx, by, bz; extent gx, gy, gz): the cell coordinates
+// (int)floor(x / rl), their bounding box (one atomic per block and axis),
+// the type unsort of k_cell_coords, and the key and slot of k_cell_key.
+// Blocks stride over chunks of 256 consecutive particles, so the lanes of
+// a wavefront hold consecutive particles, as k_cell_key's runs need.
+__global__ __launch_bounds__(256) void k_cell_key_direct(const double* __restrict__ x, const double* __restrict__ y,
+                                                         const double* __restrict__ z, int n, double rl, int bx,
+                                                         int by, int bz, int gx, int gy, int gz,
+                                                         int* __restrict__ cell, int* __restrict__ slot,
+                                                         int* __restrict__ count, int* __restrict__ box,
+                                                         const int* __restrict__ iperm_old,
+                                                         const uint8_t* __restrict__ type_s_old, int n_unsort,
+                                                         uint8_t* __restrict__ type) {
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  const int lane = threadIdx.x & (WAVE - 1);
+  for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+    const int i = base + threadIdx.x;
+    const bool valid = i < n;
+    int c = -1;
+    if (valid) {
+      const int a = (int)floor(x[i] / rl), b = (int)floor(y[i] / rl), w0 = (int)floor(z[i] / rl);
+      lo[0] = min(lo[0], a);
+      hi[0] = max(hi[0], a);
+      lo[1] = min(lo[1], b);
+      hi[1] = max(hi[1], b);
+      lo[2] = min(lo[2], w0);
+      hi[2] = max(hi[2], w0);
+      if (i < n_unsort) type[i] = type_s_old[iperm_old[i]];
+      const int u = a - bx, v = b - by, w = w0 - bz;
+      if ((unsigned)u < (unsigned)gx && (unsigned)v < (unsigned)gy && (unsigned)w < (unsigned)gz) {
+        c = (w * gy + v) * gx + u;
+      } else {
+        c = 0;
+        atomicOr(&box[6], 1);
+      }
+    }
+    const int prev = __shfl_up(c, 1);
+    const bool head = valid && (lane == 0 || c != prev);
+    const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
+    const int nvalid = __popcll(__builtin_amdgcn_ballot_w64(valid));
+    const uint64_t upto = heads & (((uint64_t)2 << lane) - 1u);
+    const int rs = valid ? 63 - __builtin_clzll(upto) : lane;
+    const uint64_t later = heads & ~(((uint64_t)2 << rs) - 1u);
+    const int rend = later ? __builtin_ctzll(later) : nvalid;
+    int sbase = 0;
+    if (head) sbase = atomicAdd(&count[c], rend - rs);
+    sbase = __shfl(sbase, rs);
+    if (valid) {
+      cell[i] = c;
+      slot[i] = sbase + (lane - rs);
+    }
+  }
+  __shared__ int red[6][256 / WAVE];
+  const int wid = threadIdx.x / WAVE;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = wave_min(lo[k]);
+    hi[k] = wave_max(hi[k]);
+    if (lane == 0) {
+      red[k][wid] = lo[k];
+      red[3 + k][wid] = hi[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int v = red[threadIdx.x][0];
+    for (int w = 1; w < 256 / WAVE; ++w)
+      v = threadIdx.x < 3 ? min(v, red[threadIdx.x][w]) : max(v, red[threadIdx.x][w]);
+    if (threadIdx.x < 3) atomicMin(&box[threadIdx.x], v);
+    else atomicMax(&box[threadIdx.x], v);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ cell, const int* __restrict__ slot,
                                                  const int* __restrict__ start, int n, int* __restrict__ arrival,
                                                  int* __restrict__ skey) {
@@ -173,44 +251,39 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ cell, c
 }
 
 // Slot s of cell c holds arrival[s]; its place in ascending index order is
-// start[c] + (number of the cell's particles with a smaller index). The
+// p = start[c] + (number of the cell's particles with a smaller index). The
 // cell's indices are read by every lane of the cell (contiguous, cached).
-__global__ __launch_bounds__(256) void k_cell_rank(const int* __restrict__ arrival, const int* __restrict__ skey,
-                                                   const int* __restrict__ start, int n, int* __restrict__ perm) {
+// The particle then goes to slot p: perm, the sorted copies of its position
+// ({x, y, z, field} records; the field is evaluated or gathered here when
+// one is set, else 0), {FP32 offsets from the grid origin, cell id} records
+// (for the candidate prefilter), the cell's grid-boundary flags, its type,
+// and the inverse permutation. arrival must not alias iperm (other lanes
+// of the cell still read it).
+__global__ __launch_bounds__(256) void k_rank_gather(const int* __restrict__ arrival, const int* __restrict__ skey,
+                                                     const int* __restrict__ start, int gx, int gy, int gz,
+                                                     const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, int n, double ox, double oy,
+                                                     double oz, int* __restrict__ perm, double4* __restrict__ spos,
+                                                     float4* __restrict__ sposf, int* __restrict__ sflag,
+                                                     int* __restrict__ iperm, int fmode, QuadField q,
+                                                     const double* __restrict__ fv, const uint8_t* __restrict__ type,
+                                                     uint8_t* __restrict__ type_s) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const int c = skey[s], b = start[c], e = start[c + 1], v = arrival[s];
+  const int cl = skey[s], b = start[cl], e = start[cl + 1], i = arrival[s];
   int r = 0;
-  for (int j = b; j < e; ++j) r += arrival[j] < v ? 1 : 0;
-  perm[b + r] = v;
-}
-
-// Sorted-slot copies of positions ({x, y, z, field} records; the field is
-// evaluated or gathered here when one is set, else 0), {FP32 offsets from
-// the grid origin, cell id} records
-// (for the candidate prefilter), the cell's grid-boundary flags, and the
-// inverse permutation.
-__global__ __launch_bounds__(256) void k_gather_sorted(const int* __restrict__ perm, const int* __restrict__ skey,
-                                                       int gx, int gy, int gz, const double* __restrict__ x,
-                                                       const double* __restrict__ y, const double* __restrict__ z,
-                                                       int n, double ox, double oy, double oz,
-                                                       double4* __restrict__ spos, float4* __restrict__ sposf,
-                                                       int* __restrict__ sflag, int* __restrict__ iperm, int fmode,
-                                                       QuadField q, const double* __restrict__ fv,
-                                                       const uint8_t* __restrict__ type, uint8_t* __restrict__ type_s) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const int i = perm[s];
-  const double a = x[i], b = y[i], c = z[i];
-  const double f = fmode == FIELD_QUADRATIC ? quad_eval(q, a, b, c) : fmode == FIELD_VALUES ? fv[i] : 0.0;
-  spos[s] = make_double4(a, b, c, f);
-  const int cl = skey[s];
-  sposf[s] = make_float4((float)(a - ox), (float)(b - oy), (float)(c - oz), __int_as_float(cl));
+  for (int j = b; j < e; ++j) r += arrival[j] < i ? 1 : 0;
+  const int p = b + r;
+  perm[p] = i;
+  const double a = x[i], bb = y[i], c = z[i];
+  const double f = fmode == FIELD_QUADRATIC ? quad_eval(q, a, bb, c) : fmode == FIELD_VALUES ? fv[i] : 0.0;
+  spos[p] = make_double4(a, bb, c, f);
+  sposf[p] = make_float4((float)(a - ox), (float)(bb - oy), (float)(c - oz), __int_as_float(cl));
   const int u = cl % gx, t = cl / gx, v = t % gy, w = t / gy;
-  sflag[s] = (u > 0 ? CNL_XLO : 0) | (u < gx - 1 ? CNL_XHI : 0) | (v > 0 ? CNL_YLO : 0) | (v < gy - 1 ? CNL_YHI : 0) |
+  sflag[p] = (u > 0 ? CNL_XLO : 0) | (u < gx - 1 ? CNL_XHI : 0) | (v > 0 ? CNL_YLO : 0) | (v < gy - 1 ? CNL_YHI : 0) |
              (w > 0 ? CNL_ZLO : 0) | (w < gz - 1 ? CNL_ZHI : 0);
-  iperm[i] = s;
-  type_s[s] = type[i];
+  iperm[i] = p;
+  type_s[p] = type[i];
 }
 
 // 14-sector bit of one offset: axis-dominant direction (ties x, then y,
@@ -573,21 +646,29 @@ void launch_fill_int(int* p, int64_t n, int v, hipStream_t st) {
 }
 
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, const QuadField& q, const double* fv,
-                      int* outside, hipStream_t st) {
+                      int* box, const CellKeyDirect* direct, hipStream_t st) {
   const int64_t ncells = (int64_t)B.gx * B.gy * B.gz;
   launch_fill_int(B.ccount, ncells, 0, st);
-  if (n > 0)
+  if (direct) {
+    hipLaunchKernelGGL(k_init_box, dim3(1), dim3(64), 0, st, box);
+    if (n > 0) {
+      const unsigned g = blocks(n, 256) < 1024u ? blocks(n, 256) : 1024u;
+      hipLaunchKernelGGL(k_cell_key_direct, dim3(g), dim3(256), 0, st, B.x, B.y, B.z, n, direct->rl, box_host[0],
+                         box_host[1], box_host[2], B.gx, B.gy, B.gz, B.cell, B.slot, B.ccount, box, direct->iperm_old,
+                         direct->type_s_old, direct->n_unsort, direct->type);
+    }
+  } else if (n > 0) {
     hipLaunchKernelGGL(k_cell_key, dim3(blocks(n, 256)), dim3(256), 0, st, B.cx, B.cy, B.cz, n, box_host[0],
-                       box_host[1], box_host[2], B.gx, B.gy, B.gz, B.cell, B.slot, B.ccount, outside);
+                       box_host[1], box_host[2], B.gx, B.gy, B.gz, B.cell, B.slot, B.ccount, box + 6);
+  }
   launch_exclusive_scan(B.ccount, ncells, B.cstart, B.scan_scratch, st);
   if (n > 0) {
-    // iperm holds the arrival order until k_gather_sorted writes it.
-    hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart, n, B.iperm,
-                       B.skey);
-    hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n, 256)), dim3(256), 0, st, B.iperm, B.skey, B.cstart, n, B.perm);
-    hipLaunchKernelGGL(k_gather_sorted, dim3(blocks(n, 256)), dim3(256), 0, st, B.perm, B.skey, B.gx, B.gy, B.gz, B.x,
-                       B.y, B.z, n, B.ox, B.oy, B.oz, B.spos, B.sposf, B.sflag, B.iperm, fmode, q, fv, B.type,
-                       B.type_s);
+    // The arrival order goes to cx (free once the keys are made): the rank
+    // pass reads it while it writes iperm.
+    hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart, n, B.cx, B.skey);
+    hipLaunchKernelGGL(k_rank_gather, dim3(blocks(n, 256)), dim3(256), 0, st, B.cx, B.skey, B.cstart, B.gx, B.gy,
+                       B.gz, B.x, B.y, B.z, n, B.ox, B.oy, B.oz, B.perm, B.spos, B.sposf, B.sflag, B.iperm, fmode, q,
+                       fv, B.type, B.type_s);
   }
 }
 

@@ -74,13 +74,24 @@ void launch_cell_coords(const double* x, const double* y, const double* z, int n
                         int* cz, int* box, const int* iperm_old, const uint8_t* type_s_old, int n_unsort,
                         uint8_t* type, hipStream_t st);
 void launch_fill_int(int* p, int64_t n, int v, hipStream_t st);
+// A grid known before the build (the last build's box plus a margin): the
+// cell coordinates, the box and the previous build's type unsort are made in
+// the key pass instead of launch_cell_coords.
+struct CellKeyDirect {
+  double rl = 0.0;                       // cell size
+  const int* iperm_old = nullptr;        // previous build's slots, for the type unsort
+  const uint8_t* type_s_old = nullptr;
+  int n_unsort = 0;
+  uint8_t* type = nullptr;               // caller-order types (written for i < n_unsort)
+};
 // Stable sort by cell and the sorted copies: perm, skey, cstart, spos,
-// sposf, sflag, iperm.
+// sposf, sflag, iperm. Uses cx as scratch (cx, cy, cz are consumed first).
 // fmode (FIELD_*): the field written into spos.w, from q or from values fv
-// in caller order. box_host: the grid's first cell per axis; *outside is set
-// when a particle lies outside the grid (B.gx, B.gy, B.gz).
+// in caller order. box_host: the grid's first cell per axis; box[6] is set
+// when a particle lies outside the grid (B.gx, B.gy, B.gz). With direct,
+// box[0..5] is also computed here (k_init_box first).
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, const QuadField& q, const double* fv,
-                      int* outside, hipStream_t st);
+                      int* box, const CellKeyDirect* direct, hipStream_t st);
 // Neighbor lists and classification work on sorted-order types only; the
 // original-order arrays are produced on demand by the unsort launchers.
 // re2: squared list radius; re2c: squared radius of the neighbors that
