@@ -223,7 +223,9 @@ int lsmps_build_neighbors_skin(lsmps_ctx* ctx, double re, double skin, int32_t m
 /* Velocities of all particles, 3 per particle, caller order. */
 int lsmps_set_velocities(lsmps_ctx* ctx, const double* uvw, uint64_t n);
 /* Appends particles (xyz 3 per particle, types, velocities or NULL) after
- * the existing ones; they join the stages at the next neighbor build. A
+ * the existing ones; they join the stages at the next neighbor build.
+ * Velocities are required when the context has them (LSMPS_E_STATE
+ * otherwise), and refused when the existing particles have none. A
  * field given as values (lsmps_set_field_values) has none for them, so it
  * must be set again; a quadratic field carries over. */
 int lsmps_append_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* type, const double* uvw,

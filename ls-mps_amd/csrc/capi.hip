@@ -116,6 +116,7 @@ struct lsmps_ctx {
   uint64_t pairs = 0;
   DBuf<int> cx, cy, cz, box, cell, slot, skey, ccount, cstart, perm, iperm, sflag, nbr, cnt, cnt_orig, row_ptr, col, hist;
   DBuf<double4> spos;
+  DBuf<double4> spos_alt;  // the other buffer of a build that reads the last build's sorted positions
   DBuf<float4> sposf;
   double ox = 0.0, oy = 0.0, oz = 0.0;  // grid origin of the FP32 offsets
   DBuf<uint8_t> type_s;
@@ -410,6 +411,7 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   if (ctx->st) (void)hipStreamSynchronize(ctx->st);
   ctx->reset_graphs();
   ctx->spos.release();
+  ctx->spos_alt.release();
   ctx->sposf.release();
   DBuf<double>* d[] = {&ctx->xyz_in, &ctx->x,   &ctx->y,     &ctx->z,      &ctx->f,        &ctx->grad_s,
                        &ctx->lap_s,  &ctx->grad, &ctx->lap, &ctx->ge,    &ctx->le,     &ctx->pval,     &ctx->vrr,
@@ -740,6 +742,10 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   B.extent = rl * (double)(ext_cells + 1);
   const int fmode = !ctx->have_field ? FIELD_NONE : ctx->field_quadratic ? FIELD_QUADRATIC : FIELD_VALUES;
   CellKeyDirect dk;
+  // The particles of the last build, moved only by lsmps_advance (which
+  // moves its sorted copies alike): read them in that build's order, into
+  // the other position buffer.
+  const bool sorted = spec && ctx->have_cnl && !ctx->replay && (uint64_t)n == ctx->n_cnl;
   if (spec) {
     dk.rl = rl;
     dk.iperm_old = ctx->iperm.p;
@@ -747,12 +753,18 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
     dk.n_unsort = n_unsort;
     dk.type = ctx->type.p;
   }
+  if (sorted) {
+    HIPCHK(ctx->spos_alt.ensure(n));
+    dk.spos_old = ctx->spos.p;
+    dk.perm_old = ctx->perm.p;
+    B.spos = ctx->spos_alt.p;
+  }
   launch_cell_sort(B, n, bx, fmode, ctx->qf, fmode == FIELD_VALUES ? ctx->f.p : nullptr, ctx->box.p,
-                   spec ? &dk : nullptr, st);
+                   spec ? &dk : nullptr, ctx->counters.p, st);
+  if (sorted) std::swap(ctx->spos, ctx->spos_alt);
   if (spec) ctx->types_stale = false;
   ctx->spos_field = fmode != FIELD_NONE;
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
-  zero_counters(ctx, 16);
   if (n == 0) launch_fill_int(ctx->hist.p, maxn + 1, 0, st);  // the pass's reduction writes every bin
   launch_cnl_pass(B, n, rl * rl, re * re, maxn, classify, 1.5 * l0, ctx->type_s.p, ctx->counters.p, ctx->hist.p,
                   ctx->counters.p + 8, st, ctx->e2, ctx->e3);
@@ -858,6 +870,9 @@ int lsmps_append_particles(lsmps_ctx* ctx, const double* xyz, const uint8_t* typ
   if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
   if (ctx->n + count >= (uint64_t)INT_MAX / 4) return fail(ctx, LSMPS_E_LIMIT, "particle count beyond int32 indexing");
   if (uvw && !ctx->have_u && ctx->n) return fail(ctx, LSMPS_E_STATE, "velocities of the existing particles unset");
+  // lsmps_advance moves every particle by its velocity.
+  if (!uvw && ctx->have_u && count)
+    return fail(ctx, LSMPS_E_STATE, "the context has velocities: pass the new particles' velocities");
   GUARD_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->st;

@@ -112,6 +112,27 @@ __global__ void k_fill_int(int* __restrict__ p, int64_t n, int v) {
   }
 }
 
+// Zeroes the cell counts (16 bytes per lane where aligned, as k_fill_int)
+// and the 16 build counters, and resets the box unless box is null: one
+// launch where three went at the start of a build.
+__global__ void k_build_init(int* __restrict__ count, int64_t ncells, int* __restrict__ box,
+                             unsigned long long* __restrict__ counters) {
+  const int64_t i = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (i + 4 <= ncells && (reinterpret_cast<uintptr_t>(count) & 15u) == 0) {
+    reinterpret_cast<int4*>(count + i)[0] = make_int4(0, 0, 0, 0);
+  } else {
+    for (int64_t k = i; k < ncells && k < i + 4; ++k) count[k] = 0;
+  }
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < 16) counters[threadIdx.x] = 0ull;
+    if (box) {
+      if (threadIdx.x < 3) box[threadIdx.x] = INT_MAX;
+      else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
+      else if (threadIdx.x == 6) box[6] = 0;
+    }
+  }
+}
+
 // Counting sort by cell, stable by particle index, in four launches:
 //  k_cell_key    linear cell id (x fastest) and a slot inside the cell from
 //                an atomic on the cell's count. Inputs often come in
@@ -120,7 +141,11 @@ __global__ void k_fill_int(int* __restrict__ p, int64_t n, int v) {
 //                one atomic from its first lane (about half the atomics).
 //                With the grid known in advance (the last build's box plus a
 //                margin), k_cell_key_direct computes the cell coordinates
-//                and the box in the same pass (no k_cell_coords).
+//                and the box in the same pass (no k_cell_coords). When the
+//                particles are those of the last build, it walks them in
+//                that build's sorted order (SORTED): positions come as
+//                32-byte records in order and the lanes of a wavefront
+//                mostly share a cell, so a few atomics take all the slots.
 //  (scan)        exclusive scan of the counts: the cells' first slots.
 //  k_scatter     particle and cell id to start + slot (arrival order).
 //  k_rank_gather each slot's rank among its cell's particle indices, so
@@ -171,8 +196,13 @@ __global__ __launch_bounds__(256) void k_cell_key(const int* __restrict__ cx, co
 // the type unsort of k_cell_coords, and the key and slot of k_cell_key.
 // Blocks stride over chunks of 256 consecutive particles, so the lanes of
 // a wavefront hold consecutive particles, as k_cell_key's runs need.
+// SORTED: item i is the last build's slot i (position spos_old[i],
+// particle perm_old[i]); cell and slot are indexed by that slot.
+template <bool SORTED>
 __global__ __launch_bounds__(256) void k_cell_key_direct(const double* __restrict__ x, const double* __restrict__ y,
-                                                         const double* __restrict__ z, int n, double rl, int bx,
+                                                         const double* __restrict__ z,
+                                                         const double4* __restrict__ spos_old,
+                                                         const int* __restrict__ perm_old, int n, double rl, int bx,
                                                          int by, int bz, int gx, int gy, int gz,
                                                          int* __restrict__ cell, int* __restrict__ slot,
                                                          int* __restrict__ count, int* __restrict__ box,
@@ -186,14 +216,28 @@ __global__ __launch_bounds__(256) void k_cell_key_direct(const double* __restric
     const bool valid = i < n;
     int c = -1;
     if (valid) {
-      const int a = (int)floor(x[i] / rl), b = (int)floor(y[i] / rl), w0 = (int)floor(z[i] / rl);
+      double px, py, pz;
+      if (SORTED) {
+        const double4 p = spos_old[i];
+        px = p.x;
+        py = p.y;
+        pz = p.z;
+      } else {
+        px = x[i];
+        py = y[i];
+        pz = z[i];
+      }
+      const int a = (int)floor(px / rl), b = (int)floor(py / rl), w0 = (int)floor(pz / rl);
       lo[0] = min(lo[0], a);
       hi[0] = max(hi[0], a);
       lo[1] = min(lo[1], b);
       hi[1] = max(hi[1], b);
       lo[2] = min(lo[2], w0);
       hi[2] = max(hi[2], w0);
-      if (i < n_unsort) type[i] = type_s_old[iperm_old[i]];
+      if (i < n_unsort) {
+        if (SORTED) type[perm_old[i]] = type_s_old[i];
+        else type[i] = type_s_old[iperm_old[i]];
+      }
       const int u = a - bx, v = b - by, w = w0 - bz;
       if ((unsigned)u < (unsigned)gx && (unsigned)v < (unsigned)gy && (unsigned)w < (unsigned)gz) {
         c = (w * gy + v) * gx + u;
@@ -239,13 +283,22 @@ __global__ __launch_bounds__(256) void k_cell_key_direct(const double* __restric
   }
 }
 
+// arrival[s]: the particle at slot s in arrival order; SORTED (keys indexed
+// by the last build's slot j): also arrival_slot[s] = j.
+template <bool SORTED>
 __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ cell, const int* __restrict__ slot,
-                                                 const int* __restrict__ start, int n, int* __restrict__ arrival,
+                                                 const int* __restrict__ start, const int* __restrict__ perm_old,
+                                                 int n, int* __restrict__ arrival, int* __restrict__ arrival_slot,
                                                  int* __restrict__ skey) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const int c = cell[i], s = start[c] + slot[i];
-    arrival[s] = i;
+    if (SORTED) {
+      arrival[s] = perm_old[i];
+      arrival_slot[s] = i;
+    } else {
+      arrival[s] = i;
+    }
     skey[s] = c;
   }
 }
@@ -258,8 +311,15 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ cell, c
 // one is set, else 0), {FP32 offsets from the grid origin, cell id} records
 // (for the candidate prefilter), the cell's grid-boundary flags, its type,
 // and the inverse permutation. arrival must not alias iperm (other lanes
-// of the cell still read it).
-__global__ __launch_bounds__(256) void k_rank_gather(const int* __restrict__ arrival, const int* __restrict__ skey,
+// of the cell still read it). SORTED: the position comes from the last
+// build's sorted record at arrival_slot[s] (the same values as x, y, z of
+// the particle: both are moved by the same FP64 updates), and spos must not
+// alias spos_old.
+template <bool SORTED>
+__global__ __launch_bounds__(256) void k_rank_gather(const int* __restrict__ arrival,
+                                                     const int* __restrict__ arrival_slot,
+                                                     const double4* __restrict__ spos_old,
+                                                     const int* __restrict__ skey,
                                                      const int* __restrict__ start, int gx, int gy, int gz,
                                                      const double* __restrict__ x, const double* __restrict__ y,
                                                      const double* __restrict__ z, int n, double ox, double oy,
@@ -275,7 +335,17 @@ __global__ __launch_bounds__(256) void k_rank_gather(const int* __restrict__ arr
   for (int j = b; j < e; ++j) r += arrival[j] < i ? 1 : 0;
   const int p = b + r;
   perm[p] = i;
-  const double a = x[i], bb = y[i], c = z[i];
+  double a, bb, c;
+  if (SORTED) {
+    const double4 po = spos_old[arrival_slot[s]];
+    a = po.x;
+    bb = po.y;
+    c = po.z;
+  } else {
+    a = x[i];
+    bb = y[i];
+    c = z[i];
+  }
   const double f = fmode == FIELD_QUADRATIC ? quad_eval(q, a, bb, c) : fmode == FIELD_VALUES ? fv[i] : 0.0;
   spos[p] = make_double4(a, bb, c, f);
   sposf[p] = make_float4((float)(a - ox), (float)(bb - oy), (float)(c - oz), __int_as_float(cl));
@@ -646,16 +716,24 @@ void launch_fill_int(int* p, int64_t n, int v, hipStream_t st) {
 }
 
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, const QuadField& q, const double* fv,
-                      int* box, const CellKeyDirect* direct, hipStream_t st) {
+                      int* box, const CellKeyDirect* direct, unsigned long long* counters, hipStream_t st) {
   const int64_t ncells = (int64_t)B.gx * B.gy * B.gz;
-  launch_fill_int(B.ccount, ncells, 0, st);
+  hipLaunchKernelGGL(k_build_init, dim3(std::max(1u, blocks((ncells + 3) / 4, 256))), dim3(256), 0, st, B.ccount,
+                     ncells, direct ? box : nullptr, counters);
+  const bool sorted = direct && direct->spos_old;
   if (direct) {
-    hipLaunchKernelGGL(k_init_box, dim3(1), dim3(64), 0, st, box);
     if (n > 0) {
       const unsigned g = blocks(n, 256) < 1024u ? blocks(n, 256) : 1024u;
-      hipLaunchKernelGGL(k_cell_key_direct, dim3(g), dim3(256), 0, st, B.x, B.y, B.z, n, direct->rl, box_host[0],
-                         box_host[1], box_host[2], B.gx, B.gy, B.gz, B.cell, B.slot, B.ccount, box, direct->iperm_old,
-                         direct->type_s_old, direct->n_unsort, direct->type);
+      if (sorted)
+        hipLaunchKernelGGL(k_cell_key_direct<true>, dim3(g), dim3(256), 0, st, B.x, B.y, B.z, direct->spos_old,
+                           direct->perm_old, n, direct->rl, box_host[0], box_host[1], box_host[2], B.gx, B.gy, B.gz,
+                           B.cell, B.slot, B.ccount, box, direct->iperm_old, direct->type_s_old, direct->n_unsort,
+                           direct->type);
+      else
+        hipLaunchKernelGGL(k_cell_key_direct<false>, dim3(g), dim3(256), 0, st, B.x, B.y, B.z, direct->spos_old,
+                           direct->perm_old, n, direct->rl, box_host[0], box_host[1], box_host[2], B.gx, B.gy, B.gz,
+                           B.cell, B.slot, B.ccount, box, direct->iperm_old, direct->type_s_old, direct->n_unsort,
+                           direct->type);
     }
   } else if (n > 0) {
     hipLaunchKernelGGL(k_cell_key, dim3(blocks(n, 256)), dim3(256), 0, st, B.cx, B.cy, B.cz, n, box_host[0],
@@ -663,12 +741,22 @@ void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, c
   }
   launch_exclusive_scan(B.ccount, ncells, B.cstart, B.scan_scratch, st);
   if (n > 0) {
-    // The arrival order goes to cx (free once the keys are made): the rank
-    // pass reads it while it writes iperm.
-    hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart, n, B.cx, B.skey);
-    hipLaunchKernelGGL(k_rank_gather, dim3(blocks(n, 256)), dim3(256), 0, st, B.cx, B.skey, B.cstart, B.gx, B.gy,
-                       B.gz, B.x, B.y, B.z, n, B.ox, B.oy, B.oz, B.perm, B.spos, B.sposf, B.sflag, B.iperm, fmode, q,
-                       fv, B.type, B.type_s);
+    // The arrival order goes to cx (and the last build's slots to cy; both
+    // free once the keys are made): the rank pass reads it while it writes
+    // iperm.
+    if (sorted) {
+      hipLaunchKernelGGL(k_scatter<true>, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart,
+                         direct->perm_old, n, B.cx, B.cy, B.skey);
+      hipLaunchKernelGGL(k_rank_gather<true>, dim3(blocks(n, 256)), dim3(256), 0, st, B.cx, B.cy, direct->spos_old,
+                         B.skey, B.cstart, B.gx, B.gy, B.gz, B.x, B.y, B.z, n, B.ox, B.oy, B.oz, B.perm, B.spos,
+                         B.sposf, B.sflag, B.iperm, fmode, q, fv, B.type, B.type_s);
+    } else {
+      hipLaunchKernelGGL(k_scatter<false>, dim3(blocks(n, 256)), dim3(256), 0, st, B.cell, B.slot, B.cstart,
+                         nullptr, n, B.cx, nullptr, B.skey);
+      hipLaunchKernelGGL(k_rank_gather<false>, dim3(blocks(n, 256)), dim3(256), 0, st, B.cx, nullptr, nullptr,
+                         B.skey, B.cstart, B.gx, B.gy, B.gz, B.x, B.y, B.z, n, B.ox, B.oy, B.oz, B.perm, B.spos,
+                         B.sposf, B.sflag, B.iperm, fmode, q, fv, B.type, B.type_s);
+    }
   }
 }
 

@@ -83,15 +83,20 @@ struct CellKeyDirect {
   const uint8_t* type_s_old = nullptr;
   int n_unsort = 0;
   uint8_t* type = nullptr;               // caller-order types (written for i < n_unsort)
+  // The same particles as the last build (which left them in sorted order):
+  // its sorted positions and slot -> particle map, read in that order. Null:
+  // caller order. spos_old must not alias the build's output spos.
+  const double4* spos_old = nullptr;
+  const int* perm_old = nullptr;
 };
 // Stable sort by cell and the sorted copies: perm, skey, cstart, spos,
-// sposf, sflag, iperm. Uses cx as scratch (cx, cy, cz are consumed first).
-// fmode (FIELD_*): the field written into spos.w, from q or from values fv
-// in caller order. box_host: the grid's first cell per axis; box[6] is set
-// when a particle lies outside the grid (B.gx, B.gy, B.gz). With direct,
-// box[0..5] is also computed here (k_init_box first).
+// sposf, sflag, iperm. Uses cx and cy as scratch (cx, cy, cz are consumed
+// first). Zeroes the 16 build counters. fmode (FIELD_*): the field written
+// into spos.w, from q or from values fv in caller order. box_host: the
+// grid's first cell per axis; box[6] is set when a particle lies outside
+// the grid (B.gx, B.gy, B.gz). With direct, box[0..5] is also computed here.
 void launch_cell_sort(const CnlBufs& B, int n, const int* box_host, int fmode, const QuadField& q, const double* fv,
-                      int* box, const CellKeyDirect* direct, hipStream_t st);
+                      int* box, const CellKeyDirect* direct, unsigned long long* counters, hipStream_t st);
 // Neighbor lists and classification work on sorted-order types only; the
 // original-order arrays are produced on demand by the unsort launchers.
 // re2: squared list radius; re2c: squared radius of the neighbors that

@@ -272,20 +272,51 @@ def test_kept_grid_reused_and_outgrown(pipe):
     pipe.upload(xyz, t)
     pipe.neighbors_classify(re, 96, l0)
     (r0, c0), t0 = fresh(xyz, t)
-    # Same particles again: the kept grid holds them.
+    # Same particles again: the kept grid holds them, and the build reads
+    # them in the last build's sorted order.
     pipe.neighbors_classify(re, 96, l0)
     r1, c1 = pipe.neighbor_lists()
     assert np.array_equal(r0, r1) and np.array_equal(c0, c1)
+    assert np.array_equal(t0, pipe.types())
+    # Moved by lsmps_advance, then built again in that order: the same
+    # lists, types and FP64 WLS (bit for bit) as a fresh context given the
+    # positions moved on the host by the same x += u dt.
+    u = np.random.default_rng(8).uniform(-1.0, 1.0, xyz.shape)
+    dt = 0.3 * l0
+    pipe.set_velocities(u)
+    pipe.advance(dt)
+    moved = xyz + u * dt
+    pipe.neighbors_classify(re, 96, l0)
+    pipe.set_field(L.C1_FIELD)
+    pipe.wls(re, l0, L.WLS_FP64)
+    wm = pipe.wls_results()
+    q = L.Pipeline(0)
+    q.upload(moved, t)
+    q.neighbors_classify(re, 96, l0)
+    rq, cq = q.neighbor_lists()
+    tq = q.types()
+    q.set_field(L.C1_FIELD)
+    q.wls(re, l0, L.WLS_FP64)
+    wq = q.wls_results()
+    q.close()
+    r4, c4 = pipe.neighbor_lists()
+    assert np.array_equal(rq, r4) and np.array_equal(cq, c4)
+    assert np.array_equal(tq, pipe.types())
+    for a, b in zip(wm, wq):
+        assert np.array_equal(a, b)
     # Particles well outside the last box: the kept grid is outgrown.
     lo, hi = xyz.min(0), xyz.max(0)
     extra = lo + (hi - lo) * np.array([[1.0, 0.5, 0.5]]) + np.array([[12 * re, 0.0, 0.0]])
     extra = extra + l0 * np.arange(3)[:, None] * np.array([[1.0, 0.0, 0.0]])
     te = np.zeros(len(extra), np.uint8)
-    pipe.append(extra, te)
+    # The context has velocities: new particles need theirs too.
+    with pytest.raises(L.LsmpsError):
+        pipe.append(extra, te)
+    pipe.append(extra, te, np.zeros_like(extra))
     types_now = pipe.types()
     st = pipe.neighbors_classify(re, 96, l0)
     say("after append", st[0]["pairs"])
-    both = np.concatenate([xyz, extra])
+    both = np.concatenate([moved, extra])
     (r2, c2), t2 = fresh(both, types_now)
     r3, c3 = pipe.neighbor_lists()
     assert np.array_equal(r2, r3) and np.array_equal(c2, c3)
