@@ -51,14 +51,16 @@ __device__ __forceinline__ void store_result(int i, uint8_t st, const double* g3
 // within one ulp of a/b, the remainder b q0 - a is exact in one FMA, and
 // RN(q0 - (b q0 - a) y) is the correctly rounded quotient; negating the
 // remainder keeps the sign of a zero quotient. It holds while the remainder
-// is clear of underflow and the quotient of overflow: `off` is set when
-// |q0| leaves [2^-900, 2^900] for a nonzero a (NaN included), and the
-// caller then must not use the result.
-__device__ __forceinline__ double div_rn(double a, double b, double y, bool& off) {
+// is clear of underflow and the quotient of overflow, so the caller tracks
+// the binary exponents of the q0 (emin, emax; frexp gives 0 for a zero) and
+// does not use the results when one leaves [2^-900, 2^900] or a sum is not
+// finite (an infinite or NaN quotient).
+__device__ __forceinline__ double div_rn(double a, double b, double y, int& emin, int& emax) {
   const double q0 = a * y;
   const double e = fma(b, q0, -a);
-  const double aq = fabs(q0);
-  off = off || (!(aq >= 0x1p-900 && aq <= 0x1p900) && a != 0.0);
+  const int ex = __builtin_amdgcn_frexp_exp(q0);  // 0 for a zero quotient
+  emin = min(emin, ex);
+  emax = max(emax, ex);
   return fma(-e, y, q0);
 }
 
@@ -75,8 +77,10 @@ __device__ __forceinline__ double div_rn(double a, double b, double y, bool& off
 // 17 divisions per neighbor pair and the 54 per particle become three
 // instructions each, 1.70 -> 1.41 ms at 3M (industrial_mix), the same
 // results bit for bit; a row with a quotient div_rn cannot vouch for goes
-// to k_wls_redo, which divides. Forcing three wavefronts per SIMD (168
-// VGPRs) measured 1.75 ms against 1.67, four (spills) 7.2 ms.
+// to k_wls_redo, which divides. Tracking the quotients' exponents (frexp,
+// min, max) instead of testing each against the range: 207 VGPRs and 4%
+// less time. Forcing three wavefronts per SIMD (168 VGPRs) measured 1.75 ms
+// against 1.67, four (spills) 7.2 ms.
 __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ spos,
                                                     const uint8_t* __restrict__ type, const int* __restrict__ nbr,
                                                     const int* __restrict__ cnt, int maxn, int n, double re, double l0,
@@ -96,7 +100,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
       const int kn = cnt[i];
       const double h2 = 2 * l0;
       double M[45], B[9], ws = 0.0;
-      bool off = false;  // a div_rn quotient out of its range: divide in k_wls_redo
+      int emin = 0, emax = 0;  // exponent range of the div_rn quotients
 #pragma unroll
       for (int k = 0; k < 45; ++k) M[k] = 0.0;
 #pragma unroll
@@ -111,7 +115,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         if (r <= 1e-12 || r >= re) continue;
         const double inv = 1.0 / r, h = h2 * r, g = l0 * r;
-        const double tq = div_rn(re, r, inv, off) - 1.0;  // re / r >= 1: in range
+        const double tq = div_rn(re, r, inv, emin, emax) - 1.0;  // re / r >= 1: in range
         const double w = tq * tq;
         if (w == 0.0) continue;
         const double yh = 1.0 / h, yg = 1.0 / g;
@@ -119,18 +123,18 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
         q[0] = dx * inv;
         q[1] = dy * inv;
         q[2] = dz * inv;
-        q[3] = div_rn(dx * dx, h, yh, off);
-        q[4] = div_rn(dy * dy, h, yh, off);
-        q[5] = div_rn(dz * dz, h, yh, off);
-        q[6] = div_rn(dx * dy, g, yg, off);
-        q[7] = div_rn(dx * dz, g, yg, off);
-        q[8] = div_rn(dy * dz, g, yg, off);
+        q[3] = div_rn(dx * dx, h, yh, emin, emax);
+        q[4] = div_rn(dy * dy, h, yh, emin, emax);
+        q[5] = div_rn(dz * dz, h, yh, emin, emax);
+        q[6] = div_rn(dx * dy, g, yg, emin, emax);
+        q[7] = div_rn(dx * dz, g, yg, emin, emax);
+        q[8] = div_rn(dy * dz, g, yg, emin, emax);
         const double df = pj.w - pi.w;
         ws += w;
 #pragma unroll
         for (int a = 0; a < 9; ++a) {
           const double wq = w * q[a];
-          B[a] += div_rn(wq * df, r, inv, off);
+          B[a] += div_rn(wq * df, r, inv, emin, emax);
 #pragma unroll
           for (int c = 0; c <= a; ++c) M[tri(a, c)] += wq * q[c];
         }
@@ -140,10 +144,18 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
       } else {
         const double yw = 1.0 / ws;
 #pragma unroll
-        for (int k = 0; k < 45; ++k) M[k] = div_rn(M[k], ws, yw, off);
+        for (int k = 0; k < 45; ++k) M[k] = div_rn(M[k], ws, yw, emin, emax);
 #pragma unroll
-        for (int a = 0; a < 9; ++a) B[a] = div_rn(B[a], ws, yw, off);
+        for (int a = 0; a < 9; ++a) B[a] = div_rn(B[a], ws, yw, emin, emax);
         const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        // Quotients within [2^-900, 2^900] (a zero has exponent 0) and
+        // finite sums (an overflow or a NaN input): else k_wls_redo divides.
+        bool fin = isfinite(ws);
+#pragma unroll
+        for (int k = 0; k < 45; ++k) fin = fin && isfinite(M[k]);
+#pragma unroll
+        for (int a = 0; a < 9; ++a) fin = fin && isfinite(B[a]);
+        const bool off = emin < -899 || emax > 900 || !fin;
         if (!off && chol9<double>(M, 1e-14, 0.0, zero9)) {
           double sol[9];
           chol9_solve<double>(M, B, sol);
