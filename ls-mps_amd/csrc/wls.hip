@@ -118,7 +118,12 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
         const double tq = div_rn(re, r, inv, emin, emax) - 1.0;  // re / r >= 1: in range
         const double w = tq * tq;
         if (w == 0.0) continue;
-        const double yh = 1.0 / h, yg = 1.0 / g;
+        // h = (2 l0) r is exactly 2 g, so RN(1/h) = RN(1/g) / 2 while g and
+        // 1/g are normal numbers (g's exponent joins the range check).
+        const double yg = 1.0 / g, yh = 0.5 * yg;
+        const int eg = __builtin_amdgcn_frexp_exp(g);
+        emin = min(emin, eg);
+        emax = max(emax, eg);
         double q[9];
         q[0] = dx * inv;
         q[1] = dy * inv;
