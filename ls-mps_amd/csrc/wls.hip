@@ -105,12 +105,13 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
       for (int k = 0; k < 45; ++k) M[k] = 0.0;
 #pragma unroll
       for (int a = 0; a < 9; ++a) B[a] = 0.0;
+      // Prefetches past the row's end reload its last entry (no branches).
       double4 pn = kn > 0 ? spos[col[0]] : pi;
-      int jn = kn > 1 ? col[1] : 0;
+      int jn = col[kn > 1 ? 1 : 0];
       for (int k = 0; k < kn; ++k) {
         const double4 pj = pn;
-        if (k + 1 < kn) pn = spos[jn];
-        if (k + 2 < kn) jn = col[k + 2];
+        pn = spos[jn];
+        jn = col[k + 2 < kn ? k + 2 : kn - 1];
         const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         if (r <= 1e-12 || r >= re) continue;
