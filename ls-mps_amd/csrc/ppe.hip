@@ -346,8 +346,14 @@ __device__ __noinline__ bool wls_lap_weights(const double4 pi, const int* __rest
     double M[45], ws = 0.0;
 #pragma unroll
     for (int e = 0; e < 45; ++e) M[e] = 0.0;
+    // Each record is loaded one neighbor ahead and its index two ahead;
+    // past the row's end they reload its last entry.
+    double4 pn = k > 0 ? spos[row[0]] : pi;
+    int jn = row[k > 1 ? 1 : 0];
     for (int q = 0; q < k; ++q) {
-      const double4 pj = spos[row[q]];
+      const double4 pj = pn;
+      pn = spos[jn];
+      jn = row[q + 2 < k ? q + 2 : k - 1];
       const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
       const double r = sqrt(dx * dx + dy * dy + dz * dz);
       if (r <= 1e-12 || r >= re) continue;
@@ -422,9 +428,14 @@ __global__ __launch_bounds__(PPE_WLS_THREADS) void k_ppe_wls_fill(
       int kept = 0, below_u = 0;
       if (ok) {
         const double h2 = 2 * l0;
+        double4 pn = k > 0 ? spos[row[0]] : pi;
+        int sn = row[0], jn = row[k > 1 ? 1 : 0];
         for (int q = 0; q < k; ++q) {
-          const int s2 = row[q];
-          const double4 pj = spos[s2];
+          const int s2 = sn;
+          const double4 pj = pn;
+          pn = spos[jn];
+          sn = jn;
+          jn = row[q + 2 < k ? q + 2 : k - 1];
           const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
           const double r = sqrt(dx * dx + dy * dy + dz * dz);
           if (r <= 1e-12 || r >= re) continue;
