@@ -177,7 +177,7 @@ struct lsmps_ctx {
   // learned preconditioner: features of the current PPE (row order u), the
   // network (FP32 master, BF16 copy, Adam moments) and the training set
   bool have_features = false, have_params = false;
-  DBuf<float> feat, pred, feat_rr, master, adam1, adam2, mgrad, gpart, dfeat, dtarget, tfeat, ttarget;
+  DBuf<float> feat, pred, feat_rr, master, adam1, adam2, mgrad, gpart, dfeat, dtarget, tfeat, ttarget, wf32;
   DBuf<uint16_t> wbf;
   DBuf<double> scal, lx0, lscale;
   DBuf<unsigned long long> fbits;
@@ -432,7 +432,8 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->st_s.release();
   ctx->ty_s.release();
   DBuf<float>* fl[] = {&ctx->feat,  &ctx->pred, &ctx->feat_rr, &ctx->master, &ctx->adam1,  &ctx->adam2,
-                       &ctx->mgrad, &ctx->gpart, &ctx->dfeat, &ctx->dtarget, &ctx->tfeat, &ctx->ttarget};
+                       &ctx->mgrad, &ctx->gpart, &ctx->dfeat, &ctx->dtarget, &ctx->tfeat, &ctx->ttarget,
+                       &ctx->wf32};
   for (auto* p : fl) p->release();
   ctx->wbf.release();
   ctx->u.release();
@@ -1569,6 +1570,7 @@ int ensure_mlp_buffers(lsmps_ctx* ctx) {
   HIPCHK(ctx->adam2.ensure(LEARN_P));
   HIPCHK(ctx->mgrad.ensure(LEARN_P + 1));
   HIPCHK(ctx->wbf.ensure(LEARN_P));
+  HIPCHK(ctx->wf32.ensure(LEARN_P));
   return LSMPS_OK;
 }
 
@@ -1796,7 +1798,7 @@ int lsmps_solve_ppe(lsmps_ctx* ctx, const lsmps_solve_params* p, lsmps_solve_res
     HIPCHK(ctx->pred.ensure((size_t)m * LEARN_O));
     HIPCHK(ctx->lx0.ensure(m));
     HIPCHK(ctx->lscale.ensure(m));
-    launch_mlp_forward(ctx->feat.p, m, ctx->wbf.p, ctx->master.p, ctx->pred.p, st);
+    launch_mlp_forward(ctx->feat.p, m, ctx->wbf.p, ctx->master.p, ctx->wf32.p, ctx->pred.p, st);
     launch_learned_apply(ctx->pred.p, ctx->diag.p, m, ctx->scal.p, ctx->lx0.p, ctx->lscale.p, st);
     if (int r2 = check_launch(ctx, "learned_apply")) return r2;
     rc = solve_mixed(ctx, ctx->lscale.p, ctx->lx0.p, true, p->max_iter, p->tol, chunk, out);
@@ -2010,7 +2012,7 @@ int lsmps_mlp_predict(lsmps_ctx* ctx, const float* feat, uint64_t n, float* out)
   HIPCHK(ctx->ttarget.ensure(n * LEARN_O));
   if (n) {
     HIPCHK(hipMemcpyAsync(ctx->tfeat.p, feat, n * LEARN_F * sizeof(float), hipMemcpyHostToDevice, ctx->st));
-    launch_mlp_forward(ctx->tfeat.p, (int)n, ctx->wbf.p, ctx->master.p, ctx->ttarget.p, ctx->st);
+    launch_mlp_forward(ctx->tfeat.p, (int)n, ctx->wbf.p, ctx->master.p, ctx->wf32.p, ctx->ttarget.p, ctx->st);
     if (int rc = check_launch(ctx, "mlp_forward")) return rc;
     HIPCHK(hipMemcpyAsync(out, ctx->ttarget.p, n * LEARN_O * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
   }

@@ -241,21 +241,85 @@ __device__ __forceinline__ void mlp_forward_rows(MlpSmem& S, int r, int part, fl
   }
 }
 
-__global__ __launch_bounds__(MLP_THREADS) void k_mlp_forward(const float* __restrict__ feat, int n,
-                                                             const uint16_t* __restrict__ wbf,
-                                                             const float* __restrict__ master,
-                                                             float* __restrict__ out) {
-  __shared__ MlpSmem S;
-  load_weights(S, wbf, master);
-  const int r = threadIdx.x / 4, part = threadIdx.x & 3;
-  const int i = blockIdx.x * MLP_ROWS + r;
-  for (int k = part; k < LEARN_F; k += 4) S.x[r][k] = i < n ? feat[(size_t)i * LEARN_F + k] : 0.0f;
+// The parameters as the network uses them, in FP32: weight matrices
+// rounded to BF16, biases as kept.
+__global__ __launch_bounds__(256) void k_mlp_unpack(const uint16_t* __restrict__ wbf, const float* __restrict__ master,
+                                                    float* __restrict__ wf) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= LEARN_P) return;
+  const bool bias = (e >= LEARN_OB1 && e < LEARN_OW2) || (e >= LEARN_OB2 && e < LEARN_OW3) || e >= LEARN_OB3;
+  wf[e] = bias ? master[e] : bf16_to_f32(wbf[e]);
+}
+
+// Inference: one row per lane, blocks striding over the rows. The block
+// stages the FP32 parameters in LDS once; every weight read has the same
+// address across the wavefront (an LDS broadcast, four weights per 16-byte
+// read). The 64 first-layer activations stay in registers, and each
+// second-layer unit is folded into the two outputs as soon as it is made.
+// The same FP32 arithmetic as mlp_forward_rows (a unit's inputs summed in
+// the same order; the outputs summed over the 64 units in order rather
+// than as four partial sums).
+constexpr int MLP_FWD_THREADS = 256;
+__global__ __launch_bounds__(MLP_FWD_THREADS) void k_mlp_forward(const float* __restrict__ feat, int n,
+                                                                 const float* __restrict__ wf,
+                                                                 float* __restrict__ out) {
+  __shared__ float4 W4[(LEARN_P + 3) / 4];
+  float* W = reinterpret_cast<float*>(W4);
+  for (int e = threadIdx.x; e < LEARN_P; e += MLP_FWD_THREADS) W[e] = wf[e];
   __syncthreads();
-  float y[LEARN_O];
-  mlp_forward_rows(S, r, part, y);
-  if (i < n && part == 0) {
-    out[(size_t)i * LEARN_O] = y[0];
-    out[(size_t)i * LEARN_O + 1] = y[1];
+  const float4* w1 = reinterpret_cast<const float4*>(W + LEARN_OW1);
+  const float4* w2 = reinterpret_cast<const float4*>(W + LEARN_OW2);
+  for (int base = blockIdx.x * MLP_FWD_THREADS; base < n; base += gridDim.x * MLP_FWD_THREADS) {
+    const int i = base + threadIdx.x;
+    const bool live = i < n;
+    float x[LEARN_F];
+    {
+      const float4* f4 = reinterpret_cast<const float4*>(feat + (size_t)(live ? i : 0) * LEARN_F);
+#pragma unroll
+      for (int q = 0; q < LEARN_F / 4; ++q) {
+        const float4 v = f4[q];
+        x[4 * q] = v.x;
+        x[4 * q + 1] = v.y;
+        x[4 * q + 2] = v.z;
+        x[4 * q + 3] = v.w;
+      }
+    }
+    float h1[LEARN_H];
+#pragma unroll
+    for (int o = 0; o < LEARN_H; ++o) {
+      float a = W[LEARN_OB1 + o];
+#pragma unroll
+      for (int q = 0; q < LEARN_F / 4; ++q) {
+        const float4 w = w1[o * (LEARN_F / 4) + q];
+        a = fmaf(w.x, x[4 * q], a);
+        a = fmaf(w.y, x[4 * q + 1], a);
+        a = fmaf(w.z, x[4 * q + 2], a);
+        a = fmaf(w.w, x[4 * q + 3], a);
+      }
+      h1[o] = tanhf(a);
+      // Keep the scheduler from hoisting the whole layer's weight reads
+      // (they would take every register).
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    float y0 = 0.0f, y1 = 0.0f;
+    for (int o = 0; o < LEARN_H; ++o) {
+      float a = W[LEARN_OB2 + o];
+#pragma unroll
+      for (int q = 0; q < LEARN_H / 4; ++q) {
+        const float4 w = w2[o * (LEARN_H / 4) + q];
+        a = fmaf(w.x, h1[4 * q], a);
+        a = fmaf(w.y, h1[4 * q + 1], a);
+        a = fmaf(w.z, h1[4 * q + 2], a);
+        a = fmaf(w.w, h1[4 * q + 3], a);
+      }
+      const float h = tanhf(a);
+      y0 = fmaf(W[LEARN_OW3 + o], h, y0);
+      y1 = fmaf(W[LEARN_OW3 + LEARN_H + o], h, y1);
+    }
+    if (live) {
+      out[(size_t)i * LEARN_O] = y0 + W[LEARN_OB3];
+      out[(size_t)i * LEARN_O + 1] = y1 + W[LEARN_OB3 + 1];
+    }
   }
 }
 
@@ -478,10 +542,12 @@ void launch_iota(int n, int* idx, hipStream_t st) {
 void launch_mlp_cast(const float* master, uint16_t* wbf, hipStream_t st) {
   hipLaunchKernelGGL(k_mlp_cast, dim3(nblocks(LEARN_P, 256)), dim3(256), 0, st, master, wbf);
 }
-void launch_mlp_forward(const float* feat, int n, const uint16_t* wbf, const float* master, float* out,
+void launch_mlp_forward(const float* feat, int n, const uint16_t* wbf, const float* master, float* wf, float* out,
                         hipStream_t st) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_mlp_forward, dim3(nblocks(n, MLP_ROWS)), dim3(MLP_THREADS), 0, st, feat, n, wbf, master, out);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mlp_unpack, dim3(nblocks(LEARN_P, 256)), dim3(256), 0, st, wbf, master, wf);
+  const unsigned g = nblocks(n, MLP_FWD_THREADS) < 2048u ? nblocks(n, MLP_FWD_THREADS) : 2048u;
+  hipLaunchKernelGGL(k_mlp_forward, dim3(g), dim3(MLP_FWD_THREADS), 0, st, feat, n, wf, out);
 }
 int mlp_grad_blocks(int nb) { return (int)nblocks(nb, MLP_ROWS); }
 void launch_mlp_grad(const float* feat, const float* target, const int* idx, int nb, const uint16_t* wbf,

@@ -23,7 +23,8 @@ void launch_features(const double4* spos, const uint8_t* type_s, const int* uslo
                      hipStream_t st);
 void launch_rows_to_rr(const int* rru, const float* src, int m, int width, float* dst, hipStream_t st);
 void launch_mlp_cast(const float* master, uint16_t* wbf, hipStream_t st);
-void launch_mlp_forward(const float* feat, int n, const uint16_t* wbf, const float* master, float* out,
+// wf: LEARN_P floats of scratch (the parameters unpacked to FP32).
+void launch_mlp_forward(const float* feat, int n, const uint16_t* wbf, const float* master, float* wf, float* out,
                         hipStream_t st);
 // partials: mlp_grad_blocks(nb) * (LEARN_P + 1) floats; grad: LEARN_P + 1
 // (the last entry is the loss).
