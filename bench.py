@@ -12,8 +12,9 @@ K steps, `value` = particles / step time (Mparticles/s).
 
 Also measured, outside the timed steps: PPE assembly over all fluid
 unknowns, and the PPE solve to a true relative residual of 1e-8 with plain
-FP64 right-Jacobi BiCGStab, with mixed-precision BiCGStab (FP32 Krylov
-vectors and matrix, FP64 reliable updates), and with the learned start and
+FP64 right-Jacobi BiCGStab, with mixed-precision BiCGStab (FP32 iteration
+matrix, FP64 Krylov vectors and dot products, FP64 true-residual
+replacements and restarts), and with the learned start and
 diagonal inside the mixed solver (network weights from
 ls-mps_amd/weights/mlp_industrial_mix.npy, trained by python -m
 lsmps_amd.learn --case industrial_mix), all on the same matrix, over five

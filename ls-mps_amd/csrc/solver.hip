@@ -201,6 +201,48 @@ __device__ __forceinline__ double sell_dot_packed(const int* __restrict__ sptr, 
   const SellRow R = sell_row(sptr, row);
   double acc = 0.0;
   int k = 0;
+  if constexpr (sizeof(TM) == 4) {
+    // FP32 values: four entries per step, and the next step's column and
+    // value loads issued before this step's gathers are waited on (3.7% per
+    // mixed iteration at 3M; the FP64 product measured 1.7% slower with it,
+    // so it keeps the plain loop below). Same products, same order.
+    if (R.width >= 4) {
+      unsigned pk[4];
+      TM v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e = R.base + j * SLICE;
+        pk[j] = pcol[e];
+        v[j] = val[e];
+      }
+      for (;; k += 4) {
+        double xv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          xv[j] = x[__shfl(tab, (int)(pk[j] >> SELL_PACK_OFF_BITS)) + (int)(pk[j] & SELL_PACK_OFF_MASK)];
+        const bool more = k + 8 <= R.width;
+        unsigned pn[4];
+        TM vn[4];
+        if (more) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int e = R.base + (k + 4 + j) * SLICE;
+            pn[j] = pcol[e];
+            vn[j] = val[e];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc = fma((double)v[j], xv[j], acc);
+        if (!more) break;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          pk[j] = pn[j];
+          v[j] = vn[j];
+        }
+      }
+      k += 4;
+    }
+  }
   // Four entries per step, loads first (the table exchange is a wavefront
   // operation, so the compiler does not unroll this loop by itself).
   for (; k + 4 <= R.width; k += 4) {
