@@ -111,11 +111,17 @@ typedef struct {
   double re, l0;
   double reg;        /* regularization base; the reference always uses 1e-8 */
   int32_t precision; /* LSMPS_WLS_FP64: bit-exact with the reference;
-                        LSMPS_WLS_MIXED: FP32 moments, FP64 redo of weak rows */
+                        LSMPS_WLS_MIXED: FP32 moments, FP64 for weak rows */
+  /* MIXED: a row whose FP32 Cholesky has a pivot at or below this fraction
+   * of its diagonal entry is computed in FP64 instead (the reference's bits).
+   * The FP32 error of a row grows like 1 / (smallest pivot ratio); 0 selects
+   * LSMPS_WLS_FP32_PIVOT_DEFAULT. Must lie in [0, 1]. */
+  double fp32_pivot_min;
 } lsmps_wls_params;
+#define LSMPS_WLS_FP32_PIVOT_DEFAULT 0.2
 typedef struct {
   uint64_t ok, regularized, failed, skipped;
-  uint64_t fp64_redo; /* MIXED: rows redone in FP64 */
+  uint64_t fp64_redo; /* MIXED: rows computed in FP64 */
   float ms;
   float kernel_ms;    /* GPU time of the main WLS kernel alone (no field, no FP64 redo) */
 } lsmps_wls_result;
@@ -129,8 +135,8 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out);
  * writes the gradient (3 per particle), Laplacian and status into the
  * caller's device arrays, in particle order (particles appended since the
  * last neighbor build are not written). The stage's work list, block
- * partials and counters live in a fixed device array of about 9 MB per GPU,
- * held until the process ends; a context with more than
+ * partials, counters and the mixed kernel's row flags and list live in a
+ * fixed device array of about 19 MB per GPU, held until the process ends; a context with more than
  * LSMPS_NOALLOC_MAX_PARTICLES particles is refused with LSMPS_E_LIMIT.
  * Calls on one GPU share that array: calls that overlap in time on one GPU
  * (other threads, other streams) race. The context's own WLS results

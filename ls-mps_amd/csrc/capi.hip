@@ -137,6 +137,8 @@ struct lsmps_ctx {
   QuadField qf{};
   DBuf<double> f, grad_s, lap_s, grad, lap, ge, le;
   DBuf<uint8_t> wstatus_s, wstatus, st_s, ty_s;
+  DBuf<uint8_t> wpend;    // mixed WLS: rows left to FP64 (flags in sorted order),
+  DBuf<int> wpoff, wlist;  // their offsets per 64 rows and their list
   DBuf<int> redo, nredo;
 
   // PPE
@@ -361,6 +363,11 @@ double max_speed(const double* uvw, uint64_t n) {
 // Stages that read pairs within re from the lists need the lists to still
 // hold every such pair: refused once the particles may have closed in by
 // the skin (any motion at all for lists built without one).
+// Pivot-ratio threshold of the mixed WLS (validated by the caller).
+float wls_tau(const lsmps_wls_params* p) {
+  return (float)(p->fp32_pivot_min > 0.0 ? p->fp32_pivot_min : LSMPS_WLS_FP32_PIVOT_DEFAULT);
+}
+
 int check_lists_current(lsmps_ctx* ctx, const char* stage) {
   if (ctx->moved != 0.0 && !(2.0 * ctx->moved < ctx->skin))
     return fail(ctx, LSMPS_E_STATE,
@@ -428,6 +435,9 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   ctx->type.release();
   ctx->type_s.release();
   ctx->wstatus_s.release();
+  ctx->wpend.release();
+  ctx->wpoff.release();
+  ctx->wlist.release();
   ctx->wstatus.release();
   ctx->st_s.release();
   ctx->ty_s.release();
@@ -1040,6 +1050,8 @@ int lsmps_set_field_values(lsmps_ctx* ctx, const double* f, uint64_t n) {
 
 int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) {
   if (!ctx || !p) return LSMPS_E_ARG;
+  if (!(p->fp32_pivot_min >= 0.0 && p->fp32_pivot_min <= 1.0))
+    return fail(ctx, LSMPS_E_ARG, "fp32_pivot_min must lie in [0, 1]");
   if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "WLS needs neighbors");
   if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph: use lsmps_replay_wls");
   if (!ctx->have_field) return fail(ctx, LSMPS_E_STATE, "WLS needs a field (lsmps_set_field_*)");
@@ -1057,6 +1069,15 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   HIPCHK(ctx->lap_s.ensure(n));
   HIPCHK(ctx->wstatus_s.ensure(n));
   HIPCHK(ctx->redo.ensure(std::max(n, 1)));
+  WlsMixedWork mw{};
+  if (p->precision == LSMPS_WLS_MIXED) {
+    const int64_t gm = ((int64_t)n + 63) / 64;
+    HIPCHK(ctx->wpend.ensure(std::max(n, 1)));
+    HIPCHK(ctx->wpoff.ensure((size_t)gm + 1));
+    HIPCHK(ctx->wlist.ensure(std::max(n, 1)));
+    HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems(gm)));
+    mw = {ctx->wpend.p, ctx->wpoff.p, ctx->scan_scr.p, ctx->wlist.p};
+  }
   HIPCHK(ctx->bpart.ensure(6 * ((size_t)n / 64 + 1)));
   HIPCHK(hipEventRecord(ctx->e0, st));
   // Field into the sorted records: evaluated there for a quadratic,
@@ -1077,9 +1098,9 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
                      ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, ctx->bpart.p, st,
                      ctx->e2, ctx->e3);
   } else {
-    launch_wls_mixed(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
-                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, ctx->bpart.p, st,
-                     ctx->e2, ctx->e3);
+    launch_wls_mixed(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, wls_tau(p), mw,
+                     ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p,
+                     ctx->bpart.p, st, ctx->e2, ctx->e3);
   }
   if (int rc = check_launch(ctx, "wls")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
@@ -1104,8 +1125,9 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
 namespace {
 
 // Scratch memory of lsmps_wls_noalloc, one copy per GPU, held until the
-// process ends: the stage counters, the FP64 work-list length and list, and
-// the per-block count partials. Host code never uses the array's name as a
+// process ends: the stage counters, the FP64 work-list length and list, the
+// per-block count partials, and the mixed kernel's FP64 row flags, their
+// offsets per 64 rows (with the scan's scratch) and their list. Host code never uses the array's name as a
 // device address: each device's copy is found with hipGetSymbolAddress
 // (noalloc_scratch) and only that pointer is handed to kernels and copies.
 constexpr size_t NA_COUNTERS = 0;                      // 16 x u64
@@ -1113,7 +1135,13 @@ constexpr size_t NA_NREDO = 128;                       // 1 x int
 constexpr size_t NA_BPART = 256;                       // 6 x ceil(max / 64) x u32
 constexpr size_t NA_BPART_WORDS = 6 * (size_t)(LSMPS_NOALLOC_MAX_PARTICLES / 64);
 constexpr size_t NA_REDO = NA_BPART + ((NA_BPART_WORDS * 4 + 255) / 256) * 256;  // max x int
-constexpr size_t NA_BYTES = NA_REDO + (size_t)LSMPS_NOALLOC_MAX_PARTICLES * 4;
+constexpr size_t NA_GM = LSMPS_NOALLOC_MAX_PARTICLES / 64;                      // 64-row blocks
+constexpr size_t NA_PEND = NA_REDO + (size_t)LSMPS_NOALLOC_MAX_PARTICLES * 4;     // max x u8
+constexpr size_t NA_PLIST = NA_PEND + (size_t)LSMPS_NOALLOC_MAX_PARTICLES;        // max x int
+constexpr size_t NA_POFF = NA_PLIST + (size_t)LSMPS_NOALLOC_MAX_PARTICLES * 4;    // (max / 64 + 1) x int
+constexpr size_t NA_PSCAN = NA_POFF + (((NA_GM + 1) * 4 + 255) / 256) * 256;      // scan scratch (int64)
+constexpr size_t NA_PSCAN_ELEMS = 64;  // >= scan_scratch_elems(max / 64), checked before use
+constexpr size_t NA_BYTES = NA_PSCAN + NA_PSCAN_ELEMS * 8;
 static_assert(LSMPS_NOALLOC_MAX_PARTICLES % 64 == 0, "block partials sized per 64 particles");
 __device__ __align__(256) unsigned char scratch[NA_BYTES];
 
@@ -1143,6 +1171,8 @@ extern "C" {
 int lsmps_wls_noalloc(lsmps_ctx* ctx, const lsmps_wls_params* p, double* grad3, double* lap, uint8_t* status,
                       lsmps_wls_result* out) {
   if (!ctx || !p) return LSMPS_E_ARG;
+  if (!(p->fp32_pivot_min >= 0.0 && p->fp32_pivot_min <= 1.0))
+    return fail(ctx, LSMPS_E_ARG, "fp32_pivot_min must lie in [0, 1]");
   if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "WLS needs neighbors");
   if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph: use lsmps_replay_wls");
   if (!ctx->have_field) return fail(ctx, LSMPS_E_STATE, "WLS needs a field (lsmps_set_field_*)");
@@ -1164,6 +1194,10 @@ int lsmps_wls_noalloc(lsmps_ctx* ctx, const lsmps_wls_params* p, double* grad3, 
   int* nredo = reinterpret_cast<int*>(sp + NA_NREDO);
   unsigned* bpart = reinterpret_cast<unsigned*>(sp + NA_BPART);
   int* redo = reinterpret_cast<int*>(sp + NA_REDO);
+  if (scan_scratch_elems((int64_t)NA_GM) > NA_PSCAN_ELEMS)
+    return fail(ctx, LSMPS_E_LIMIT, "lsmps_wls_noalloc: scan scratch region too small");
+  const WlsMixedWork mw{reinterpret_cast<uint8_t*>(sp + NA_PEND), reinterpret_cast<int*>(sp + NA_POFF),
+                        reinterpret_cast<int64_t*>(sp + NA_PSCAN), reinterpret_cast<int*>(sp + NA_PLIST)};
   hipStream_t st = ctx->st;
   HIPCHK(hipEventRecord(ctx->e0, st));
   if (!ctx->spos_field) {
@@ -1181,8 +1215,9 @@ int lsmps_wls_noalloc(lsmps_ctx* ctx, const lsmps_wls_params* p, double* grad3, 
     launch_wls_exact(ctx->spos.p, ctx->type_s.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, n, p->re, p->l0, p->reg, grad3, lap,
                      status, redo, nredo, counters, bpart, st, ctx->e2, ctx->e3, ctx->perm.p);
   else
-    launch_wls_mixed(ctx->spos.p, ctx->type_s.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, n, p->re, p->l0, p->reg, grad3, lap,
-                     status, redo, nredo, counters, bpart, st, ctx->e2, ctx->e3, ctx->perm.p);
+    launch_wls_mixed(ctx->spos.p, ctx->type_s.p, ctx->nbr.p, ctx->cnt.p, ctx->maxn, n, p->re, p->l0, p->reg,
+                     wls_tau(p), mw, grad3, lap, status, redo, nredo, counters, bpart, st, ctx->e2, ctx->e3,
+                     ctx->perm.p);
   if (int rc = check_launch(ctx, "wls_noalloc")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));
   unsigned long long c[6];

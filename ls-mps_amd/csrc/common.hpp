@@ -145,14 +145,14 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblk) {
 // lds needs K * (blockDim.x / 64) words.
 template <int K>
 __device__ __forceinline__ void block_count_partials(unsigned (&v)[K], unsigned* __restrict__ partials,
-                                                     unsigned* lds) {
+                                                     unsigned* lds, unsigned blk = blockIdx.x) {
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
   if (nw == 1) {
     if (lane == 0)
 #pragma unroll
-      for (int k = 0; k < K; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
+      for (int k = 0; k < K; ++k) partials[(size_t)k * gridDim.x + blk] = v[k];
     return;
   }
   if (lane == 0)
@@ -162,7 +162,7 @@ __device__ __forceinline__ void block_count_partials(unsigned (&v)[K], unsigned*
   if (threadIdx.x < K) {
     unsigned s = 0;
     for (int w = 0; w < nw; ++w) s += lds[threadIdx.x * nw + w];
-    partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+    partials[(size_t)threadIdx.x * gridDim.x + blk] = s;
   }
 }
 

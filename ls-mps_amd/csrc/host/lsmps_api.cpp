@@ -159,7 +159,7 @@ Metrics run_benchmark(const Config& c, std::vector<Particle>& ps) {
   // The reference's manufactured quadratic (src/lsmps.cpp:28).
   const lsmps_quad_field field = {{1, 2, -3, .5, .7, -.2, .4, .3, -.1, .25}, 1.8};
   check(ctx, lsmps_set_field_quadratic(ctx, &field), "field");
-  lsmps_wls_params wp = {re, c.l0, c.wls_regularization, LSMPS_WLS_FP64};
+  lsmps_wls_params wp = {re, c.l0, c.wls_regularization, LSMPS_WLS_FP64, 0.0};
   lsmps_wls_result wr;
   check(ctx, lsmps_wls(ctx, &wp, &wr), "wls");
   lsmps_wls_stats ws;

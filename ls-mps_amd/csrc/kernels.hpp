@@ -124,10 +124,23 @@ void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, 
                       double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
                       int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,
                       hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, const int* out_perm = nullptr);
+// Work buffers of the mixed WLS's FP64 rows: n flags (sorted order), the
+// exclusive scan of the per-64-row counts (ceil(n / 64) + 1 ints, the last
+// the list length), its scan scratch (scan_scratch_elems(ceil(n / 64))) and
+// the list (n ints).
+struct WlsMixedWork {
+  uint8_t* pend;
+  int* poff;
+  int64_t* scan_scratch;
+  int* list;
+};
+// tau: a row whose FP32 Cholesky pivot is not above tau times its diagonal
+// entry is computed in FP64 (k_wls_pending).
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
-                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
-                      int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,
-                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, const int* out_perm = nullptr);
+                      double re, double l0, double reg, float tau, const WlsMixedWork& w, double* grad, double* lap,
+                      uint8_t* status, int* redo, int* nredo, unsigned long long* counts, unsigned* bpart,
+                      hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
+                      const int* out_perm = nullptr);
 void launch_wls_sample_errors(const double4* spos, const uint8_t* type, const double* grad, const double* lap,
                               const uint8_t* status, const int* iperm, int64_t stride, int ns, const QuadField& q,
                               double* ge, double* le, uint8_t* st_out, uint8_t* ty_out, hipStream_t st);

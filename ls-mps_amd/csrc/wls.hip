@@ -17,10 +17,13 @@
 //    the work list of k_wls_redo (below), which runs the whole ladder with
 //    the same results.
 //  * k_wls_mixed  FP32 moments on offsets normalised by re, FMA
-//    accumulation, FP32 Cholesky. A row whose FP32 pivots lose more than
-//    three digits (pivot <= 1e-3 * diagonal), or that would need the
-//    reference's regularisation, is appended to a work list and redone by
-//    k_wls_redo in FP64 with the reference's ladder.
+//    accumulation, FP32 Cholesky. A row whose FP32 pivots are not all above
+//    tau times their diagonal entries (an ill-conditioned moment matrix,
+//    whose FP32 solution carries an error of about eps32 / pivot ratio
+//    times the linear part of the fit), or that would need the reference's
+//    regularisation, is flagged; k_wls_pending computes the flagged rows in
+//    FP64 (the reference's bits) and k_wls_redo runs the reference's
+//    ladder for those whose first FP64 factorisation fails.
 #include "kernels.hpp"
 #include "wls_core.hpp"
 
@@ -81,6 +84,87 @@ __device__ __forceinline__ double div_rn(double a, double b, double y, int& emin
 // min, max) instead of testing each against the range: 207 VGPRs and 4%
 // less time. Forcing three wavefronts per SIMD (168 VGPRs) measured 1.75 ms
 // against 1.67, four (spills) 7.2 ms.
+// One row of the FP64 WLS (the body of k_wls_exact): WLS_OK with the
+// gradient and Laplacian, WLS_NO_WEIGHT, or -1 when the row needs the
+// ladder or plain divisions (k_wls_redo).
+__device__ __forceinline__ int wls_exact_row(const double4* __restrict__ spos, const int* __restrict__ nbr,
+                                             const int* __restrict__ cnt, int maxn, int i, double re, double l0,
+                                             double (&g3)[3], double& L) {
+  const double4 pi = spos[i];
+  const int* col = nbr + (size_t)i * maxn;
+  const int kn = cnt[i];
+  const double h2 = 2 * l0;
+  double M[45], B[9], ws = 0.0;
+  int emin = 0, emax = 0;  // exponent range of the div_rn quotients
+#pragma unroll
+  for (int k = 0; k < 45; ++k) M[k] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 9; ++a) B[a] = 0.0;
+  // Prefetches past the row's end reload its last entry (no branches).
+  double4 pn = kn > 0 ? spos[col[0]] : pi;
+  int jn = col[kn > 1 ? 1 : 0];
+  for (int k = 0; k < kn; ++k) {
+    const double4 pj = pn;
+    pn = spos[jn];
+    jn = col[k + 2 < kn ? k + 2 : kn - 1];
+    const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r <= 1e-12 || r >= re) continue;
+    const double inv = 1.0 / r, h = h2 * r, g = l0 * r;
+    const double tq = div_rn(re, r, inv, emin, emax) - 1.0;  // re / r >= 1: in range
+    const double w = tq * tq;
+    if (w == 0.0) continue;
+    // h = (2 l0) r is exactly 2 g, so RN(1/h) = RN(1/g) / 2 while g and
+    // 1/g are normal numbers (g's exponent joins the range check).
+    const double yg = 1.0 / g, yh = 0.5 * yg;
+    const int eg = __builtin_amdgcn_frexp_exp(g);
+    emin = min(emin, eg);
+    emax = max(emax, eg);
+    double q[9];
+    q[0] = dx * inv;
+    q[1] = dy * inv;
+    q[2] = dz * inv;
+    q[3] = div_rn(dx * dx, h, yh, emin, emax);
+    q[4] = div_rn(dy * dy, h, yh, emin, emax);
+    q[5] = div_rn(dz * dz, h, yh, emin, emax);
+    q[6] = div_rn(dx * dy, g, yg, emin, emax);
+    q[7] = div_rn(dx * dz, g, yg, emin, emax);
+    q[8] = div_rn(dy * dz, g, yg, emin, emax);
+    const double df = pj.w - pi.w;
+    ws += w;
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      const double wq = w * q[a];
+      B[a] += div_rn(wq * df, r, inv, emin, emax);
+#pragma unroll
+      for (int c = 0; c <= a; ++c) M[tri(a, c)] += wq * q[c];
+    }
+  }
+  if (ws <= 0) return WLS_NO_WEIGHT;
+  const double yw = 1.0 / ws;
+#pragma unroll
+  for (int k = 0; k < 45; ++k) M[k] = div_rn(M[k], ws, yw, emin, emax);
+#pragma unroll
+  for (int a = 0; a < 9; ++a) B[a] = div_rn(B[a], ws, yw, emin, emax);
+  const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // Quotients within [2^-900, 2^900] (a zero has exponent 0) and finite
+  // sums (an overflow or a NaN input): else k_wls_redo divides.
+  bool fin = isfinite(ws);
+#pragma unroll
+  for (int k = 0; k < 45; ++k) fin = fin && isfinite(M[k]);
+#pragma unroll
+  for (int a = 0; a < 9; ++a) fin = fin && isfinite(B[a]);
+  const bool off = emin < -899 || emax > 900 || !fin;
+  if (off || !chol9<double>(M, 1e-14, 0.0, zero9)) return -1;
+  double sol[9];
+  chol9_solve<double>(M, B, sol);
+  g3[0] = sol[0];
+  g3[1] = sol[1];
+  g3[2] = sol[2];
+  L = (sol[3] + sol[4] + sol[5]) / l0;
+  return WLS_OK;
+}
+
 __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ spos,
                                                     const uint8_t* __restrict__ type, const int* __restrict__ nbr,
                                                     const int* __restrict__ cnt, int maxn, int n, double re, double l0,
@@ -93,88 +177,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
   if (i < n) {
     const uint8_t t = type[i];
     double g3[3] = {0.0, 0.0, 0.0}, L = 0.0;
-    int st = WLS_SKIPPED;
-    if (t != T_WALL && t != T_SPLASH) {
-      const double4 pi = spos[i];
-      const int* col = nbr + (size_t)i * maxn;
-      const int kn = cnt[i];
-      const double h2 = 2 * l0;
-      double M[45], B[9], ws = 0.0;
-      int emin = 0, emax = 0;  // exponent range of the div_rn quotients
-#pragma unroll
-      for (int k = 0; k < 45; ++k) M[k] = 0.0;
-#pragma unroll
-      for (int a = 0; a < 9; ++a) B[a] = 0.0;
-      // Prefetches past the row's end reload its last entry (no branches).
-      double4 pn = kn > 0 ? spos[col[0]] : pi;
-      int jn = col[kn > 1 ? 1 : 0];
-      for (int k = 0; k < kn; ++k) {
-        const double4 pj = pn;
-        pn = spos[jn];
-        jn = col[k + 2 < kn ? k + 2 : kn - 1];
-        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        if (r <= 1e-12 || r >= re) continue;
-        const double inv = 1.0 / r, h = h2 * r, g = l0 * r;
-        const double tq = div_rn(re, r, inv, emin, emax) - 1.0;  // re / r >= 1: in range
-        const double w = tq * tq;
-        if (w == 0.0) continue;
-        // h = (2 l0) r is exactly 2 g, so RN(1/h) = RN(1/g) / 2 while g and
-        // 1/g are normal numbers (g's exponent joins the range check).
-        const double yg = 1.0 / g, yh = 0.5 * yg;
-        const int eg = __builtin_amdgcn_frexp_exp(g);
-        emin = min(emin, eg);
-        emax = max(emax, eg);
-        double q[9];
-        q[0] = dx * inv;
-        q[1] = dy * inv;
-        q[2] = dz * inv;
-        q[3] = div_rn(dx * dx, h, yh, emin, emax);
-        q[4] = div_rn(dy * dy, h, yh, emin, emax);
-        q[5] = div_rn(dz * dz, h, yh, emin, emax);
-        q[6] = div_rn(dx * dy, g, yg, emin, emax);
-        q[7] = div_rn(dx * dz, g, yg, emin, emax);
-        q[8] = div_rn(dy * dz, g, yg, emin, emax);
-        const double df = pj.w - pi.w;
-        ws += w;
-#pragma unroll
-        for (int a = 0; a < 9; ++a) {
-          const double wq = w * q[a];
-          B[a] += div_rn(wq * df, r, inv, emin, emax);
-#pragma unroll
-          for (int c = 0; c <= a; ++c) M[tri(a, c)] += wq * q[c];
-        }
-      }
-      if (ws <= 0) {
-        st = WLS_NO_WEIGHT;
-      } else {
-        const double yw = 1.0 / ws;
-#pragma unroll
-        for (int k = 0; k < 45; ++k) M[k] = div_rn(M[k], ws, yw, emin, emax);
-#pragma unroll
-        for (int a = 0; a < 9; ++a) B[a] = div_rn(B[a], ws, yw, emin, emax);
-        const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        // Quotients within [2^-900, 2^900] (a zero has exponent 0) and
-        // finite sums (an overflow or a NaN input): else k_wls_redo divides.
-        bool fin = isfinite(ws);
-#pragma unroll
-        for (int k = 0; k < 45; ++k) fin = fin && isfinite(M[k]);
-#pragma unroll
-        for (int a = 0; a < 9; ++a) fin = fin && isfinite(B[a]);
-        const bool off = emin < -899 || emax > 900 || !fin;
-        if (!off && chol9<double>(M, 1e-14, 0.0, zero9)) {
-          double sol[9];
-          chol9_solve<double>(M, B, sol);
-          g3[0] = sol[0];
-          g3[1] = sol[1];
-          g3[2] = sol[2];
-          L = (sol[3] + sol[4] + sol[5]) / l0;
-          st = WLS_OK;
-        } else {
-          st = -1;  // the ladder (or plain divisions): k_wls_redo writes this row
-        }
-      }
-    }
+    const int st = t != T_WALL && t != T_SPLASH ? wls_exact_row(spos, nbr, cnt, maxn, i, re, l0, g3, L) : WLS_SKIPPED;
     if (st < 0) {
       redo[atomicAdd(nredo, 1)] = i;
     } else {
@@ -183,6 +186,49 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
     }
   }
   count_status(slot, bpart);
+}
+
+// The rows k_wls_mixed left to FP64 (pend[s] = 1) as one list in
+// ascending order: block b of the mixed grid (rows 64 b .. 64 b + 63) writes
+// its flagged rows from offset poff[b], the exclusive scan of its count.
+__global__ __launch_bounds__(WAVE) void k_wls_pend_list(const uint8_t* __restrict__ pend, int n,
+                                                        const int* __restrict__ poff, int* __restrict__ list) {
+  const int s = blockIdx.x * WAVE + threadIdx.x;
+  const bool p = s < n && pend[s] != 0;
+  const uint64_t mk = __builtin_amdgcn_ballot_w64(p);
+  if (p) list[poff[blockIdx.x] + __popcll(mk & ((1ull << threadIdx.x) - 1ull))] = s;
+}
+
+// The listed rows (count list_len[0]) through the same row function as
+// k_wls_exact, so they are the reference's bits; one row per lane, every
+// wavefront a full batch of 64 (the FP64 row is a latency chain, so the
+// time is the number of wavefronts over the ones resident at once). Rows
+// that need the ladder go to the k_wls_redo list. Per block: rows finished
+// here with WLS_OK, with WLS_NO_WEIGHT, and all of them.
+__global__ __launch_bounds__(WAVE) void k_wls_pending(const double4* __restrict__ spos, const int* __restrict__ list,
+                                                      const int* __restrict__ list_len, const int* __restrict__ nbr,
+                                                      const int* __restrict__ cnt, int maxn, double re, double l0,
+                                                      double* __restrict__ grad, double* __restrict__ lap,
+                                                      uint8_t* __restrict__ status, int* __restrict__ redo,
+                                                      int* __restrict__ nredo, unsigned* __restrict__ bpart,
+                                                      const int* __restrict__ out_perm) {
+  const int k = blockIdx.x * WAVE + threadIdx.x;
+  unsigned n_ok = 0, n_nw = 0, n_done = 0;
+  if (k < *list_len) {
+    const int i = list[k];
+    double g3[3] = {0.0, 0.0, 0.0}, L = 0.0;
+    const int st = wls_exact_row(spos, nbr, cnt, maxn, i, re, l0, g3, L);
+    if (st < 0) {
+      redo[atomicAdd(nredo, 1)] = i;
+    } else {
+      store_result(out_perm ? out_perm[i] : i, (uint8_t)st, g3, L, grad, lap, status);
+      n_ok = st == WLS_OK ? 1u : 0u;
+      n_nw = st == WLS_NO_WEIGHT ? 1u : 0u;
+      n_done = 1;
+    }
+  }
+  unsigned v[3] = {n_ok, n_nw, n_done};
+  block_count_partials<3>(v, bpart, nullptr);
 }
 
 // The FP64 redo of the rows the FP32 kernel could not trust, eight lanes
@@ -197,7 +243,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_exact(const double4* __restrict__ 
 //    steps of the ladder are factorised side by side by lanes 0..4 and the
 //    first that succeeds is taken.
 // A row costs about a quarter of the sequential latency; the list is short
-// (rows whose FP32 pivots lost three digits), so that latency is the time.
+// (rows whose first FP64 factorisation failed), so that latency is the time.
 constexpr int RX_LANES = 8, RX_ROWS = WAVE / RX_LANES, RX_CHUNK = 32, RX_REC = 12, RX_OWN = 7;
 
 __global__ __launch_bounds__(WAVE) void k_wls_redo(const double4* __restrict__ spos, const int* __restrict__ nbr,
@@ -344,8 +390,9 @@ __global__ __launch_bounds__(WAVE) void k_wls_redo(const double4* __restrict__ s
 //           DPP adds per particle on the combine and ran 0.81 ms at 3M;
 //           4 lanes 0.62 ms, 2 lanes 0.58 ms, measured on MI355X.)
 //  phase 2: one lane per particle factorises its 9x9 from LDS (FP32
-//           Cholesky with reciprocal square roots) and solves. Rows whose
-//           pivots lose more than three digits go to the FP64 work list.
+//           Cholesky with reciprocal square roots) and solves. Rows with a
+//           pivot at or below tau times its diagonal entry are flagged for
+//           FP64 (pend[s] = 1; every row's flag is written).
 constexpr int WLS_GROUP = 2;   // lanes per particle in phase 1
 constexpr int WLS_NACC = 55;   // 45 moments + 9 RHS + weight sum
 constexpr int WLS_STRIDE = 57; // LDS floats per particle (odd: no bank conflicts)
@@ -389,8 +436,8 @@ __device__ __forceinline__ void wls_accumulate(const double4 pj, const double4 p
 }
 
 // FP32 Cholesky + solve from the 55 sums; false when a pivot is not
-// > 1e-3 of its diagonal (or not finite).
-__device__ __forceinline__ bool wls_solve_f32(const float* __restrict__ a, float (&sol)[9]) {
+// > tau times its diagonal (or not finite).
+__device__ __forceinline__ bool wls_solve_f32(const float* __restrict__ a, float tau, float (&sol)[9]) {
   const float ws = a[54];
   if (!(ws > 0.0f)) return false;
   const float iw = 1.0f / ws;
@@ -409,7 +456,7 @@ __device__ __forceinline__ bool wls_solve_f32(const float* __restrict__ a, float
 #pragma unroll
       for (int k = 0; k < j; ++k) sm = fmaf(-m[tri(i, k)], m[tri(j, k)], sm);
       if (i == j) {
-        if (!(sm > 1e-3f * d0) || !isfinite(sm)) ok = false;
+        if (!(sm > tau * d0) || !isfinite(sm)) ok = false;
         const float r = __builtin_amdgcn_rsqf(sm);
         il[i] = r;
         m[tri(i, i)] = sm * r;
@@ -440,17 +487,16 @@ __device__ __forceinline__ bool wls_solve_f32(const float* __restrict__ a, float
 __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ spos,
                                                     const uint8_t* __restrict__ type, const int* __restrict__ nbr,
                                                     const int* __restrict__ cnt, int maxn, int n, double re, double l0,
-                                                    double* __restrict__ grad, double* __restrict__ lap,
-                                                    uint8_t* __restrict__ status, int* __restrict__ redo,
-                                                    int* __restrict__ nredo, unsigned* __restrict__ bpart,
-                                                    const int* __restrict__ out_perm) {
+                                                    float tau, double* __restrict__ grad, double* __restrict__ lap,
+                                                    uint8_t* __restrict__ status, uint8_t* __restrict__ pend,
+                                                    unsigned* __restrict__ bpart, const int* __restrict__ out_perm) {
   __shared__ float sums[WLS_HALF * WLS_STRIDE];
   const int lane = threadIdx.x;
   const double inv_re = 1.0 / re;
   const float l0s = (float)(l0 * inv_re);
   const float rmin = (float)(1e-12 * inv_re), rmin2 = rmin * rmin;
   const int s0 = (int)xcd_block(blockIdx.x, gridDim.x) * WAVE;
-  int n_ok = 0, n_skip = 0;
+  int n_ok = 0, n_skip = 0, n_pend = 0;
   for (int half = 0; half < WAVE / WLS_HALF; ++half) {
     // phase 1: WLS_GROUP lanes per particle, WAVE / WLS_GROUP particles per
     // pass, WLS_HALF / (WAVE / WLS_GROUP) passes (one with two lanes).
@@ -502,13 +548,14 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
       const int s = s0 + half * WLS_HALF + lane;
       if (s < n) {
         const uint8_t t = type[s];
+        bool fp64 = false;
         if (t == T_WALL || t == T_SPLASH) {
           const double g0[3] = {0.0, 0.0, 0.0};
           store_result(out_perm ? out_perm[s] : s, WLS_SKIPPED, g0, 0.0, grad, lap, status);
           ++n_skip;
         } else {
           float sol[9];
-          if (wls_solve_f32(sums + lane * WLS_STRIDE, sol)) {
+          if (wls_solve_f32(sums + lane * WLS_STRIDE, tau, sol)) {
             // The reference's quadratic coefficients are the ones of the
             // unscaled monomials times 2 l0 / re.
             const double g3[3] = {(double)sol[0], (double)sol[1], (double)sol[2]};
@@ -516,17 +563,20 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
             store_result(out_perm ? out_perm[s] : s, WLS_OK, g3, (double)quad / l0, grad, lap, status);
             ++n_ok;
           } else {
-            const int at = atomicAdd(nredo, 1);
-            redo[at] = s;
+            fp64 = true;
           }
         }
+        pend[s] = fp64 ? 1 : 0;
+        n_pend += fp64 ? 1 : 0;
       }
     }
     __syncthreads();
   }
-  // Per-block status counts (rows sent to the FP64 list are counted there).
-  unsigned v[2] = {(unsigned)n_ok, (unsigned)n_skip};
-  block_count_partials<2>(v, bpart, nullptr);
+  // Per-block status counts, and the count of rows flagged for FP64 (their
+  // statuses are counted where they are computed), stored at the logical
+  // block (rows s0 .. s0 + 63), which the list's offsets follow.
+  unsigned v[3] = {(unsigned)n_ok, (unsigned)n_skip, (unsigned)n_pend};
+  block_count_partials<3>(v, bpart, nullptr, (unsigned)s0 / WAVE);
 }
 
 // Quadratic field f = c0 + cx x + cy y + cz z + cxx x^2 + cyy y^2 + czz z^2
@@ -622,19 +672,29 @@ void launch_wls_exact(const double4* spos, const uint8_t* type, const int* nbr, 
 }
 
 void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, const int* cnt, int maxn, int n,
-                      double re, double l0, double reg, double* grad, double* lap, uint8_t* status, int* redo,
-                      int* nredo, unsigned long long* counts, unsigned* bpart, hipStream_t st,
-                      hipEvent_t ev_begin, hipEvent_t ev_end, const int* out_perm) {
+                      double re, double l0, double reg, float tau, const WlsMixedWork& w, double* grad, double* lap,
+                      uint8_t* status, int* redo, int* nredo, unsigned long long* counts, unsigned* bpart,
+                      hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end, const int* out_perm) {
   if (n <= 0) return;
   const unsigned gm = blocks(n, WAVE);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
-  hipLaunchKernelGGL(k_wls_mixed, dim3(gm), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, grad, lap, status,
-                     redo, nredo, bpart, out_perm);
+  hipLaunchKernelGGL(k_wls_mixed, dim3(gm), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, tau, grad, lap,
+                     status, w.pend, bpart, out_perm);
   if (ev_end) (void)hipEventRecord(ev_end, st);
-  CountSlots cs{2, {WLS_OK, WLS_SKIPPED, 0, 0, 0, 0, 0, 0}};
+  // The flagged rows as one ascending list: offsets per mixed block, then
+  // the list (its length lands in poff[gm]).
+  launch_exclusive_scan(reinterpret_cast<const int*>(bpart + 2 * (size_t)gm), gm, w.poff, w.scan_scratch, st);
+  hipLaunchKernelGGL(k_wls_pend_list, dim3(gm), dim3(WAVE), 0, st, w.pend, n, w.poff, w.list);
+  // One row per lane over the list; blocks past its length only report
+  // zero counts. Its three partial rows follow the mixed kernel's three,
+  // and one launch adds all six (the flagged-row count into an unused slot).
+  hipLaunchKernelGGL(k_wls_pending, dim3(gm), dim3(WAVE), 0, st, spos, w.list, w.poff + gm, nbr, cnt, maxn, re, l0,
+                     grad, lap, status, redo, nredo, bpart + 3 * (size_t)gm, out_perm);
+  CountSlots cs{6, {WLS_OK, WLS_SKIPPED, WLS_REDO_SLOT + 1, WLS_OK, WLS_NO_WEIGHT, WLS_REDO_SLOT, 0, 0}};
   launch_sum_counts(bpart, (int)gm, cs, counts, st);
-  // The redo list length is only known on the device: a bounded grid
-  // strides over it (typical lists are empty or a few thousand entries).
+  // Rows whose first FP64 factorisation failed: the reference's ladder.
+  // The list length is only known on the device: a bounded grid strides
+  // over it (typical lists are empty or a few hundred entries).
   const unsigned gr = blocks(n, RX_ROWS) < 1024u ? blocks(n, RX_ROWS) : 1024u;
   hipLaunchKernelGGL(k_wls_redo, dim3(gr), dim3(WAVE), 0, st, spos, nbr, cnt, maxn, redo, nredo, re, l0, reg, grad, lap,
                      status, counts, out_perm);

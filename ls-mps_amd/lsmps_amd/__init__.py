@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "liblsmps_hip.so")
 INTERIOR, FREE_SURFACE, SPLASH, NEAR_WALL, WALL = range(5)
 WLS_OK, WLS_REGULARIZED, WLS_FAILED, WLS_SKIPPED, WLS_NO_WEIGHT = range(5)
 WLS_FP64, WLS_MIXED = 0, 1
+WLS_FP32_PIVOT_DEFAULT = 0.2  # LSMPS_WLS_FP32_PIVOT_DEFAULT
 SOLVE_PLAIN, SOLVE_JACOBI, SOLVE_MIXED, SOLVE_LEARNED = range(4)
 XSTAR_QUADRATIC, XSTAR_RANDOM = 0, 1
 
@@ -46,7 +47,7 @@ class QuadField(C.Structure):
 
 
 class WlsParams(C.Structure):
-    _fields_ = [("re", _d), ("l0", _d), ("reg", _d), ("precision", _i32)]
+    _fields_ = [("re", _d), ("l0", _d), ("reg", _d), ("precision", _i32), ("fp32_pivot_min", _d)]
 
 
 class WlsResult(C.Structure):
@@ -239,8 +240,10 @@ class Pipeline:
             f = np.ascontiguousarray(field, np.float64)
             self._check(self.L.lsmps_set_field_values(self.h, _ptr(f), f.shape[0]), "set_field_values")
 
-    def wls(self, re, l0, precision=WLS_FP64, reg=1e-8):
-        p = WlsParams(re, l0, reg, precision)
+    def wls(self, re, l0, precision=WLS_FP64, reg=1e-8, fp32_pivot_min=0.0):
+        """fp32_pivot_min (WLS_MIXED): rows whose FP32 Cholesky pivot ratio is at
+        or below it are computed in FP64; 0 selects the library default."""
+        p = WlsParams(re, l0, reg, precision, fp32_pivot_min)
         s = WlsResult()
         self._check(self.L.lsmps_wls(self.h, C.byref(p), C.byref(s)), "wls")
         return as_dict(s)

@@ -249,6 +249,14 @@ def test_mixed_wls_redo_rows_equal_fp64(pipe):
     ladder = (s64 == L.WLS_REGULARIZED) | (s64 == L.WLS_FAILED) | (s64 == L.WLS_NO_WEIGHT)
     say("ladder rows", int(ladder.sum()), "redone", r32["fp64_redo"])
     assert np.all(same[ladder])
+    # A pivot-ratio threshold of 1 sends every row to FP64: the FP64 results.
+    r1 = pipe.wls(re, l0, L.WLS_MIXED, fp32_pivot_min=1.0)
+    g1, l1, s1 = pipe.wls_results()
+    assert r1["fp64_redo"] == r1["ok"] + r1["regularized"] + r1["failed"]
+    assert np.array_equal(s1, s64) and np.array_equal(g1, g64) and np.array_equal(l1, l64)
+    for bad in (-0.1, 1.5, float("nan")):
+        with pytest.raises(L.LsmpsError):
+            pipe.wls(re, l0, L.WLS_MIXED, fp32_pivot_min=bad)
 
 
 def test_kept_grid_reused_and_outgrown(pipe):
