@@ -175,6 +175,10 @@ struct lsmps_ctx {
   GraphCache graph[N_GRAPHS];
   int graph_next = 0;
   bool solver_trace = false;  // LSMPS_SOLVER_TRACE set at context creation: per-chunk solver state on stderr
+  // LSMPS_SOLVER_GRAPHS=0 at context creation: solver iterations are launched
+  // directly instead of captured and replayed (the same kernels and
+  // arguments, so the same results), for tools that cannot follow HIP graphs.
+  bool solver_graphs = true;
 
   // learned preconditioner: features of the current PPE (row order u), the
   // network (FP32 master, BF16 copy, Adam moments) and the training set
@@ -396,6 +400,7 @@ int lsmps_create(int device, lsmps_ctx** out) {
   ctx->device = device;
   // Read once here: the solve loop itself makes no libc environment calls.
   ctx->solver_trace = std::getenv("LSMPS_SOLVER_TRACE") != nullptr;
+  if (const char* g = std::getenv("LSMPS_SOLVER_GRAPHS")) ctx->solver_graphs = std::strcmp(g, "0") != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->e0) != hipSuccess || hipEventCreate(&ctx->e1) != hipSuccess ||
       hipEventCreate(&ctx->e2) != hipSuccess || hipEventCreate(&ctx->e3) != hipSuccess ||
@@ -1617,6 +1622,27 @@ int ensure_mlp_buffers(lsmps_ctx* ctx) {
 constexpr double RELIABLE_DELTA = 1e-2;
 
 template <typename T>
+int finish_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, lsmps_solve_result* out);
+
+// The iterations of run_bicgstab launched directly, `chunk` at a time, with
+// the same polling of the device state between chunks.
+template <typename T>
+int run_bicgstab_direct(lsmps_ctx* ctx, SolverBufs<T>& B, bool with_replace, int max_iter, int chunk,
+                        lsmps_solve_result* out) {
+  hipStream_t st = ctx->st;
+  const int max_launch = max_iter / chunk + 3;
+  HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int l = 0; l < max_launch && ctx->host_sc->state == SOLVE_RUNNING; ++l) {
+    for (int k = 0; k < chunk; ++k) launch_solver_iteration<T>(B, with_replace, ctx->sc.p, st);
+    if (int rc = check_launch(ctx, "solver_iteration")) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->host_sc, ctx->sc.p, sizeof(SolverScalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return finish_bicgstab<T>(ctx, B, out);
+}
+
+template <typename T>
 int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_iter, double tol, int flags, int chunk,
                  lsmps_solve_result* out) {
   hipStream_t st = ctx->st;
@@ -1629,6 +1655,7 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
   // matrix, the vectors (one block, B.r first), the partials and scalars.
   const void* key[GRAPH_KEY] = {B.sptr, B.col, B.val, B.val64, B.b, B.scale, B.y, B.r, B.partials, ctx->sc.p,
                                 reinterpret_cast<const void*>((uintptr_t)ctx->sell_nnz), B.pcol, B.ptab};
+  if (!ctx->solver_graphs) return run_bicgstab_direct(ctx, B, with_replace, max_iter, chunk, out);
   GraphCache* hitp = nullptr;
   for (auto& c : ctx->graph)
     if (c.exec && c.m == B.m && c.chunk == chunk && c.with_replace == (int)with_replace &&
@@ -1666,6 +1693,13 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
                    s.it, s.state, s.rn, s.rn / s.thresh, s.rho, s.alpha, s.omega, s.beta, s.n_replace);
     }
   }
+  return finish_bicgstab<T>(ctx, B, out);
+}
+
+// The final residual and error of a solve, and its result record.
+template <typename T>
+int finish_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, lsmps_solve_result* out) {
+  hipStream_t st = ctx->st;
   launch_solver_final<T>(B, ctx->xe.p, ctx->xsol.p, ctx->sc.p, st);
   if (int rc = check_launch(ctx, "solver_final")) return rc;
   HIPCHK(hipEventRecord(ctx->e1, st));

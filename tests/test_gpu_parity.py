@@ -450,6 +450,35 @@ def test_deterministic(pipe):
     assert outs[0][2] == outs[1][2] and np.array_equal(outs[0][3], outs[1][3])
 
 
+def test_solver_without_graphs_gives_the_same_solves():
+    """LSMPS_SOLVER_GRAPHS=0 (read at context creation) launches the solver's
+    iterations directly instead of replaying captured graphs: the same
+    kernels and arguments, so every method gives the same iterations and the
+    same solution bit for bit."""
+    xyz, t = L.generate_case("rain_injection", 30000, 2, 0.01)
+    runs = []
+    for graphs in ("1", "0"):
+        os.environ["LSMPS_SOLVER_GRAPHS"] = graphs
+        try:
+            p = L.Pipeline(0)
+        finally:
+            del os.environ["LSMPS_SOLVER_GRAPHS"]
+        try:
+            p.upload(xyz, t)
+            p.neighbors_classify(0.021, 96, 0.01)
+            p.assemble(0.021, 10 ** 9, L.XSTAR_RANDOM, 5)
+            out = []
+            for method in (L.SOLVE_PLAIN, L.SOLVE_JACOBI, L.SOLVE_MIXED):
+                st = p.solve(method, 5000, 1e-8, chunk=16)
+                out.append((st["iterations"], st["converged"], st["final_residual"], p.solution().copy()))
+            runs.append(out)
+        finally:
+            p.close()
+    say("graphs vs direct:", [(a[0], b[0]) for a, b in zip(*runs)])
+    for a, b in zip(*runs):
+        assert a[:3] == b[:3] and np.array_equal(a[3], b[3])
+
+
 def test_far_outliers_take_the_compressed_grid(pipe):
     """Particles 10^6 re and 3-4 x 10^5 re away from the rest (a box far too
     large for the dense grid): the build compresses the occupied cell
