@@ -202,45 +202,47 @@ __device__ __forceinline__ double sell_dot_packed(const int* __restrict__ sptr, 
   double acc = 0.0;
   int k = 0;
   if constexpr (sizeof(TM) == 4) {
-    // FP32 values: four entries per step, and the next step's column and
-    // value loads issued before this step's gathers are waited on (3.7% per
-    // mixed iteration at 3M; the FP64 product measured 1.7% slower with it,
-    // so it keeps the plain loop below). Same products, same order.
-    if (R.width >= 4) {
-      unsigned pk[4];
-      TM v[4];
+    // FP32 values: eight entries per step, and the next step's column and
+    // value loads issued before this step's gathers are waited on (386 ->
+    // 361 us per mixed iteration at 3M; 4 or 12 entries per step measured
+    // slower, and the FP64 product slower with either change, so it keeps
+    // the plain loop below). Same products, same order.
+    constexpr int NB = 8;
+    if (R.width >= NB) {
+      unsigned pk[NB];
+      TM v[NB];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < NB; ++j) {
         const int e = R.base + j * SLICE;
         pk[j] = pcol[e];
         v[j] = val[e];
       }
-      for (;; k += 4) {
-        double xv[4];
+      for (;; k += NB) {
+        double xv[NB];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < NB; ++j)
           xv[j] = x[__shfl(tab, (int)(pk[j] >> SELL_PACK_OFF_BITS)) + (int)(pk[j] & SELL_PACK_OFF_MASK)];
-        const bool more = k + 8 <= R.width;
-        unsigned pn[4];
-        TM vn[4];
+        const bool more = k + 2 * NB <= R.width;
+        unsigned pn[NB];
+        TM vn[NB];
         if (more) {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int e = R.base + (k + 4 + j) * SLICE;
+          for (int j = 0; j < NB; ++j) {
+            const int e = R.base + (k + NB + j) * SLICE;
             pn[j] = pcol[e];
             vn[j] = val[e];
           }
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc = fma((double)v[j], xv[j], acc);
+        for (int j = 0; j < NB; ++j) acc = fma((double)v[j], xv[j], acc);
         if (!more) break;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NB; ++j) {
           pk[j] = pn[j];
           v[j] = vn[j];
         }
       }
-      k += 4;
+      k += NB;
     }
   }
   // Four entries per step, loads first (the table exchange is a wavefront
