@@ -192,7 +192,9 @@ enum {
   LSMPS_SOLVE_PLAIN = 0,   /* FP64, unpreconditioned, the reference's stopping rule */
   LSMPS_SOLVE_JACOBI = 1,  /* FP64, right Jacobi, true-residual check */
   LSMPS_SOLVE_MIXED = 2,   /* FP32 matrix, FP64 Krylov vectors, FP64 reliable
-                              updates and restarts, right Jacobi */
+                              updates and restarts, right Jacobi; FP64 JACOBI
+                              continues from its iterate when it does not
+                              converge or a reliable update stalls (fallback) */
   LSMPS_SOLVE_LEARNED = 3, /* MIXED with the MLP diagonal and initial guess
                               (lsmps_mlp_set_params first); plain MIXED when
                               the guess raises the residual */
@@ -206,7 +208,8 @@ typedef struct {
 } lsmps_solve_params;
 typedef struct {
   int32_t iterations, converged, state, replacements;
-  int32_t fallback;     /* LEARNED: 1 when the Jacobi path was used */
+  int32_t fallback;     /* MIXED, LEARNED: 1 when FP64 JACOBI finished the solve (LEARNED: also
+                           when the guess raised the residual and plain MIXED ran) */
   double initial_residual, final_residual, relative_l2_error, max_error;
   float ms;             /* GPU time from setup to the final residual */
   int32_t breakdown_restarts; /* JACOBI, MIXED, LEARNED: restarts from the true residual

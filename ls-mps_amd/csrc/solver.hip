@@ -499,6 +499,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_residual(const double* _
       sc->thresh = sc->tol * (sc->bnorm > 1.0 ? sc->bnorm : 1.0);
       sc->init_res = sqrt(tot[0]);
       sc->rn = sc->rn_max = sc->init_res;
+      sc->rn_best = sc->init_res;
       sc->rho_prev = sc->alpha = sc->omega = 1.0;
       sc->rho = tot[0];
       sc->beta = sc->rho;
@@ -530,6 +531,14 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_residual(const double* _
         sc->state = SOLVE_CONVERGED;
         return;
       }
+      // A reliable update that leaves the true residual above stall times
+      // the best one so far: the FP32 iteration matrix no longer drives the
+      // refinement (its error is as large as what is left to remove).
+      if (kind == 1 && (sc->flags & SOLVE_STALL_CHECK) && !(rn <= sc->stall * sc->rn_best)) {
+        sc->state = SOLVE_STALLED;
+        return;
+      }
+      if (rn < sc->rn_best) sc->rn_best = rn;
       sc->rho = tot[1];
       if (restart) sc->rho_prev = sc->alpha = sc->omega = 1.0;
       if (fabs(sc->rho) < 1e-30 || fabs(sc->omega) < 1e-30) {
@@ -606,12 +615,13 @@ __global__ __launch_bounds__(256) void k_y_from_x(const double* __restrict__ x0,
   if (i < m) y[i] = x0 ? (scale ? x0[i] / scale[i] : x0[i]) : 0.0;
 }
 
-__global__ void k_solver_setup(SolverScalars* sc, int max_iter, double tol, int flags, double delta) {
+__global__ void k_solver_setup(SolverScalars* sc, int max_iter, double tol, int flags, double delta, double stall) {
   if (threadIdx.x == 0) {
     sc->max_iter = max_iter;
     sc->tol = tol;
     sc->flags = flags;
     sc->delta = delta;
+    sc->stall = stall;
     sc->max_err = 0.0;
     sc->state = SOLVE_RUNNING;
   }
@@ -702,9 +712,9 @@ void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st) {
 
 template <typename TM>
 void launch_solver_init(const SolverBufs<TM>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,
-                        int flags, double delta, hipStream_t st) {
+                        int flags, double delta, double stall, hipStream_t st) {
   const int m = B.m;
-  hipLaunchKernelGGL(k_solver_setup, dim3(1), dim3(64), 0, st, sc, max_iter, tol, flags, delta);
+  hipLaunchKernelGGL(k_solver_setup, dim3(1), dim3(64), 0, st, sc, max_iter, tol, flags, delta, stall);
   if (m > 0) hipLaunchKernelGGL(k_y_from_x, dim3(blocks(m, 256)), dim3(256), 0, st, x0, B.scale, m, B.y);
   const unsigned g = solver_grid(m);
   hipLaunchKernelGGL(k_bicg_true_residual<true>, dim3(g), dim3(SOLVER_THREADS), 0, st, B.sptr, B.col, B.val64, m,
@@ -742,9 +752,9 @@ void launch_solver_final(const SolverBufs<TM>& B, const double* xe, double* x, S
 }
 
 template void launch_solver_init<float>(const SolverBufs<float>&, const double*, SolverScalars*, int, double, int, double,
-                                        hipStream_t);
+                                        double, hipStream_t);
 template void launch_solver_init<double>(const SolverBufs<double>&, const double*, SolverScalars*, int, double, int,
-                                         double, hipStream_t);
+                                         double, double, hipStream_t);
 template void launch_solver_iteration<float>(const SolverBufs<float>&, bool, SolverScalars*, hipStream_t);
 template void launch_solver_iteration<double>(const SolverBufs<double>&, bool, SolverScalars*, hipStream_t);
 template void launch_solver_final<float>(const SolverBufs<float>&, const double*, double*, SolverScalars*, hipStream_t);

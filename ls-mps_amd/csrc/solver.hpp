@@ -13,6 +13,7 @@ enum : int {
   SOLVE_BREAKDOWN = 3,
   SOLVE_MAXITER = 4,
   SOLVE_RESTARTING = 5,   // a breakdown inside this iteration: the rest of it is skipped, the restart follows
+  SOLVE_STALLED = 6,      // a reliable update's true residual missed stall x the best so far (SOLVE_STALL_CHECK)
 };
 enum : int {
   SOLVE_S_EXIT = 1,      // stop on ||s|| as the reference does
@@ -20,6 +21,8 @@ enum : int {
   SOLVE_TRUE_CHECK = 4,  // confirm convergence with the true FP64 residual
   SOLVE_RESTART = 8,     // restart the Krylov space (r0 = r, p = v = 0) at each replacement
   SOLVE_BREAK_RESTART = 16,  // on a rho, omega or r0.v breakdown, restart from the true residual of the iterate
+  SOLVE_STALL_CHECK = 32,    // end the solve (SOLVE_STALLED) when a reliable update does not lower the true residual
+                             // below stall x the best one so far: the FP32 matrix no longer carries the refinement
 };
 // Breakdown restarts allowed per solve (SOLVE_BREAK_RESTART) before the
 // solve ends in SOLVE_BREAKDOWN.
@@ -32,6 +35,7 @@ struct SolverScalars {
   double tol, bnorm, thresh, init_res;
   double sn, rn, rn_max, delta;
   double final_res, rel_l2, max_err;
+  double rn_best, stall;  // SOLVE_STALL_CHECK: smallest true residual so far, the factor
   int it, max_iter, state, iterations;
   int flags, replace, n_replace, n_break;  // replace: 1 reliable, 2 convergence check, 3 breakdown restart
 };
@@ -76,7 +80,7 @@ void launch_sell_scale(const int* col, const double* val64, const double* scale,
 void launch_inv_diag(const double* diag, int m, double* s, hipStream_t st);
 template <typename TM>
 void launch_solver_init(const SolverBufs<TM>& B, const double* x0, SolverScalars* sc, int max_iter, double tol,
-                        int flags, double delta, hipStream_t st);
+                        int flags, double delta, double stall, hipStream_t st);
 template <typename TM>
 void launch_solver_iteration(const SolverBufs<TM>& B, bool with_replace, SolverScalars* sc, hipStream_t st);
 template <typename TM>

@@ -62,8 +62,22 @@ def test_industrial_mix_3m_bitexact():
         A = p.ppe()
         ss = p.solve(L.SOLVE_MIXED, 20000, 1e-8)
         x = p.solution()
+        # The reference's own smooth x* = x^2 + y^2: |b| = 4.9e5, so the
+        # target residual is far below what the FP32 matrix resolves on
+        # |A||x|; the mixed refinement stalls and FP64 Jacobi finishes from
+        # its iterate (fallback), which meets 1e-8 on the true residual.
+        p.assemble(RE, 1 << 30, L.XSTAR_QUADRATIC, 0)
+        Aq = p.ppe()
+        sq = p.solve(L.SOLVE_MIXED, 20000, 1e-8)
+        xq = p.solution()
     finally:
         p.close()
+    say("x* = x^2 + y^2, mixed solve:", sq)
+    assert sq["converged"] == 1 and sq["fallback"] == 1
+    axq = np.add.reduceat(Aq["val"] * xq[Aq["col"]], Aq["row"][:-1])
+    resq = np.linalg.norm(Aq["b"] - axq)
+    assert resq <= 1e-8 * max(1.0, np.linalg.norm(Aq["b"])) * 1.01, resq
+    del Aq, xq, axq
     say("GPU done: pairs", int(row[-1]), "unknowns", len(A["b"]), "mixed solve", ss, f"{time.time() - t0:.1f} s")
     # Mixed WLS. An FP32 row's Laplacian error grows as its smallest
     # Cholesky pivot ratio falls: the curvature is a small part of the fit
