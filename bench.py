@@ -199,10 +199,16 @@ def _cpu_baseline_timed(ref, xyz, types, l0, maxn, n, host):
     if ref is not None:
         # wls_samples = n makes the reference's wls_test visit every particle
         # (stride 1), so its WLS stage covers the whole workload.
-        ref.setup_particles(xyz, types, l0, 2.1, maxn=maxn, wls_samples=n, max_unknowns=n)
+        # max_iter 10 and tol 0: the reference's solve_ppe runs exactly ten
+        # BiCGStab iterations on the full PPE (a whole solve would take an
+        # hour on one core); its time per iteration is reported beside the
+        # GPU's.
+        ref.setup_particles(xyz, types, l0, 2.1, maxn=maxn, wls_samples=n, max_unknowns=n, max_iter=10, tol=0.0)
         out = np.zeros(4)
         ref.L.ref_time_stages(out.ctypes.data_as(C.c_void_p))
         cpu_ms = out[0] + out[1] + out[2]
+        sv = ref.solve()
+        per_it = sv["solve_ms"] / max(1, sv["iterations"])
         return dict({
             "value": round(n / cpu_ms / 1e3, 4),
             "unit": "Mparticles/s",
@@ -210,6 +216,9 @@ def _cpu_baseline_timed(ref, xyz, types, l0, maxn, n, host):
             "precision": "FP64 WLS (compare value_fp64)",
             "stage_ms": {"neighbors": round(out[0], 1), "classify": round(out[1], 1), "wls_all": round(out[2], 1),
                          "assemble_ppe": round(out[3], 1)},
+            "ppe_bicgstab": {"iterations": sv["iterations"], "ms_per_iteration": round(per_it, 1),
+                             "note": "the reference's solve_ppe (unpreconditioned FP64 BiCGStab on its CSR) over "
+                                     "all unknowns, ten iterations, the final residual product included"},
             "sample": f"the full {n}-particle workload through the reference's src/lsmps.cpp (g++ -O3, its own "
                       f"flags), one thread pinned to core {host['pinned_core']}: neighbors {out[0]:.0f} ms + "
                       f"classify {out[1]:.0f} ms + wls_test over all particles {out[2]:.0f} ms; its assemble_ppe "
