@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LSMPS_HIP_ABI_VERSION 5
+#define LSMPS_HIP_ABI_VERSION 6
 
 enum {
   LSMPS_OK = 0,
@@ -30,6 +30,9 @@ enum {
   LSMPS_E_HIP = -2,    /* HIP runtime error (message in lsmps_last_error) */
   LSMPS_E_STATE = -3,  /* stage called before the stage it depends on */
   LSMPS_E_LIMIT = -4,  /* size beyond a documented limit (int32 indices, grid) */
+  LSMPS_E_CHECK = -5,  /* a host-side self-check failed (LSMPS_CHECK_BUILD=1 at context creation:
+                          after each neighbor build the cell list and neighbor lists are
+                          downloaded and their invariants checked; message in lsmps_last_error) */
 };
 
 /* Particle types, numbered as the reference's ParticleType (include/lsmps.hpp:8). */

@@ -179,6 +179,9 @@ struct lsmps_ctx {
   // directly instead of captured and replayed (the same kernels and
   // arguments, so the same results), for tools that cannot follow HIP graphs.
   bool solver_graphs = true;
+  // LSMPS_CHECK_BUILD=1 at context creation: every neighbor build is checked
+  // on the host (check_build), failing with LSMPS_E_CHECK.
+  bool check_build = false;
 
   // learned preconditioner: features of the current PPE (row order u), the
   // network (FP32 master, BF16 copy, Adam moments) and the training set
@@ -401,6 +404,7 @@ int lsmps_create(int device, lsmps_ctx** out) {
   // Read once here: the solve loop itself makes no libc environment calls.
   ctx->solver_trace = std::getenv("LSMPS_SOLVER_TRACE") != nullptr;
   if (const char* g = std::getenv("LSMPS_SOLVER_GRAPHS")) ctx->solver_graphs = std::strcmp(g, "0") != 0;
+  if (const char* g = std::getenv("LSMPS_CHECK_BUILD")) ctx->check_build = std::strcmp(g, "0") != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->e0) != hipSuccess || hipEventCreate(&ctx->e1) != hipSuccess ||
       hipEventCreate(&ctx->e2) != hipSuccess || hipEventCreate(&ctx->e3) != hipSuccess ||
@@ -647,6 +651,54 @@ int compress_grid(lsmps_ctx* ctx, int n, int64_t g[3]) {
   return LSMPS_OK;
 }
 
+// Host-side invariants of a finished build (LSMPS_CHECK_BUILD): the cell
+// starts are an exclusive scan ending at n; every sorted slot lies in the
+// range of its cell; perm and iperm are inverse permutations; every list
+// has at most maxn entries, all in [0, n), strictly ascending (the
+// reference's visiting order is ascending in sorted slots), never the row
+// itself and within the list radius of it in FP64; the lengths add up to the
+// reported pair count. Downloads everything: a debugging aid, not for
+// production sizes.
+int check_build(lsmps_ctx* ctx, int n, int64_t ncells, int maxn, uint64_t pairs) {
+  hipStream_t st = ctx->st;
+  std::vector<int> cs((size_t)ncells + 1), sk(n), pm(n), ip(n), ct(n), nb((size_t)n * (size_t)std::max(1, maxn));
+  std::vector<double4> sp(n);
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(cs.data(), ctx->cstart.p, cs.size() * sizeof(int), hipMemcpyDeviceToHost));
+  if (n) {
+    HIPCHK(hipMemcpy(sk.data(), ctx->skey.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pm.data(), ctx->perm.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ip.data(), ctx->iperm.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ct.data(), ctx->cnt.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sp.data(), ctx->spos.p, (size_t)n * sizeof(double4), hipMemcpyDeviceToHost));
+    if (maxn > 0) HIPCHK(hipMemcpy(nb.data(), ctx->nbr.p, nb.size() * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  auto bad = [&](const std::string& what) { return fail(ctx, LSMPS_E_CHECK, "build check: " + what); };
+  if (cs[0] != 0 || cs[(size_t)ncells] != n) return bad("cell starts do not run from 0 to n");
+  for (int64_t c = 0; c < ncells; ++c)
+    if (cs[c + 1] < cs[c]) return bad("cell starts decrease at cell " + std::to_string(c));
+  const double r2 = ctx->re * ctx->re;
+  uint64_t total = 0;
+  for (int s = 0; s < n; ++s) {
+    const int c = sk[s];
+    if (c < 0 || c >= ncells || s < cs[c] || s >= cs[c + 1]) return bad("slot " + std::to_string(s) + " outside its cell");
+    if (pm[s] < 0 || pm[s] >= n || ip[pm[s]] != s) return bad("perm/iperm not inverse at slot " + std::to_string(s));
+    const int k = ct[s];
+    if (k < 0 || k > maxn) return bad("list length out of range at slot " + std::to_string(s));
+    total += (uint64_t)k;
+    const int* row = nb.data() + (size_t)s * (size_t)maxn;
+    for (int q = 0; q < k; ++q) {
+      const int j = row[q];
+      if (j < 0 || j >= n || j == s || (q && j <= row[q - 1]))
+        return bad("list of slot " + std::to_string(s) + " not ascending in [0, n) without itself");
+      const double dx = sp[j].x - sp[s].x, dy = sp[j].y - sp[s].y, dz = sp[j].z - sp[s].z;
+      if (!(dx * dx + dy * dy + dz * dz < r2)) return bad("pair beyond the list radius at slot " + std::to_string(s));
+    }
+  }
+  if (total != pairs) return bad("list lengths do not add up to the pair count");
+  return LSMPS_OK;
+}
+
 // Cell sort + one neighbor pass (+ classification when classify).
 // re: classification radius; lists hold every pair within re + skin.
 int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool classify, double l0,
@@ -818,6 +870,10 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   lsmps_cnl_stats cs;
   fill_cnl_stats(ctx, hist, c[0], &cs);
   if (cs.pairs > (uint64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "neighbor pairs beyond int32 indexing");
+  if (ctx->check_build) {
+    ctx->re = rl;  // the list radius the check tests against
+    if (int rc = check_build(ctx, n, ncells, maxn, cs.pairs)) return rc;
+  }
   cs.ms = elapsed(ctx);
   cs.kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
   ctx->re = rl;

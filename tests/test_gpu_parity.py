@@ -479,6 +479,47 @@ def test_solver_without_graphs_gives_the_same_solves():
         assert a[:3] == b[:3] and np.array_equal(a[3], b[3])
 
 
+def test_build_self_checks():
+    """LSMPS_CHECK_BUILD=1: every neighbor build downloads its cell list and
+    lists and checks their invariants on the host (cell starts an exclusive
+    scan to n, each slot inside its cell, perm and iperm inverse, every list
+    ascending in [0, n) without the row itself and within the list radius,
+    lengths adding up to the pair count). Exercised on the build sequences
+    the tests and the timeline run: a fused build, the same particles again
+    (kept grid, sorted reads), motion, a skin build, appended particles,
+    outflow removal, and a scene with far outliers (compressed grid)."""
+    os.environ["LSMPS_CHECK_BUILD"] = "1"
+    try:
+        p = L.Pipeline(0)
+    finally:
+        del os.environ["LSMPS_CHECK_BUILD"]
+    try:
+        l0 = 0.01
+        re = 2.1 * l0
+        xyz, t = L.generate_case("wall_film", 20000, 3, l0)
+        say("build checks", len(xyz))
+        p.upload(xyz, t)
+        p.neighbors_classify(re, 96, l0)
+        p.neighbors_classify(re, 96, l0)  # kept grid, sorted reads
+        rng = np.random.default_rng(2)
+        u = rng.uniform(-1.0, 1.0, xyz.shape)
+        p.set_velocities(u)
+        p.advance(0.1 * l0)
+        p.neighbors_classify(re, 96, l0)  # moved particles on the kept grid
+        p.advance(0.1 * l0)
+        p.neighbors_skin(re, 0.2 * re, 128, l0, True)
+        p.append(xyz[:500] + np.array([0.0, 0.0, 0.5 * l0]), t[:500], u[:500])
+        p.neighbors_skin(re, 0.2 * re, 128, l0, True)
+        lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+        p.remove_outside(lo, hi - np.array([0.1, 0.0, 0.0]))
+        p.neighbors_classify(re, 96, l0)
+        far = np.concatenate([xyz, xyz[:1] + np.array([[1e6 * re, 0.0, 0.0]])])
+        p.upload(far, np.concatenate([t, t[:1]]))
+        p.neighbors_classify(re, 96, l0)  # compressed grid
+    finally:
+        p.close()
+
+
 def test_far_outliers_take_the_compressed_grid(pipe):
     """Particles 10^6 re and 3-4 x 10^5 re away from the rest (a box far too
     large for the dense grid): the build compresses the occupied cell
