@@ -1871,7 +1871,10 @@ int solve_mixed(lsmps_ctx* ctx, const double* scale, const double* x0, bool prec
   bind_vectors(B, ctx->vec64.p, m);
   if (int rc = ensure_packed(ctx)) return rc;
   bind_packed(ctx, B);
-  const int flags = SOLVE_RELIABLE | SOLVE_TRUE_CHECK | SOLVE_RESTART | SOLVE_BREAK_RESTART | SOLVE_STALL_CHECK;
+  // tol = 0 asks for exactly max_iter iterations (no hand-over), so the
+  // stall rule applies only to solves with a tolerance.
+  const int flags = SOLVE_RELIABLE | SOLVE_TRUE_CHECK | SOLVE_RESTART | SOLVE_BREAK_RESTART |
+                    (tol > 0 ? SOLVE_STALL_CHECK : 0);
   if (precheck && x0) {
     launch_solver_init<float>(B, x0, ctx->sc.p, max_iter, tol, flags, RELIABLE_DELTA, RELIABLE_STALL, st);
     if (int rc = check_launch(ctx, "solver_init")) return rc;
