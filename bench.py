@@ -7,7 +7,7 @@ block, 20% wall film 2-4 l0 thick on a plane tilted 12 deg, 10% Poisson-disk
 splash at density 0.2 / l0^3), l0 = 0.004, re = 2.1 l0, max_neighbors 128.
 Particles are uploaded to HBM before timing. One step = cell-list neighbor
 search + classification + WLS gradient/Laplacian of a quadratic field for
-every particle (FP32 moments, FP64 redo of weak rows); the timed loop runs
+every particle (FP32 moments, FP64 for ill-conditioned rows); the timed loop runs
 K steps, `value` = particles / step time (Mparticles/s).
 
 Also measured, outside the timed steps: PPE assembly over all fluid
@@ -299,8 +299,9 @@ def main():
         return wall, {key: float(np.mean(v)) if v else 0.0 for key, v in stage.items()}, cs, ts, wr
 
     # The step at the reference's precision (FP64 WLS, bit-exact with
-    # wls_test) and with the mixed WLS (FP32 moments, FP64 redo of the rows
-    # the reference would regularise); the same neighbor pass in both.
+    # wls_test) and with the mixed WLS (FP32 moments; rows whose FP32 pivots
+    # fall to 0.2 of their diagonal, which include those the reference would
+    # regularise, in FP64); the same neighbor pass in both.
     run_steps(L.WLS_FP64, args.warmup)
     log(f"warmup done ({args.warmup}); timing {args.steps} FP64 steps")
     wall64, avg64, cs, ts, wr64 = run_steps(L.WLS_FP64, args.steps)
@@ -432,7 +433,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 moments (f64 positions, f64 redo of the rows the reference regularises); value_fp64: f64",
+        "dtype": "f32 moments (f64 positions; f64 rows where the f32 pivot ratio is <= 0.2); value_fp64: f64",
         "data": "synthetic",
         "config": {"workload": f"industrial_mix {n} particles: 70% jittered bulk, 20% tilted wall film 2-4 l0, "
                                f"10% Poisson-disk splash at 0.2/l0^3; l0=0.004, re=2.1 l0, max_neighbors=128; "

@@ -126,7 +126,7 @@ typedef struct {
   uint64_t ok, regularized, failed, skipped;
   uint64_t fp64_redo; /* MIXED: rows computed in FP64 */
   float ms;
-  float kernel_ms;    /* GPU time of the main WLS kernel alone (no field, no FP64 redo) */
+  float kernel_ms;    /* GPU time of the main WLS kernel alone (no field pass, no FP64 rows of MIXED) */
 } lsmps_wls_result;
 /* Gradient and Laplacian for every particle except Wall and Splash. */
 int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out);

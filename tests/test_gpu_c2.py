@@ -6,7 +6,7 @@ layers, a film of 3 l0 on it and droplet spheres of radius U(2, 4) l0 above
 it, so the neighbor counts are heavy-tailed (5-60). On it:
 - neighbor lists and types (reference rules, absolute-z NearWall) equal the
   oracle's bit for bit;
-- the mixed WLS (FP32 moments, FP64 redo of untrusted rows) stays within
+- the mixed WLS (FP32 moments, FP64 for ill-conditioned rows) stays within
   FP32 accuracy of the oracle's FP64 operator on every solved row;
 - the PPE over all fluid unknowns equals the oracle's bit for bit;
 - the mixed solve (FP32 matrix, FP64 vectors and true-residual refinement)
