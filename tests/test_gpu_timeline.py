@@ -10,6 +10,7 @@ order). Classification with skin lists counts only neighbors within re, so
 it equals the classification without a skin. The timeline driver conserves
 particles (start + injected - removed) and every solve converges.
 """
+import os
 import sys
 
 import numpy as np
@@ -267,8 +268,16 @@ def test_skin_lists_after_motion_match_oracle_fresh_lists():
 
 
 def test_timeline_conserves_particles_and_converges():
+    """The timeline's own sequence (skin builds after injections and outflow
+    removal, motion in between) with every build checked on the host
+    (LSMPS_CHECK_BUILD=1: cell list and lists downloaded, invariants
+    asserted inside the library)."""
     say("timeline")
-    s, rows = timeline.run(particles=40000, steps=25, inject=400, rebuild=10, l0=0.01, verbose=True)
+    os.environ["LSMPS_CHECK_BUILD"] = "1"
+    try:
+        s, rows = timeline.run(particles=40000, steps=25, inject=400, rebuild=10, l0=0.01, verbose=True)
+    finally:
+        del os.environ["LSMPS_CHECK_BUILD"]
     say("summary", s)
     assert s["all_converged"]
     assert s["particles_end"] == s["particles_start"] + s["injected"] - s["removed"]
