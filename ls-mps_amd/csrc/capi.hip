@@ -1757,8 +1757,10 @@ int run_bicgstab(lsmps_ctx* ctx, SolverBufs<T>& B, const double* x0, int max_ite
     HIPCHK(hipStreamSynchronize(st));
     if (ctx->solver_trace) {
       const SolverScalars& s = *ctx->host_sc;
-      std::fprintf(stderr, "[solver] it=%d state=%d rn=%.6e rn/thr=%.3e rho=%.6e alpha=%.6e omega=%.6e beta=%.6e repl=%d\n",
-                   s.it, s.state, s.rn, s.rn / s.thresh, s.rho, s.alpha, s.omega, s.beta, s.n_replace);
+      std::fprintf(stderr, "[solver] it=%d state=%d rn=%.6e rn/thr=%.3e rho=%.6e alpha=%.6e omega=%.6e beta=%.6e repl=%d "
+                   "fp32_floor/thr=%.3e\n",
+                   s.it, s.state, s.rn, s.rn / s.thresh, s.rho, s.alpha, s.omega, s.beta, s.n_replace,
+                   s.fp32_floor / s.thresh);
     }
   }
   return finish_bicgstab<T>(ctx, B, out);

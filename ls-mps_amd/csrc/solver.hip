@@ -23,7 +23,7 @@
 //   k_bicg_p   p = r + beta (p - omega v)
 //   k_bicg_v   v = A p, partial r0.v
 //   k_bicg_s   alpha = rho / r0.v, s = r - alpha v
-//   k_bicg_t   t = A s, partial s.s, t.t, t.s
+//   k_bicg_t   t = A s (slices in reverse order), partial s.s, t.t, t.s
 //   k_bicg_xr  omega = t.s / t.t, y += alpha p + omega s (FP64),
 //              r = s - omega t, partial r.r, r0.r     -> k_fin_iter
 // Every block of k_bicg_s and k_bicg_xr adds the previous launch's block
@@ -57,6 +57,23 @@
 namespace lsmps_hip {
 
 constexpr int SOLVER_THREADS = 256;
+
+// Vector streams of the iteration (p, s, r, y, v, t and r0 in the streaming
+// kernels) use non-temporal accesses, so they displace less of the
+// iteration matrix from the memory-side cache (MALL, 256 MB): the two
+// products of an iteration sweep the matrix in opposite directions
+// (k_bicg_v forward, k_bicg_t backward), and each starts on the part the
+// other streamed last. At 3M (FP32 matrix 0.52 GB, FP64 0.87 GB) this took
+// 6% off an iteration in both precisions. The gathered vector entries stay
+// ordinary loads (each is read ~30 times, from L2).
+template <typename T>
+__device__ __forceinline__ T nt_load(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ __forceinline__ void nt_store(T v, T* p) {
+  __builtin_nontemporal_store(v, p);
+}
 constexpr int SLICE = 64;
 
 // The state word is read by every block of k_bicg_s and k_bicg_xr while
@@ -134,12 +151,15 @@ __device__ __forceinline__ SellRow sell_row(const int* __restrict__ sptr, int ro
 
 // (A S y)_row with the FP64 matrix, S applied at the gathered column (S = 1
 // when null); the products added in storage order, four entries per step
-// with their loads first (a serial gather chain otherwise).
+// with their loads first (a serial gather chain otherwise). ABS: also
+// sum |a s y| over the row into *abs_sum (the scale of the row's product,
+// against which an FP32 copy of A S rounds).
+template <bool ABS = false>
 __device__ __forceinline__ double sell_dot_scaled(const int* __restrict__ sptr, const int* __restrict__ col,
                                                   const double* __restrict__ val, const double* __restrict__ scale,
-                                                  const double* __restrict__ y, int row) {
+                                                  const double* __restrict__ y, int row, double* abs_sum = nullptr) {
   const SellRow R = sell_row(sptr, row);
-  double acc = 0.0;
+  double acc = 0.0, ab = 0.0;
   int k = 0;
   for (; k + 4 <= R.width; k += 4) {
     int c[4];
@@ -156,13 +176,20 @@ __device__ __forceinline__ double sell_dot_scaled(const int* __restrict__ sptr, 
       sv[j] = scale ? scale[c[j]] : 1.0;
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc = fma(v[j], scale ? sv[j] * yv[j] : yv[j], acc);
+    for (int j = 0; j < 4; ++j) {
+      const double sy = scale ? sv[j] * yv[j] : yv[j];
+      acc = fma(v[j], sy, acc);
+      if (ABS) ab += fabs(v[j] * sy);
+    }
   }
   for (; k < R.width; ++k) {
     const int e = R.base + k * SLICE;
     const int c = col[e];
-    acc = fma(val[e], scale ? scale[c] * y[c] : y[c], acc);
+    const double sy = scale ? scale[c] * y[c] : y[c];
+    acc = fma(val[e], sy, acc);
+    if (ABS) ab += fabs(val[e] * sy);
   }
+  if (ABS) *abs_sum = ab;
   return acc;
 }
 
@@ -172,7 +199,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_p(int m, const double* 
   if (!iter_active(sc)) return;
   const double beta = sc->beta, omega = sc->omega;
   for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS)
-    p[i] = fma(beta, fma(-omega, v[i], p[i]), r[i]);
+    nt_store(fma(beta, fma(-omega, nt_load(v + i), nt_load(p + i)), nt_load(r + i)), p + i);
 }
 
 // (A x)_row on the sliced-ELL matrix (values TM, FP64 x, FP64 sum), one
@@ -305,8 +332,8 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_v(const int* __restrict
     const int row = r0w + (threadIdx.x & (WAVE - 1));
     const double acc = sell_row_product(sptr, col, pcol, ptab, val, p, row);
     if (row < m) {
-      v[row] = acc;
-      dot += r0[row] * acc;
+      nt_store(acc, v + row);
+      dot += nt_load(r0 + row) * acc;
     }
   }
   double vv[1] = {dot};
@@ -341,7 +368,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_s(int m, const double* 
   }
   if (broke) return;
   for (int i = blockIdx.x * SOLVER_THREADS + threadIdx.x; i < m; i += gridDim.x * SOLVER_THREADS)
-    s[i] = fma(-alpha, v[i], r[i]);
+    nt_store(fma(-alpha, nt_load(v + i), nt_load(r + i)), s + i);
 }
 
 template <typename TM>
@@ -357,11 +384,13 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_t(const int* __restrict
   const int mpad = (m + SLICE - 1) / SLICE * SLICE;
   for (int r0w = blockIdx.x * SOLVER_THREADS + (threadIdx.x & ~(WAVE - 1)); r0w < mpad;
        r0w += gridDim.x * SOLVER_THREADS) {
-    const int row = r0w + (threadIdx.x & (WAVE - 1));
+    // Slices in reverse order: the end of the matrix, which v = A p
+    // streamed last, is the part still in the memory-side cache.
+    const int row = (mpad - SLICE - r0w) + (threadIdx.x & (WAVE - 1));
     const double acc = sell_row_product(sptr, col, pcol, ptab, val, s, row);
     if (row < m) {
-      const double si = s[row];
-      t[row] = acc;
+      const double si = nt_load(s + row);
+      nt_store(acc, t + row);
       ss += si * si;
       tt += acc * acc;
       ts += acc * si;
@@ -418,12 +447,12 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_xr(int m, const double*
     if (sconv) {
       y[i] += alpha * p[i];
     } else {
-      const double si = s[i];
-      y[i] += alpha * p[i] + omega * si;
-      const double ri = fma(-omega, t[i], si);
-      r[i] = ri;
+      const double si = nt_load(s + i);
+      nt_store(nt_load(y + i) + (alpha * nt_load(p + i) + omega * si), y + i);
+      const double ri = fma(-omega, nt_load(t + i), si);
+      nt_store(ri, r + i);
       rr += ri * ri;
-      r0r += r0[i] * ri;
+      r0r += nt_load(r0 + i) * ri;
     }
   }
   double vv[2] = {rr, r0r};
@@ -460,9 +489,12 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_bicg_true_residual(
   if (!INIT && sc->replace == 0) return;
   const bool restart = !INIT && ((sc->flags & SOLVE_RESTART) || sc->replace == 3);
   __shared__ double lds[3 * SOLVER_THREADS / WAVE];
+  // bb: INIT, ||b||^2; otherwise the sum over rows of (sum |a s y|)^2.
   double rr = 0.0, r0r = 0.0, bb = 0.0;
   for (int row = blockIdx.x * SOLVER_THREADS + threadIdx.x; row < m; row += gridDim.x * SOLVER_THREADS) {
-    const double acc = y ? sell_dot_scaled(sptr, col, val, scale, y, row) : 0.0;
+    double ab = 0.0;
+    const double acc = y ? sell_dot_scaled<!INIT>(sptr, col, val, scale, y, row, &ab) : 0.0;
+    if (!INIT) bb += ab * ab;
     const double bi = b[row];
     const double ri = bi - acc;
     r[row] = ri;
@@ -500,6 +532,7 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_residual(const double* _
       sc->init_res = sqrt(tot[0]);
       sc->rn = sc->rn_max = sc->init_res;
       sc->rn_best = sc->init_res;
+      sc->fp32_floor = 0.0;
       sc->rho_prev = sc->alpha = sc->omega = 1.0;
       sc->rho = tot[0];
       sc->beta = sc->rho;
@@ -535,6 +568,16 @@ __global__ __launch_bounds__(SOLVER_THREADS) void k_fin_residual(const double* _
       // the best one so far: the FP32 iteration matrix no longer drives the
       // refinement (its error is as large as what is left to remove).
       if (kind == 1 && (sc->flags & SOLVE_STALL_CHECK) && !(rn <= sc->stall * sc->rn_best)) {
+        sc->state = SOLVE_STALLED;
+        return;
+      }
+      // The same hand-over when the tolerance lies far below what an FP32
+      // copy of A S resolves on this iterate: rounding its entries moves the
+      // product by up to 2^-24 sum |a s y| per row, and a target under
+      // SOLVE_FP32_REACH times that norm takes one refinement restart per
+      // decade or so, which costs more than FP64 iterations.
+      sc->fp32_floor = 0x1p-24 * sqrt(tot[2]);
+      if ((sc->flags & SOLVE_STALL_CHECK) && sc->thresh < SOLVE_FP32_REACH * sc->fp32_floor) {
         sc->state = SOLVE_STALLED;
         return;
       }

@@ -27,6 +27,15 @@ enum : int {
 // Breakdown restarts allowed per solve (SOLVE_BREAK_RESTART) before the
 // solve ends in SOLVE_BREAKDOWN.
 constexpr int SOLVER_MAX_BREAK_RESTARTS = 32;
+// SOLVE_STALL_CHECK also hands over when the tolerance is below this
+// fraction of the FP32 rounding scale of the product (fp32_floor). At 3M
+// (industrial_mix) the seeded right-hand sides at 1e-8 sit at 0.1 of it and
+// stay mixed; at 1e-10 (1e-3 of it) the mixed refinement crept to 3x the
+// target and the FP64 solve continued from there diverged for 40,000
+// iterations, while handing over at the first replacement converged in
+// 4,824 (2.2 s against 2.8 s for FP64 from zero); the reference's
+// x* = x^2 + y^2 sits at 1e-5 of it.
+constexpr double SOLVE_FP32_REACH = 2e-3;
 
 // Read by every block of the streaming kernels, written only by the
 // one-block finalize kernels between them, so no intra-kernel races.
@@ -36,6 +45,7 @@ struct SolverScalars {
   double sn, rn, rn_max, delta;
   double final_res, rel_l2, max_err;
   double rn_best, stall;  // SOLVE_STALL_CHECK: smallest true residual so far, the factor
+  double fp32_floor;      // 2^-24 || sum_k |a s y| || at the last true residual (FP32 rounding scale)
   int it, max_iter, state, iterations;
   int flags, replace, n_replace, n_break;  // replace: 1 reliable, 2 convergence check, 3 breakdown restart
 };
