@@ -288,8 +288,9 @@ def main():
         p.L.lsmps_synchronize(p.h)
         t0 = time.perf_counter()
         for _ in range(k):
-            cs, ts = p.neighbors_classify(re, maxn, l0)
-            wr = p.wls(re, l0, precision)
+            # lsmps_step: the neighbor build and the WLS stage queued
+            # together, one wait for both readbacks.
+            cs, ts, wr = p.step(re, maxn, l0, precision)
             stage["cnl"].append(cs["ms"])
             stage["wls"].append(wr["ms"])
             stage["k_cnl_pass"].append(cs["kernel_ms"])

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LSMPS_HIP_ABI_VERSION 6
+#define LSMPS_HIP_ABI_VERSION 7
 
 enum {
   LSMPS_OK = 0,
@@ -130,6 +130,22 @@ typedef struct {
 } lsmps_wls_result;
 /* Gradient and Laplacian for every particle except Wall and Splash. */
 int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out);
+
+/* ---- one particle step: the body of run_benchmark's timed sequence
+ * (neighbors, classify, wls_test; src/lsmps.cpp:26-28,73) as one call.
+ * Same results as lsmps_build_neighbors_classify(re, max_neighbors, l0)
+ * followed by lsmps_wls(wls), bit for bit, but both stages are queued before
+ * the host waits once for their readbacks, so the GPU runs them back to
+ * back. cnl->ms and wls->ms time each stage as the separate calls do.
+ * wls.re must not exceed re. */
+typedef struct {
+  double re;             /* neighbor radius (lists and classification) */
+  int32_t max_neighbors;
+  double l0;             /* classification spacing */
+  lsmps_wls_params wls;
+} lsmps_step_params;
+int lsmps_step(lsmps_ctx* ctx, const lsmps_step_params* p, lsmps_cnl_stats* cnl, lsmps_type_stats* types,
+               lsmps_wls_result* wls);
 
 /* lsmps_wls without allocating device memory. No counterpart in the
  * reference: for callers that cannot allocate (work captured into a HIP

@@ -73,9 +73,12 @@ struct GraphCache {
 }  // namespace
 
 // Pinned readback area: 16 64-bit counters, then (as ints) up to 512
-// histogram bins (max_neighbors <= 256) and the 8-int cell box.
+// histogram bins (max_neighbors <= 256), the 8-int cell box, and 16 more
+// 64-bit counters for the WLS stage of lsmps_step (whose build's counters
+// occupy the first 16).
 constexpr int HOST_RB_HIST = 32, HOST_RB_BOX = HOST_RB_HIST + 512;  // int offsets
-constexpr size_t HOST_RB_BYTES = (HOST_RB_BOX + 8) * sizeof(int);
+constexpr int HOST_RB_WLS = HOST_RB_BOX + 8;                         // int offset, 8-byte aligned
+constexpr size_t HOST_RB_BYTES = (HOST_RB_WLS + 32) * sizeof(int);
 // Cells of margin around the box when a build's grid is kept for the next.
 constexpr int CNL_GRID_MARGIN = 2;
 
@@ -83,6 +86,7 @@ struct lsmps_ctx {
   int device = 0;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;  // stage / kernel
+  hipEvent_t eb[4] = {};  // the same for the neighbor stage of lsmps_step (e0-e3 time its WLS stage)
   std::string err;
 
   // particles (SoA FP64 positions, types)
@@ -232,24 +236,34 @@ int check_launch(lsmps_ctx* ctx, const char* what) {
   return LSMPS_OK;
 }
 
-float elapsed(lsmps_ctx* ctx) {
+float event_ms(hipEvent_t a, hipEvent_t b) {
   float ms = 0.0f;
-  if (hipEventSynchronize(ctx->e1) != hipSuccess) return -1.0f;
-  if (hipEventElapsedTime(&ms, ctx->e0, ctx->e1) != hipSuccess) return -1.0f;
+  if (hipEventSynchronize(b) != hipSuccess) return -1.0f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0f;
   return ms;
 }
 
+float elapsed(lsmps_ctx* ctx) { return event_ms(ctx->e0, ctx->e1); }
+
 // Time between e2 and e3, recorded around a stage's main kernel.
-float kernel_elapsed(lsmps_ctx* ctx) {
-  float ms = 0.0f;
-  if (hipEventSynchronize(ctx->e3) != hipSuccess) return -1.0f;
-  if (hipEventElapsedTime(&ms, ctx->e2, ctx->e3) != hipSuccess) return -1.0f;
-  return ms;
-}
+float kernel_elapsed(lsmps_ctx* ctx) { return event_ms(ctx->e2, ctx->e3); }
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
+}
+
+// A stage's readback in one launch, stored straight into the pinned area
+// (host-visible once the stream has been synchronized): nc counters at
+// rb_counters, then optionally nh histogram bins and nb box words at their
+// int offsets in rb. Three runtime copies took ~4.5 us each.
+__global__ void k_readback(const unsigned long long* __restrict__ counters, int nc, const int* __restrict__ hist,
+                           int nh, const int* __restrict__ box, int nb, unsigned long long* __restrict__ rb_counters,
+                           int* __restrict__ rb) {
+  const int t = threadIdx.x;
+  if (t < nc) rb_counters[t] = counters[t];
+  for (int i = t; i < nh; i += blockDim.x) rb[HOST_RB_HIST + i] = hist[i];
+  if (t < nb) rb[HOST_RB_BOX + t] = box[t];
 }
 __global__ void k_split_xyz(const double* __restrict__ in, int64_t n, double* __restrict__ x, double* __restrict__ y,
                             double* __restrict__ z) {
@@ -348,9 +362,11 @@ void zero_counters(lsmps_ctx* ctx, int k) {
 
 // Counters (and the neighbor histogram) come back through pinned memory: a
 // copy into pageable memory takes the runtime's staged path, tens of
-// microseconds per call. Waits for the stream.
+// microseconds per call. Waits for the stream. k <= 16.
 hipError_t read_counters(lsmps_ctx* ctx, unsigned long long* c, size_t k) {
-  hipError_t e = hipMemcpyAsync(ctx->host_rb, ctx->counters.p, k * sizeof(*c), hipMemcpyDeviceToHost, ctx->st);
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, ctx->st, ctx->counters.p, (int)k, nullptr, 0, nullptr, 0,
+                     ctx->host_rb, reinterpret_cast<int*>(ctx->host_rb));
+  hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
   if (e == hipSuccess) std::memcpy(c, ctx->host_rb, k * sizeof(*c));
   return e;
@@ -408,6 +424,8 @@ int lsmps_create(int device, lsmps_ctx** out) {
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->e0) != hipSuccess || hipEventCreate(&ctx->e1) != hipSuccess ||
       hipEventCreate(&ctx->e2) != hipSuccess || hipEventCreate(&ctx->e3) != hipSuccess ||
+      hipEventCreate(&ctx->eb[0]) != hipSuccess || hipEventCreate(&ctx->eb[1]) != hipSuccess ||
+      hipEventCreate(&ctx->eb[2]) != hipSuccess || hipEventCreate(&ctx->eb[3]) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->host_sc), sizeof(SolverScalars)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->host_small), 64 * sizeof(int64_t)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->host_rb), HOST_RB_BYTES) != hipSuccess ||
@@ -483,6 +501,8 @@ void lsmps_destroy(lsmps_ctx* ctx) {
   if (ctx->e1) (void)hipEventDestroy(ctx->e1);
   if (ctx->e2) (void)hipEventDestroy(ctx->e2);
   if (ctx->e3) (void)hipEventDestroy(ctx->e3);
+  for (hipEvent_t e : ctx->eb)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->st) (void)hipStreamDestroy(ctx->st);
   delete ctx;
 }
@@ -699,11 +719,27 @@ int check_build(lsmps_ctx* ctx, int n, int64_t ncells, int maxn, uint64_t pairs)
   return LSMPS_OK;
 }
 
-// Cell sort + one neighbor pass (+ classification when classify).
-// re: classification radius; lists hold every pair within re + skin.
-int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool classify, double l0,
-                    lsmps_cnl_stats* out,
-                    lsmps_type_stats* tout) {
+// Dense grid: bounded so a sparse outlier cannot blow up memory.
+bool grid_too_large(int64_t cells, int n) {
+  return cells > ((int64_t)1 << 30) || cells > 64 * (int64_t)n + ((int64_t)1 << 24);
+}
+
+// A neighbor build in flight: what build_finish needs once the stream has
+// passed the build's readback. ev: stage begin/end, pass begin/end.
+struct BuildRun {
+  double rl = 0.0, skin = 0.0;
+  int n = 0, maxn = 0;
+  int64_t ncells = 0;
+  bool classify = false, spec = false, compressed = false;
+  hipEvent_t ev[4] = {};
+};
+
+// Cell sort + one neighbor pass (+ classification when classify), queued on
+// the stream with the readback of its counters, histogram and box into the
+// pinned area. re: classification radius; lists hold every pair within
+// re + skin.
+int build_submit(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool classify, double l0,
+                 const hipEvent_t ev[4], BuildRun* br) {
   if (!(re > 0) || !std::isfinite(re)) return fail(ctx, LSMPS_E_ARG, "re must be positive and finite");
   if (!(skin >= 0) || !std::isfinite(skin)) return fail(ctx, LSMPS_E_ARG, "skin must be finite and >= 0");
   if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph (no positions): upload particles first");
@@ -740,7 +776,7 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   HIPCHK(ctx->nbr.ensure((size_t)n * (size_t)std::max(1, maxn)));
   HIPCHK(ctx->hist.ensure((size_t)maxn + 1 + (size_t)(maxn + 6) * CNL_STATS_BLOCKS));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
-  HIPCHK(hipEventRecord(ctx->e0, st));
+  HIPCHK(hipEventRecord(ev[0], st));
   // With the last build's grid the key pass makes the cell coordinates and
   // the box itself (launch_cell_sort with CellKeyDirect); otherwise the box
   // comes first.
@@ -782,20 +818,18 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   ctx->ox = (double)bx[0] * rl;
   ctx->oy = (double)bx[1] * rl;
   ctx->oz = (double)bx[2] * rl;
-  // Dense grid: bounded so a sparse outlier cannot blow up memory.
-  auto too_many = [n](int64_t c) { return c > ((int64_t)1 << 30) || c > 64 * (int64_t)n + ((int64_t)1 << 24); };
   bool compressed = false;
-  if (too_many(ncells)) {
+  if (grid_too_large(ncells, n)) {
     if (spec) {
       // The kept grid was sized for an earlier (larger) particle count: the
       // exact box may still fit.
       ctx->spec_ok = false;
-      return build_cell_list(ctx, re, skin, maxn, classify, l0, out, tout);
+      return build_submit(ctx, re, skin, maxn, classify, l0, ev, br);
     }
     // A sparse box (outliers): compress the occupied coordinates per axis.
     if (int rc = compress_grid(ctx, n, g)) return rc;
     ncells = cells_of(g);
-    if (too_many(ncells))
+    if (grid_too_large(ncells, n))
       return fail(ctx, LSMPS_E_LIMIT, "particle cells too scattered for the dense grid, even compressed per axis");
     bx[0] = bx[1] = bx[2] = 0;
     compressed = true;
@@ -835,19 +869,52 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
   if (n == 0) launch_fill_int(ctx->hist.p, maxn + 1, 0, st);  // the pass's reduction writes every bin
   launch_cnl_pass(B, n, rl * rl, re * re, maxn, classify, 1.5 * l0, ctx->type_s.p, ctx->counters.p, ctx->hist.p,
-                  ctx->counters.p + 8, st, ctx->e2, ctx->e3);
+                  ctx->counters.p + 8, st, ev[2], ev[3]);
   if (int rc = check_launch(ctx, "cnl_pass")) return rc;
-  HIPCHK(hipEventRecord(ctx->e1, st));
+  HIPCHK(hipEventRecord(ev[1], st));
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(256), 0, st, ctx->counters.p, 16, ctx->hist.p, maxn + 1, ctx->box.p,
+                     7, ctx->host_rb, reinterpret_cast<int*>(ctx->host_rb));
+  if (int rc = check_launch(ctx, "readback")) return rc;
+  br->rl = rl;
+  br->skin = skin;
+  br->n = n;
+  br->maxn = maxn;
+  br->ncells = ncells;
+  br->classify = classify;
+  br->spec = spec;
+  br->compressed = compressed;
+  for (int k = 0; k < 4; ++k) br->ev[k] = ev[k];
+  return LSMPS_OK;
+}
+
+// The context's record of a build whose kernels are queued: the lists'
+// radius, size and classification state. Everything downstream is stale.
+void commit_build(lsmps_ctx* ctx, const BuildRun& br) {
+  ctx->re = br.rl;
+  ctx->skin = br.skin;
+  ctx->moved = 0.0;
+  ctx->after_skin_build = br.skin > 0.0;
+  ctx->replay = false;
+  ctx->n_cnl = (uint64_t)br.n;
+  ctx->maxn = br.maxn;
+  ctx->types_stale = br.classify;
+  ctx->have_cnl = true;
+  ctx->have_wls = ctx->have_ppe = ctx->have_solution = false;
+}
+
+// Waits for a submitted build and takes its readback: the next build's
+// grid, the statistics, the optional host checks. *redo: a particle left
+// the kept grid, so the build must be submitted again (on the exact box).
+int build_finish(lsmps_ctx* ctx, const BuildRun& br, bool* redo, lsmps_cnl_stats* out, lsmps_type_stats* tout) {
+  *redo = false;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const int n = br.n, maxn = br.maxn;
   std::vector<int> hist((size_t)maxn + 1);
   unsigned long long c[16];
-  int* hist_rb = reinterpret_cast<int*>(ctx->host_rb) + HOST_RB_HIST;
-  int* box_rb = reinterpret_cast<int*>(ctx->host_rb) + HOST_RB_BOX;
-  HIPCHK(hipMemcpyAsync(hist_rb, ctx->hist.p, hist.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(box_rb, ctx->box.p, 7 * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(read_counters(ctx, c, 16));
-  std::memcpy(hist.data(), hist_rb, hist.size() * sizeof(int));
+  std::memcpy(c, ctx->host_rb, sizeof(c));
+  std::memcpy(hist.data(), reinterpret_cast<int*>(ctx->host_rb) + HOST_RB_HIST, hist.size() * sizeof(int));
   int box_now[7];
-  std::memcpy(box_now, box_rb, sizeof(box_now));
+  std::memcpy(box_now, reinterpret_cast<int*>(ctx->host_rb) + HOST_RB_BOX, sizeof(box_now));
   if (n > 0) {
     // Next build's grid: this box plus the margin (kept within the int
     // range of the cell coordinates).
@@ -857,42 +924,132 @@ int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool c
       ctx->spec[3 + k] = (int)std::min<int64_t>(hi - ctx->spec[k] + 1, INT_MAX / 2);
     }
     const int64_t sc = (int64_t)ctx->spec[3] * ctx->spec[4] * ctx->spec[5];
-    ctx->spec_ok = !compressed && !too_many(sc);
-    ctx->spec_rl = rl;
+    ctx->spec_ok = !br.compressed && !grid_too_large(sc, n);
+    ctx->spec_rl = br.rl;
   } else {
     ctx->spec_ok = false;
   }
-  if (spec && box_now[6]) {
+  if (br.spec && box_now[6]) {
     // A particle left the grid of the last build: redo on the exact box.
     ctx->spec_ok = false;
-    return build_cell_list(ctx, re, skin, maxn, classify, l0, out, tout);
+    *redo = true;
+    return LSMPS_OK;
   }
   lsmps_cnl_stats cs;
   fill_cnl_stats(ctx, hist, c[0], &cs);
   if (cs.pairs > (uint64_t)INT_MAX) return fail(ctx, LSMPS_E_LIMIT, "neighbor pairs beyond int32 indexing");
   if (ctx->check_build) {
-    ctx->re = rl;  // the list radius the check tests against
-    if (int rc = check_build(ctx, n, ncells, maxn, cs.pairs)) return rc;
+    ctx->re = br.rl;  // the list radius the check tests against
+    if (int rc = check_build(ctx, n, br.ncells, maxn, cs.pairs)) return rc;
   }
-  cs.ms = elapsed(ctx);
-  cs.kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
-  ctx->re = rl;
-  ctx->skin = skin;
-  ctx->moved = 0.0;
-  ctx->after_skin_build = skin > 0.0;
-  ctx->replay = false;
-  ctx->n_cnl = (uint64_t)n;
-  ctx->maxn = maxn;
-  ctx->types_stale = classify;
+  cs.ms = event_ms(br.ev[0], br.ev[1]);
+  cs.kernel_ms = n > 0 ? event_ms(br.ev[2], br.ev[3]) : 0.0f;
+  commit_build(ctx, br);
   ctx->pairs = cs.pairs;
-  ctx->have_cnl = true;
-  ctx->have_wls = ctx->have_ppe = ctx->have_solution = false;
   if (out) *out = cs;
   if (tout) {
     for (int k = 0; k < 5; ++k) tout->count[k] = c[8 + k];
     tout->ms = 0.0f;  // fused into the neighbor pass
   }
   return LSMPS_OK;
+}
+
+int build_cell_list(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool classify, double l0,
+                    lsmps_cnl_stats* out, lsmps_type_stats* tout) {
+  const hipEvent_t ev[4] = {ctx->e0, ctx->e1, ctx->e2, ctx->e3};
+  // At most one redo: it runs on the exact box (spec_ok is cleared).
+  for (;;) {
+    BuildRun br;
+    if (int rc = build_submit(ctx, re, skin, maxn, classify, l0, ev, &br)) return rc;
+    bool redo = false;
+    if (int rc = build_finish(ctx, br, &redo, out, tout)) return rc;
+    if (!redo) return LSMPS_OK;
+  }
+}
+
+}  // namespace
+
+namespace {
+
+// Checks of a WLS call; with lists false, only those that do not depend
+// on the neighbor lists (lsmps_step checks the rest by building them).
+int wls_check(lsmps_ctx* ctx, const lsmps_wls_params* p, bool lists) {
+  if (!(p->fp32_pivot_min >= 0.0 && p->fp32_pivot_min <= 1.0))
+    return fail(ctx, LSMPS_E_ARG, "fp32_pivot_min must lie in [0, 1]");
+  if (lists && !ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "WLS needs neighbors");
+  if (lists && ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph: use lsmps_replay_wls");
+  if (!ctx->have_field) return fail(ctx, LSMPS_E_STATE, "WLS needs a field (lsmps_set_field_*)");
+  if (p->precision != LSMPS_WLS_FP64 && p->precision != LSMPS_WLS_MIXED)
+    return fail(ctx, LSMPS_E_ARG, "unknown WLS precision");
+  if (!(p->re > 0) || !(p->l0 > 0) || !std::isfinite(p->re) || !std::isfinite(p->l0))
+    return fail(ctx, LSMPS_E_ARG, "re and l0 must be positive and finite");
+  if (!lists) return LSMPS_OK;
+  if (p->re > ctx->re) return fail(ctx, LSMPS_E_ARG, "re exceeds the radius of the neighbor lists");
+  return check_lists_current(ctx, "WLS");
+}
+
+// Queues the WLS stage over the current lists (events e0-e3) and the
+// readback of its six counters into rb (pinned).
+int wls_submit(lsmps_ctx* ctx, const lsmps_wls_params* p, unsigned long long* rb) {
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = (int)ctx->n_cnl;
+  hipStream_t st = ctx->st;
+  HIPCHK(ctx->grad_s.ensure(3 * (size_t)n));
+  HIPCHK(ctx->lap_s.ensure(n));
+  HIPCHK(ctx->wstatus_s.ensure(n));
+  HIPCHK(ctx->redo.ensure(std::max(n, 1)));
+  WlsMixedWork mw{};
+  if (p->precision == LSMPS_WLS_MIXED) {
+    const int64_t gm = ((int64_t)n + 63) / 64;
+    HIPCHK(ctx->wpend.ensure(std::max(n, 1)));
+    HIPCHK(ctx->wpoff.ensure((size_t)gm + 1));
+    HIPCHK(ctx->wlist.ensure(std::max(n, 1)));
+    HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems(gm)));
+    mw = {ctx->wpend.p, ctx->wpoff.p, ctx->scan_scr.p, ctx->wlist.p};
+  }
+  HIPCHK(ctx->bpart.ensure(6 * ((size_t)n / 64 + 1)));
+  HIPCHK(hipEventRecord(ctx->e0, st));
+  // Field into the sorted records: evaluated there for a quadratic,
+  // gathered for values.
+  if (!ctx->spos_field) {
+    if (ctx->field_quadratic)
+      launch_field_quadratic(ctx->spos.p, n, ctx->qf, st);
+    else
+      launch_gather_field(ctx->perm.p, ctx->f.p, n, ctx->spos.p, st);
+    ctx->spos_field = true;
+  }
+  zero_counters(ctx, 16);
+  const int* nb = ctx->nbr.p;
+  const int* nc = ctx->cnt.p;
+  launch_fill_int(ctx->nredo.p, 1, 0, st);
+  if (p->precision == LSMPS_WLS_FP64) {
+    launch_wls_exact(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
+                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, ctx->bpart.p, st,
+                     ctx->e2, ctx->e3);
+  } else {
+    launch_wls_mixed(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, wls_tau(p), mw,
+                     ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p,
+                     ctx->bpart.p, st, ctx->e2, ctx->e3);
+  }
+  if (int rc = check_launch(ctx, "wls")) return rc;
+  HIPCHK(hipEventRecord(ctx->e1, st));
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, st, ctx->counters.p, 6, nullptr, 0, nullptr, 0, rb,
+                     reinterpret_cast<int*>(ctx->host_rb));
+  return check_launch(ctx, "readback");
+}
+
+// Results of a WLS stage whose readback (c, six counters) has arrived.
+void wls_finish(lsmps_ctx* ctx, const unsigned long long* c, lsmps_wls_result* out) {
+  ctx->have_wls = true;
+  if (out) {
+    out->ok = c[WLS_OK];
+    out->regularized = c[WLS_REGULARIZED];
+    out->failed = c[WLS_FAILED] + c[WLS_NO_WEIGHT];
+    out->skipped = c[WLS_SKIPPED];
+    out->fp64_redo = c[WLS_REDO_SLOT];
+    out->ms = elapsed(ctx);
+    out->kernel_ms = ctx->n_cnl > 0 ? kernel_elapsed(ctx) : 0.0f;
+  }
 }
 
 }  // namespace
@@ -1111,72 +1268,43 @@ int lsmps_set_field_values(lsmps_ctx* ctx, const double* f, uint64_t n) {
 
 int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) {
   if (!ctx || !p) return LSMPS_E_ARG;
-  if (!(p->fp32_pivot_min >= 0.0 && p->fp32_pivot_min <= 1.0))
-    return fail(ctx, LSMPS_E_ARG, "fp32_pivot_min must lie in [0, 1]");
-  if (!ctx->have_cnl) return fail(ctx, LSMPS_E_STATE, "WLS needs neighbors");
-  if (ctx->replay) return fail(ctx, LSMPS_E_STATE, "the context holds a replay graph: use lsmps_replay_wls");
-  if (!ctx->have_field) return fail(ctx, LSMPS_E_STATE, "WLS needs a field (lsmps_set_field_*)");
-  if (p->precision != LSMPS_WLS_FP64 && p->precision != LSMPS_WLS_MIXED)
-    return fail(ctx, LSMPS_E_ARG, "unknown WLS precision");
-  if (!(p->re > 0) || !(p->l0 > 0) || !std::isfinite(p->re) || !std::isfinite(p->l0))
-    return fail(ctx, LSMPS_E_ARG, "re and l0 must be positive and finite");
-  if (p->re > ctx->re) return fail(ctx, LSMPS_E_ARG, "re exceeds the radius of the neighbor lists");
-  if (int rc = check_lists_current(ctx, "WLS")) return rc;
+  if (int rc = wls_check(ctx, p, true)) return rc;
   GUARD_BEGIN
-  HIPCHK(hipSetDevice(ctx->device));
-  const int n = (int)ctx->n_cnl;
-  hipStream_t st = ctx->st;
-  HIPCHK(ctx->grad_s.ensure(3 * (size_t)n));
-  HIPCHK(ctx->lap_s.ensure(n));
-  HIPCHK(ctx->wstatus_s.ensure(n));
-  HIPCHK(ctx->redo.ensure(std::max(n, 1)));
-  WlsMixedWork mw{};
-  if (p->precision == LSMPS_WLS_MIXED) {
-    const int64_t gm = ((int64_t)n + 63) / 64;
-    HIPCHK(ctx->wpend.ensure(std::max(n, 1)));
-    HIPCHK(ctx->wpoff.ensure((size_t)gm + 1));
-    HIPCHK(ctx->wlist.ensure(std::max(n, 1)));
-    HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems(gm)));
-    mw = {ctx->wpend.p, ctx->wpoff.p, ctx->scan_scr.p, ctx->wlist.p};
+  if (int rc = wls_submit(ctx, p, ctx->host_rb)) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  wls_finish(ctx, ctx->host_rb, out);
+  return LSMPS_OK;
+  GUARD_END
+}
+
+int lsmps_step(lsmps_ctx* ctx, const lsmps_step_params* p, lsmps_cnl_stats* cs, lsmps_type_stats* ts,
+               lsmps_wls_result* wr) {
+  if (!ctx || !p) return LSMPS_E_ARG;
+  if (int rc = wls_check(ctx, &p->wls, false)) return rc;
+  if (!(p->wls.re <= p->re)) return fail(ctx, LSMPS_E_ARG, "re exceeds the radius of the neighbor lists");
+  GUARD_BEGIN
+  // Both stages are queued before the one wait: the build's readback goes
+  // to the first counters of the pinned area, the WLS stage's after the box.
+  BuildRun br;
+  if (int rc = build_submit(ctx, p->re, 0.0, p->max_neighbors, true, p->l0, ctx->eb, &br)) return rc;
+  commit_build(ctx, br);
+  unsigned long long* rbw = ctx->host_rb + HOST_RB_WLS / 2;
+  if (int rc = wls_submit(ctx, &p->wls, rbw)) {
+    ctx->have_cnl = false;  // the build's readback is not taken
+    return rc;
   }
-  HIPCHK(ctx->bpart.ensure(6 * ((size_t)n / 64 + 1)));
-  HIPCHK(hipEventRecord(ctx->e0, st));
-  // Field into the sorted records: evaluated there for a quadratic,
-  // gathered for values.
-  if (!ctx->spos_field) {
-    if (ctx->field_quadratic)
-      launch_field_quadratic(ctx->spos.p, n, ctx->qf, st);
-    else
-      launch_gather_field(ctx->perm.p, ctx->f.p, n, ctx->spos.p, st);
-    ctx->spos_field = true;
+  bool redo = false;
+  if (int rc = build_finish(ctx, br, &redo, cs, ts)) {
+    ctx->have_cnl = ctx->have_wls = false;
+    return rc;
   }
-  zero_counters(ctx, 16);
-  const int* nb = ctx->nbr.p;
-  const int* nc = ctx->cnt.p;
-  launch_fill_int(ctx->nredo.p, 1, 0, st);
-  if (p->precision == LSMPS_WLS_FP64) {
-    launch_wls_exact(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
-                     ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, ctx->bpart.p, st,
-                     ctx->e2, ctx->e3);
-  } else {
-    launch_wls_mixed(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, wls_tau(p), mw,
-                     ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p,
-                     ctx->bpart.p, st, ctx->e2, ctx->e3);
+  if (redo) {
+    // A particle left the kept grid: the lists are built again on the
+    // exact box and the WLS stage runs on them.
+    if (int rc = build_cell_list(ctx, p->re, 0.0, p->max_neighbors, true, p->l0, cs, ts)) return rc;
+    return lsmps_wls(ctx, &p->wls, wr);
   }
-  if (int rc = check_launch(ctx, "wls")) return rc;
-  HIPCHK(hipEventRecord(ctx->e1, st));
-  unsigned long long c[6];
-  HIPCHK(read_counters(ctx, c, sizeof(c) / sizeof(c[0])));
-  ctx->have_wls = true;
-  if (out) {
-    out->ok = c[WLS_OK];
-    out->regularized = c[WLS_REGULARIZED];
-    out->failed = c[WLS_FAILED] + c[WLS_NO_WEIGHT];
-    out->skipped = c[WLS_SKIPPED];
-    out->fp64_redo = c[WLS_REDO_SLOT];
-    out->ms = elapsed(ctx);
-    out->kernel_ms = n > 0 ? kernel_elapsed(ctx) : 0.0f;
-  }
+  wls_finish(ctx, rbw, wr);
   return LSMPS_OK;
   GUARD_END
 }

@@ -50,6 +50,10 @@ class WlsParams(C.Structure):
     _fields_ = [("re", _d), ("l0", _d), ("reg", _d), ("precision", _i32), ("fp32_pivot_min", _d)]
 
 
+class StepParams(C.Structure):
+    _fields_ = [("re", _d), ("max_neighbors", _i32), ("l0", _d), ("wls", WlsParams)]
+
+
 class WlsResult(C.Structure):
     _fields_ = [("ok", _u64), ("regularized", _u64), ("failed", _u64), ("skipped", _u64), ("fp64_redo", _u64),
                 ("ms", _f), ("kernel_ms", _f)]
@@ -119,6 +123,8 @@ ABI = {
     "lsmps_set_field_quadratic": (C.c_int, [_p, C.POINTER(QuadField)]),
     "lsmps_set_field_values": (C.c_int, [_p, _p, _u64]),
     "lsmps_wls": (C.c_int, [_p, C.POINTER(WlsParams), C.POINTER(WlsResult)]),
+    "lsmps_step": (C.c_int, [_p, C.POINTER(StepParams), C.POINTER(CnlStats), C.POINTER(TypeStats),
+                             C.POINTER(WlsResult)]),
     "lsmps_wls_noalloc": (C.c_int, [_p, C.POINTER(WlsParams), _p, _p, _p, C.POINTER(WlsResult)]),
     "lsmps_wls_sampled_stats": (C.c_int, [_p, _i32, C.POINTER(WlsStats)]),
     "lsmps_assemble_ppe": (C.c_int, [_p, C.POINTER(PpeParams), C.POINTER(PpeResult)]),
@@ -247,6 +253,16 @@ class Pipeline:
         s = WlsResult()
         self._check(self.L.lsmps_wls(self.h, C.byref(p), C.byref(s)), "wls")
         return as_dict(s)
+
+    def step(self, re, max_neighbors, l0, precision=WLS_FP64, reg=1e-8, fp32_pivot_min=0.0, wls_re=None):
+        """neighbors_classify(re, max_neighbors, l0) then wls(wls_re or re, l0,
+        precision, ...) as one call (one wait for both stages' readbacks)."""
+        p = StepParams(re, max_neighbors, l0, WlsParams(re if wls_re is None else wls_re, l0, reg, precision,
+                                                        fp32_pivot_min))
+        s, t, w = CnlStats(), TypeStats(), WlsResult()
+        self._check(self.L.lsmps_step(self.h, C.byref(p), C.byref(s), C.byref(t), C.byref(w)), "step")
+        self.pairs = s.pairs
+        return as_dict(s), as_dict(t), as_dict(w)
 
     def wls_sampled_stats(self, wls_samples):
         s = WlsStats()

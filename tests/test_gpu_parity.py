@@ -331,6 +331,72 @@ def test_kept_grid_reused_and_outgrown(pipe):
     assert np.array_equal(t2, pipe.types())
 
 
+def test_step_matches_separate_calls():
+    """lsmps_step queues the neighbor build and the WLS stage before one wait;
+    it gives what lsmps_build_neighbors_classify followed by lsmps_wls gives,
+    bit for bit (lists, types, statistics, WLS results): on a first build, on
+    repeated builds (kept grid, last build's order), in both precisions, and
+    when particles have left the kept grid (build and WLS redone on the exact
+    box)."""
+    l0 = 0.01
+    re = 2.1 * l0
+    xyz, t = L.generate_case("wall_film", 20000, 5, l0)
+    say("step against separate calls", len(xyz))
+    a, b = L.Pipeline(0), L.Pipeline(0)
+    try:
+        for q in (a, b):
+            q.upload(xyz, t)
+            q.set_field(L.C1_FIELD)
+
+        def same(sa, sb):
+            (ca, ta, wa), (cb, tb, wb) = sa, sb
+            for k in ("pairs", "truncated", "mean", "p50", "p90", "p99", "max_count", "grid"):
+                assert ca[k] == cb[k], k
+            assert ta["count"] == tb["count"]
+            for k in ("ok", "regularized", "failed", "skipped", "fp64_redo"):
+                assert wa[k] == wb[k], k
+            assert ca["ms"] > 0 and wa["ms"] > 0 and wa["kernel_ms"] > 0
+            ra, ka = a.neighbor_lists()
+            rb, kb = b.neighbor_lists()
+            assert np.array_equal(ra, rb) and np.array_equal(ka, kb)
+            assert np.array_equal(a.types(), b.types())
+            for x, y in zip(a.wls_results(), b.wls_results()):
+                assert x.tobytes() == y.tobytes()
+
+        def both(prec):
+            sa = a.step(re, 96, l0, prec)
+            cb, tb = b.neighbors_classify(re, 96, l0)
+            same(sa, (cb, tb, b.wls(re, l0, prec)))
+
+        for prec in (L.WLS_FP64, L.WLS_MIXED, L.WLS_MIXED, L.WLS_FP64):
+            both(prec)
+        # A few particles pushed ten list radii along x: outside the kept grid.
+        u = np.zeros_like(xyz)
+        u[:5, 0] = 1.0
+        for q in (a, b):
+            q.set_velocities(u)
+            q.advance(10 * re)
+        both(L.WLS_MIXED)
+        both(L.WLS_FP64)
+        # A smaller WLS radius than the lists'.
+        sa = a.step(re, 96, l0, L.WLS_FP64, wls_re=1.9 * l0)
+        cb, tb = b.neighbors_classify(re, 96, l0)
+        same(sa, (cb, tb, b.wls(1.9 * l0, l0, L.WLS_FP64)))
+        # Checked before anything is queued.
+        with pytest.raises(L.LsmpsError):
+            a.step(re, 96, l0, L.WLS_FP64, wls_re=1.1 * re)
+        with pytest.raises(L.LsmpsError):
+            a.step(re, 96, l0, 7)
+        c = L.Pipeline(0)
+        c.upload(xyz, t)
+        with pytest.raises(L.LsmpsError):
+            c.step(re, 96, l0, L.WLS_FP64)  # no field
+        c.close()
+    finally:
+        a.close()
+        b.close()
+
+
 def test_field_values_and_quadratic_paths_agree(pipe):
     """The field reaches the sorted records four ways: evaluated (quadratic)
     or gathered (values), by the neighbor build when the field is set before
