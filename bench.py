@@ -48,8 +48,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--particles", type=int, default=3_000_000)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solvers", default="fp64_jacobi,mixed,learned_mixed,wls_rows,xstar_quadratic",

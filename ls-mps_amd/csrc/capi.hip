@@ -386,8 +386,12 @@ double max_speed(const double* uvw, uint64_t n) {
 // Stages that read pairs within re from the lists need the lists to still
 // hold every such pair: refused once the particles may have closed in by
 // the skin (any motion at all for lists built without one).
-// Pivot-ratio threshold of the mixed WLS (validated by the caller).
+// Pivot-ratio threshold of the mixed WLS (validated by the caller). The
+// FP32 moments are sums of squared offsets in the caller's length unit, so
+// for a radius outside [1e-12, 1e12] every row is flagged for FP64 (no
+// FP32 pivot exceeds its whole diagonal entry).
 float wls_tau(const lsmps_wls_params* p) {
+  if (!(p->re >= 1e-12 && p->re <= 1e12)) return 1.0f;
   return (float)(p->fp32_pivot_min > 0.0 ? p->fp32_pivot_min : LSMPS_WLS_FP32_PIVOT_DEFAULT);
 }
 

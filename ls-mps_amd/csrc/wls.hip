@@ -16,7 +16,7 @@
 //    factorisation only; a row that needs the regularisation ladder goes to
 //    the work list of k_wls_redo (below), which runs the whole ladder with
 //    the same results.
-//  * k_wls_mixed  FP32 moments on offsets normalised by re, FMA
+//  * k_wls_mixed  FP32 moments on scale-free monomials (wls_accumulate), FMA
 //    accumulation, FP32 Cholesky. A row whose FP32 pivots are not all above
 //    tau times their diagonal entries (an ill-conditioned moment matrix,
 //    whose FP32 solution carries an error of about eps32 / pivot ratio
@@ -398,22 +398,26 @@ constexpr int WLS_NACC = 55;   // 45 moments + 9 RHS + weight sum
 constexpr int WLS_STRIDE = 57; // LDS floats per particle (odd: no bank conflicts)
 constexpr int WLS_HALF = 32;   // particles per phase-2 batch
 
-__device__ __forceinline__ void wls_accumulate(const double4 pj, const double4 pi, double inv_re, float rmin2,
+// Offsets in FP32 from the FP64 differences, in the caller's length unit:
+// the linear monomials d/r are scale-free, the quadratic ones d (d/r) lack
+// the reference's constant factors (1/(2 l0) on the squares, 1/l0 on the
+// products), so the moments are S^-1 M S^-1 for a constant diagonal S. The
+// solution is then S times the reference's, undone on the Laplacian (whose
+// l0 cancels); the pivot test is relative to each diagonal entry, which S
+// leaves unchanged. (Normalising the offsets by re took four FP64 products
+// per pair, 3-4% of the kernel at the same accuracy.) The sums stay in the
+// FP32 range for re in [1e-12, 1e12]; outside it the caller flags every row
+// for FP64 (tau = 1).
+__device__ __forceinline__ void wls_accumulate(const double4 pj, const double4 pi, float re, float rmin2, float re2,
                                                float (&a)[WLS_NACC]) {
-  const float dx = (float)((pj.x - pi.x) * inv_re), dy = (float)((pj.y - pi.y) * inv_re),
-              dz = (float)((pj.z - pi.z) * inv_re);
+  const float dx = (float)(pj.x - pi.x), dy = (float)(pj.y - pi.y), dz = (float)(pj.z - pi.z);
   const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
   // Out of range (or the row's own slot standing in for a missing
   // neighbor): weight 0 and finite monomials, so every sum adds 0.
-  const bool ok = r2 > rmin2 && r2 < 1.0f;
-  const float inv = __builtin_amdgcn_rsqf(ok ? r2 : 1.0f);
-  const float tq = inv - 1.0f;
+  const bool ok = r2 > rmin2 && r2 < re2;
+  const float inv = __builtin_amdgcn_rsqf(ok ? r2 : re2);
+  const float tq = re * inv - 1.0f;
   const float w = ok ? tq * tq : 0.0f;
-  // Monomials over r without the reference's constant factors 1/(2 l0) and
-  // 1/l0 on the quadratic ones (the diagonal scaling S): the moments are
-  // then S^-1 M S^-1, whose solution is S times the reference's, undone on
-  // the solution; the pivot test is relative to each diagonal entry, which
-  // that scaling does not change. One product per quadratic monomial.
   float q[9];
   q[0] = dx * inv;
   q[1] = dy * inv;
@@ -424,7 +428,7 @@ __device__ __forceinline__ void wls_accumulate(const double4 pj, const double4 p
   q[6] = q[0] * dy;
   q[7] = q[0] * dz;
   q[8] = q[1] * dz;
-  const float dfr = (float)((pj.w - pi.w) * inv_re) * inv;
+  const float dfr = (float)(pj.w - pi.w) * inv;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     const float wq = w * q[i];
@@ -486,15 +490,14 @@ __device__ __forceinline__ bool wls_solve_f32(const float* __restrict__ a, float
 
 __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ spos,
                                                     const uint8_t* __restrict__ type, const int* __restrict__ nbr,
-                                                    const int* __restrict__ cnt, int maxn, int n, double re, double l0,
-                                                    float tau, double* __restrict__ grad, double* __restrict__ lap,
+                                                    const int* __restrict__ cnt, int maxn, int n, double re, float tau,
+                                                    double* __restrict__ grad, double* __restrict__ lap,
                                                     uint8_t* __restrict__ status, uint8_t* __restrict__ pend,
                                                     unsigned* __restrict__ bpart, const int* __restrict__ out_perm) {
   __shared__ float sums[WLS_HALF * WLS_STRIDE];
   const int lane = threadIdx.x;
-  const double inv_re = 1.0 / re;
-  const float l0s = (float)(l0 * inv_re);
-  const float rmin = (float)(1e-12 * inv_re), rmin2 = rmin * rmin;
+  const float re_f = (float)re, re2 = (float)(re * re);
+  const float rmin2 = 1e-24f;  // the reference's r > 1e-12
   const int s0 = (int)xcd_block(blockIdx.x, gridDim.x) * WAVE;
   int n_ok = 0, n_skip = 0, n_pend = 0;
   for (int half = 0; half < WAVE / WLS_HALF; ++half) {
@@ -530,7 +533,7 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
 #pragma unroll
             for (int u = 0; u < 4; ++u) pj[u] = spos[j[u]];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) wls_accumulate(pj[u], pi, inv_re, rmin2, a);
+            for (int u = 0; u < 4; ++u) wls_accumulate(pj[u], pi, re_f, rmin2, re2, a);
           }
         }
       }
@@ -557,10 +560,11 @@ __global__ __launch_bounds__(WAVE) void k_wls_mixed(const double4* __restrict__ 
           float sol[9];
           if (wls_solve_f32(sums + lane * WLS_STRIDE, tau, sol)) {
             // The reference's quadratic coefficients are the ones of the
-            // unscaled monomials times 2 l0 / re.
+            // unscaled monomials times 2 l0, and its Laplacian is their sum
+            // over l0.
             const double g3[3] = {(double)sol[0], (double)sol[1], (double)sol[2]};
-            const float quad = (sol[3] + sol[4] + sol[5]) * (2.0f * l0s);
-            store_result(out_perm ? out_perm[s] : s, WLS_OK, g3, (double)quad / l0, grad, lap, status);
+            const float lp = (sol[3] + sol[4] + sol[5]) * 2.0f;
+            store_result(out_perm ? out_perm[s] : s, WLS_OK, g3, (double)lp, grad, lap, status);
             ++n_ok;
           } else {
             fp64 = true;
@@ -678,7 +682,7 @@ void launch_wls_mixed(const double4* spos, const uint8_t* type, const int* nbr, 
   if (n <= 0) return;
   const unsigned gm = blocks(n, WAVE);
   if (ev_begin) (void)hipEventRecord(ev_begin, st);
-  hipLaunchKernelGGL(k_wls_mixed, dim3(gm), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, l0, tau, grad, lap,
+  hipLaunchKernelGGL(k_wls_mixed, dim3(gm), dim3(WAVE), 0, st, spos, type, nbr, cnt, maxn, n, re, tau, grad, lap,
                      status, w.pend, bpart, out_perm);
   if (ev_end) (void)hipEventRecord(ev_end, st);
   // The flagged rows as one ascending list: offsets per mixed block, then

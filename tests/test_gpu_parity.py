@@ -259,6 +259,49 @@ def test_mixed_wls_redo_rows_equal_fp64(pipe):
             pipe.wls(re, l0, L.WLS_MIXED, fp32_pivot_min=bad)
 
 
+def test_mixed_wls_scale_and_range():
+    """The mixed WLS works in the caller's length unit: the same scene scaled
+    by 2^8 reproduces the quadratic field x^2 + 2y - z as accurately as the
+    unscaled one (gradient (2x, 2, -1), Laplacian 2, in each unit). With a
+    radius beyond 1e12 (scene scaled by 2^48) every row is computed in FP64
+    and the results are the FP64 kernel's bit for bit."""
+    l0 = 0.004
+    re = 2.1 * l0
+    xyz, t = L.generate_case("rain_plate", 20000, 5, l0)
+    say("mixed WLS scale", len(xyz))
+
+    def run(k):
+        p = L.Pipeline(0)
+        try:
+            s = 2.0 ** k
+            pts = xyz * s
+            p.upload(pts, t)
+            p.neighbors_classify(re * s, 128, l0 * s)
+            p.set_field(pts[:, 0] ** 2 + 2.0 * pts[:, 1] - pts[:, 2])
+            r = p.wls(re * s, l0 * s, L.WLS_MIXED)
+            out = p.wls_results()
+            p.wls(re * s, l0 * s, L.WLS_FP64)
+            return pts, r, out, p.wls_results()
+        finally:
+            p.close()
+
+    errs = []
+    for k in (0, 8):
+        pts, r, (g, lap, st), _ = run(k)
+        ok = st == L.WLS_OK
+        assert ok.sum() > len(xyz) // 2
+        ge = np.stack([2.0 * pts[:, 0], np.full(len(pts), 2.0), np.full(len(pts), -1.0)], 1)
+        eg = np.sqrt(np.mean(np.sum((g[ok] - ge[ok]) ** 2, 1) / np.sum(ge[ok] ** 2, 1)))
+        el = np.sqrt(np.mean((lap[ok] - 2.0) ** 2))
+        say("scale 2^%d" % k, "FP64 rows", r["fp64_redo"], "rms gradient (relative)", eg, "rms Laplacian", el)
+        assert eg < 1e-3 and el < 5e-2
+        errs.append((eg, el, r["fp64_redo"]))
+    assert abs(errs[0][2] - errs[1][2]) <= max(5, errs[0][2] // 100)
+    _, rbig, (gb, lb, sb), (g64, l64, s64) = run(48)
+    assert rbig["fp64_redo"] == rbig["ok"] + rbig["regularized"] + rbig["failed"]
+    assert np.array_equal(sb, s64) and np.array_equal(gb, g64) and np.array_equal(lb, l64)
+
+
 def test_kept_grid_reused_and_outgrown(pipe):
     """A build reuses the last build's grid (box plus a margin) without a
     round trip for the box; lists and types match a fresh context's whether
