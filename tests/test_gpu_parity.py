@@ -596,7 +596,8 @@ def test_build_self_checks():
     lengths adding up to the pair count). Exercised on the build sequences
     the tests and the timeline run: a fused build, the same particles again
     (kept grid, sorted reads), motion, a skin build, appended particles,
-    outflow removal, and a scene with far outliers (compressed grid)."""
+    outflow removal, a one-call step, and a scene with far outliers
+    (compressed grid)."""
     os.environ["LSMPS_CHECK_BUILD"] = "1"
     try:
         p = L.Pipeline(0)
@@ -622,6 +623,8 @@ def test_build_self_checks():
         lo, hi = xyz.min(axis=0), xyz.max(axis=0)
         p.remove_outside(lo, hi - np.array([0.1, 0.0, 0.0]))
         p.neighbors_classify(re, 96, l0)
+        p.set_field(L.C1_FIELD)
+        p.step(re, 96, l0, L.WLS_MIXED)  # the one-call step (checks after its WLS stage)
         far = np.concatenate([xyz, xyz[:1] + np.array([[1e6 * re, 0.0, 0.0]])])
         p.upload(far, np.concatenate([t, t[:1]]))
         p.neighbors_classify(re, 96, l0)  # compressed grid
