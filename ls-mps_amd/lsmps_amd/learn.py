@@ -208,7 +208,7 @@ def main(argv=None):
                       "particles": [100_000, 300_000], "stride": args.stride, "samples": n_samples,
                       "steps": args.steps, "batch": args.batch, "lr": args.lr, "solve": "FP64 Jacobi to 1e-10",
                       "loss_curve": curve},
-            "init": args.init,
+            "init": os.path.basename(args.init) if args.init else None,
             "eval": {"case": eval_case, "seeds": eval_seeds, "particles": args.eval_particles, "tol": 1e-8,
                      "summary": summary}}
     save_params(args.out, params, meta)
