@@ -155,6 +155,8 @@ def main(argv=None):
     ap.add_argument("--eval-case", default=None, choices=sorted(SCENES))
     ap.add_argument("--init", default=None, help="start from these weights instead of a fresh initialisation")
     ap.add_argument("--train-seeds", type=int, default=200)
+    ap.add_argument("--train-particles", default="100000,300000",
+                    help="LO,HI: training scene sizes, spread over [LO, HI] by seed")
     ap.add_argument("--eval-seeds", type=int, default=20)
     ap.add_argument("--eval-particles", type=int, default=1_000_000)
     ap.add_argument("--stride", type=int, default=4)
@@ -195,7 +197,9 @@ def main(argv=None):
         return
     pipe.set_params(load_params(args.init) if args.init else init_params(0))
     train_seeds = list(range(1, args.train_seeds + 1))
-    n_samples = collect(pipe, train_seeds, case=args.case, stride=args.stride)
+    lo, hi = (int(v) for v in args.train_particles.split(","))
+    sizes = [scene_size(sd, lo, hi) for sd in train_seeds]
+    n_samples = collect(pipe, train_seeds, case=args.case, stride=args.stride, sizes=sizes)
     log(f"training set: {n_samples} rows from {len(train_seeds)} scenes ({time.time() - t0:.0f} s)")
     curve = train(pipe, args.steps, args.batch, args.lr)
     params = pipe.get_params()
@@ -205,7 +209,7 @@ def main(argv=None):
     summary = summarize(rows)
     l0, maxn = SCENES[args.case]
     meta = {"train": {"case": args.case, "l0": l0, "re": 2.1 * l0, "max_neighbors": maxn, "seeds": [1, args.train_seeds],
-                      "particles": [100_000, 300_000], "stride": args.stride, "samples": n_samples,
+                      "particles": [lo, hi], "stride": args.stride, "samples": n_samples,
                       "steps": args.steps, "batch": args.batch, "lr": args.lr, "solve": "FP64 Jacobi to 1e-10",
                       "loss_curve": curve},
             "init": os.path.basename(args.init) if args.init else None,
