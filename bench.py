@@ -45,6 +45,8 @@ sys.path.insert(0, os.path.join(ROOT, "ls-mps_amd"))
 HBM_PEAK_GBS = 8000.0
 # Generator seed of the timed industrial_mix scene.
 SCENE_SEED = 23
+# Untimed steps with every HIP event, for the stage times.
+STAGE_STEPS = 10
 
 
 def parse():
@@ -346,8 +348,11 @@ def main():
     p.upload(xyz, types)
     p.set_field(L.C1_FIELD)
 
-    def run_steps(precision, k):
+    full = L.TIMING_STAGES | L.TIMING_KERNELS
+
+    def run_steps(precision, k, timing=full):
         stage = {"cnl": [], "wls": [], "k_cnl_pass": [], "k_wls": []}
+        p.set_timing(timing)
         p.L.lsmps_synchronize(p.h)
         t0 = time.perf_counter()
         for _ in range(k):
@@ -366,12 +371,19 @@ def main():
     # wls_test) and with the mixed WLS (FP32 moments; rows whose FP32 pivots
     # fall to 0.2 of their diagonal, which include those the reference would
     # regularise, in FP64); the same neighbor pass in both.
+    # The timed steps record only the HIP events around the stages' main
+    # kernels (each recorded event costs the stream a few microseconds); the
+    # stage times come from ten more steps with every event, after them.
     run_steps(L.WLS_FP64, args.warmup)
     log(f"warmup done ({args.warmup}); timing {args.steps} FP64 steps")
-    wall64, avg64, cs, ts, wr64 = run_steps(L.WLS_FP64, args.steps)
+    wall64, avg64, cs, ts, wr64 = run_steps(L.WLS_FP64, args.steps, L.TIMING_KERNELS)
+    _, st64, _, _, _ = run_steps(L.WLS_FP64, STAGE_STEPS)
     run_steps(L.WLS_MIXED, args.warmup)
     log(f"timing {args.steps} mixed steps")
-    wall32, avg32, cs, ts, wr = run_steps(L.WLS_MIXED, args.steps)
+    wall32, avg32, cs, ts, wr = run_steps(L.WLS_MIXED, args.steps, L.TIMING_KERNELS)
+    _, st32, _, _, _ = run_steps(L.WLS_MIXED, STAGE_STEPS)
+    for a, b in ((avg64, st64), (avg32, st32)):
+        a["cnl"], a["wls"] = b["cnl"], b["wls"]
     kmean = cs["mean"]
     log(f"step wall fp64 {wall64:.3f} ms {avg64}; mixed {wall32:.3f} ms {avg32}; k_mean {kmean:.2f}; wls {wr}")
     avg = {"cnl": avg32["cnl"], "wls_mixed": avg32["wls"], "wls_fp64": avg64["wls"],

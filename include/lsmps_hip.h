@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LSMPS_HIP_ABI_VERSION 7
+#define LSMPS_HIP_ABI_VERSION 8
 
 enum {
   LSMPS_OK = 0,
@@ -56,6 +56,15 @@ void lsmps_destroy(lsmps_ctx* ctx);
 const char* lsmps_last_error(const lsmps_ctx* ctx);
 /* Waits for all work queued on the context's stream. */
 int lsmps_synchronize(lsmps_ctx* ctx);
+/* Which HIP events the neighbor builds, lsmps_wls and lsmps_step record. No
+ * reference counterpart (the reference times its stages with the host
+ * clock): each event recorded on the stream costs it a few microseconds, so
+ * a time-step loop that only needs results can switch them off.
+ * LSMPS_TIMING_STAGES: the stages' ms; LSMPS_TIMING_KERNELS: their main
+ * kernel's kernel_ms (the neighbor pass, the main WLS kernel). A field whose
+ * events are off reads 0. Default: both. */
+enum { LSMPS_TIMING_STAGES = 1, LSMPS_TIMING_KERNELS = 2 };
+int lsmps_set_timing(lsmps_ctx* ctx, int32_t flags);
 /* "name (arch, N CUs)" of the context's device into buf (NUL-terminated).
  * No reference counterpart: for reports written next to the reference's. */
 int lsmps_device_name(lsmps_ctx* ctx, char* buf, int32_t cap);

@@ -186,6 +186,9 @@ struct lsmps_ctx {
   // LSMPS_CHECK_BUILD=1 at context creation: every neighbor build is checked
   // on the host (check_build), failing with LSMPS_E_CHECK.
   bool check_build = false;
+  // LSMPS_TIMING_* flags of lsmps_set_timing: which HIP events the neighbor
+  // builds and the WLS stage record (each costs the stream a few us).
+  int timing = LSMPS_TIMING_STAGES | LSMPS_TIMING_KERNELS;
 
   // learned preconditioner: features of the current PPE (row order u), the
   // network (FP32 master, BF16 copy, Adam moments) and the training set
@@ -519,6 +522,13 @@ int lsmps_synchronize(lsmps_ctx* ctx) {
   return LSMPS_OK;
 }
 
+int lsmps_set_timing(lsmps_ctx* ctx, int32_t flags) {
+  if (!ctx) return LSMPS_E_ARG;
+  if (flags & ~(LSMPS_TIMING_STAGES | LSMPS_TIMING_KERNELS)) return fail(ctx, LSMPS_E_ARG, "unknown timing flags");
+  ctx->timing = flags;
+  return LSMPS_OK;
+}
+
 int lsmps_device_name(lsmps_ctx* ctx, char* buf, int32_t cap) {
   if (!ctx || !buf || cap < 1) return LSMPS_E_ARG;
   hipDeviceProp_t prop;
@@ -736,6 +746,7 @@ struct BuildRun {
   int64_t ncells = 0;
   bool classify = false, spec = false, compressed = false;
   hipEvent_t ev[4] = {};
+  int timing = 0;  // the context's LSMPS_TIMING_* flags at submission
 };
 
 // Cell sort + one neighbor pass (+ classification when classify), queued on
@@ -780,7 +791,8 @@ int build_submit(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool clas
   HIPCHK(ctx->nbr.ensure((size_t)n * (size_t)std::max(1, maxn)));
   HIPCHK(ctx->hist.ensure((size_t)maxn + 1 + (size_t)(maxn + 6) * CNL_STATS_BLOCKS));
   HIPCHK(ctx->scan_scr.ensure(scan_scratch_elems((int64_t)n + 1)));
-  HIPCHK(hipEventRecord(ev[0], st));
+  const int timing = ctx->timing;
+  if (timing & LSMPS_TIMING_STAGES) HIPCHK(hipEventRecord(ev[0], st));
   // With the last build's grid the key pass makes the cell coordinates and
   // the box itself (launch_cell_sort with CellKeyDirect); otherwise the box
   // comes first.
@@ -873,9 +885,10 @@ int build_submit(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool clas
   if (int rc = check_launch(ctx, "cell_sort")) return rc;
   if (n == 0) launch_fill_int(ctx->hist.p, maxn + 1, 0, st);  // the pass's reduction writes every bin
   launch_cnl_pass(B, n, rl * rl, re * re, maxn, classify, 1.5 * l0, ctx->type_s.p, ctx->counters.p, ctx->hist.p,
-                  ctx->counters.p + 8, st, ev[2], ev[3]);
+                  ctx->counters.p + 8, st, (timing & LSMPS_TIMING_KERNELS) ? ev[2] : nullptr,
+                  (timing & LSMPS_TIMING_KERNELS) ? ev[3] : nullptr);
   if (int rc = check_launch(ctx, "cnl_pass")) return rc;
-  HIPCHK(hipEventRecord(ev[1], st));
+  if (timing & LSMPS_TIMING_STAGES) HIPCHK(hipEventRecord(ev[1], st));
   hipLaunchKernelGGL(k_readback, dim3(1), dim3(256), 0, st, ctx->counters.p, 16, ctx->hist.p, maxn + 1, ctx->box.p,
                      7, ctx->host_rb, reinterpret_cast<int*>(ctx->host_rb));
   if (int rc = check_launch(ctx, "readback")) return rc;
@@ -888,6 +901,7 @@ int build_submit(lsmps_ctx* ctx, double re, double skin, int32_t maxn, bool clas
   br->spec = spec;
   br->compressed = compressed;
   for (int k = 0; k < 4; ++k) br->ev[k] = ev[k];
+  br->timing = timing;
   return LSMPS_OK;
 }
 
@@ -946,8 +960,8 @@ int build_finish(lsmps_ctx* ctx, const BuildRun& br, bool* redo, lsmps_cnl_stats
     ctx->re = br.rl;  // the list radius the check tests against
     if (int rc = check_build(ctx, n, br.ncells, maxn, cs.pairs)) return rc;
   }
-  cs.ms = event_ms(br.ev[0], br.ev[1]);
-  cs.kernel_ms = n > 0 ? event_ms(br.ev[2], br.ev[3]) : 0.0f;
+  cs.ms = (br.timing & LSMPS_TIMING_STAGES) ? event_ms(br.ev[0], br.ev[1]) : 0.0f;
+  cs.kernel_ms = n > 0 && (br.timing & LSMPS_TIMING_KERNELS) ? event_ms(br.ev[2], br.ev[3]) : 0.0f;
   commit_build(ctx, br);
   ctx->pairs = cs.pairs;
   if (out) *out = cs;
@@ -1012,7 +1026,10 @@ int wls_submit(lsmps_ctx* ctx, const lsmps_wls_params* p, unsigned long long* rb
     mw = {ctx->wpend.p, ctx->wpoff.p, ctx->scan_scr.p, ctx->wlist.p};
   }
   HIPCHK(ctx->bpart.ensure(6 * ((size_t)n / 64 + 1)));
-  HIPCHK(hipEventRecord(ctx->e0, st));
+  const bool stage_ev = ctx->timing & LSMPS_TIMING_STAGES;
+  hipEvent_t kb = (ctx->timing & LSMPS_TIMING_KERNELS) ? ctx->e2 : nullptr;
+  hipEvent_t ke = (ctx->timing & LSMPS_TIMING_KERNELS) ? ctx->e3 : nullptr;
+  if (stage_ev) HIPCHK(hipEventRecord(ctx->e0, st));
   // Field into the sorted records: evaluated there for a quadratic,
   // gathered for values.
   if (!ctx->spos_field) {
@@ -1029,21 +1046,21 @@ int wls_submit(lsmps_ctx* ctx, const lsmps_wls_params* p, unsigned long long* rb
   if (p->precision == LSMPS_WLS_FP64) {
     launch_wls_exact(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, ctx->grad_s.p,
                      ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p, ctx->bpart.p, st,
-                     ctx->e2, ctx->e3);
+                     kb, ke);
   } else {
     launch_wls_mixed(ctx->spos.p, ctx->type_s.p, nb, nc, ctx->maxn, n, p->re, p->l0, p->reg, wls_tau(p), mw,
                      ctx->grad_s.p, ctx->lap_s.p, ctx->wstatus_s.p, ctx->redo.p, ctx->nredo.p, ctx->counters.p,
-                     ctx->bpart.p, st, ctx->e2, ctx->e3);
+                     ctx->bpart.p, st, kb, ke);
   }
   if (int rc = check_launch(ctx, "wls")) return rc;
-  HIPCHK(hipEventRecord(ctx->e1, st));
+  if (stage_ev) HIPCHK(hipEventRecord(ctx->e1, st));
   hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, st, ctx->counters.p, 6, nullptr, 0, nullptr, 0, rb,
                      reinterpret_cast<int*>(ctx->host_rb));
   return check_launch(ctx, "readback");
 }
 
 // Results of a WLS stage whose readback (c, six counters) has arrived.
-void wls_finish(lsmps_ctx* ctx, const unsigned long long* c, lsmps_wls_result* out) {
+void wls_finish(lsmps_ctx* ctx, const unsigned long long* c, int timing, lsmps_wls_result* out) {
   ctx->have_wls = true;
   if (out) {
     out->ok = c[WLS_OK];
@@ -1051,8 +1068,8 @@ void wls_finish(lsmps_ctx* ctx, const unsigned long long* c, lsmps_wls_result* o
     out->failed = c[WLS_FAILED] + c[WLS_NO_WEIGHT];
     out->skipped = c[WLS_SKIPPED];
     out->fp64_redo = c[WLS_REDO_SLOT];
-    out->ms = elapsed(ctx);
-    out->kernel_ms = ctx->n_cnl > 0 ? kernel_elapsed(ctx) : 0.0f;
+    out->ms = (timing & LSMPS_TIMING_STAGES) ? elapsed(ctx) : 0.0f;
+    out->kernel_ms = ctx->n_cnl > 0 && (timing & LSMPS_TIMING_KERNELS) ? kernel_elapsed(ctx) : 0.0f;
   }
 }
 
@@ -1274,9 +1291,10 @@ int lsmps_wls(lsmps_ctx* ctx, const lsmps_wls_params* p, lsmps_wls_result* out) 
   if (!ctx || !p) return LSMPS_E_ARG;
   if (int rc = wls_check(ctx, p, true)) return rc;
   GUARD_BEGIN
+  const int timing = ctx->timing;
   if (int rc = wls_submit(ctx, p, ctx->host_rb)) return rc;
   HIPCHK(hipStreamSynchronize(ctx->st));
-  wls_finish(ctx, ctx->host_rb, out);
+  wls_finish(ctx, ctx->host_rb, timing, out);
   return LSMPS_OK;
   GUARD_END
 }
@@ -1293,6 +1311,7 @@ int lsmps_step(lsmps_ctx* ctx, const lsmps_step_params* p, lsmps_cnl_stats* cs, 
   if (int rc = build_submit(ctx, p->re, 0.0, p->max_neighbors, true, p->l0, ctx->eb, &br)) return rc;
   commit_build(ctx, br);
   unsigned long long* rbw = ctx->host_rb + HOST_RB_WLS / 2;
+  const int timing = ctx->timing;
   if (int rc = wls_submit(ctx, &p->wls, rbw)) {
     ctx->have_cnl = false;  // the build's readback is not taken
     return rc;
@@ -1308,7 +1327,7 @@ int lsmps_step(lsmps_ctx* ctx, const lsmps_step_params* p, lsmps_cnl_stats* cs, 
     if (int rc = build_cell_list(ctx, p->re, 0.0, p->max_neighbors, true, p->l0, cs, ts)) return rc;
     return lsmps_wls(ctx, &p->wls, wr);
   }
-  wls_finish(ctx, rbw, wr);
+  wls_finish(ctx, rbw, timing, wr);
   return LSMPS_OK;
   GUARD_END
 }

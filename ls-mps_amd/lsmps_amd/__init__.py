@@ -25,6 +25,7 @@ WLS_FP64, WLS_MIXED = 0, 1
 WLS_FP32_PIVOT_DEFAULT = 0.2  # LSMPS_WLS_FP32_PIVOT_DEFAULT
 SOLVE_PLAIN, SOLVE_JACOBI, SOLVE_MIXED, SOLVE_LEARNED = range(4)
 XSTAR_QUADRATIC, XSTAR_RANDOM = 0, 1
+TIMING_STAGES, TIMING_KERNELS = 1, 2  # LSMPS_TIMING_* flags of Pipeline.set_timing
 
 _u64, _i32, _d, _f, _p = C.c_uint64, C.c_int32, C.c_double, C.c_float, C.c_void_p
 
@@ -115,6 +116,7 @@ ABI = {
     "lsmps_destroy": (None, [_p]),
     "lsmps_last_error": (C.c_char_p, [_p]),
     "lsmps_synchronize": (C.c_int, [_p]),
+    "lsmps_set_timing": (C.c_int, [_p, _i32]),
     "lsmps_device_name": (C.c_int, [_p, C.c_char_p, _i32]),
     "lsmps_upload_particles": (C.c_int, [_p, _p, _p, _u64]),
     "lsmps_build_neighbors": (C.c_int, [_p, _d, _i32, C.POINTER(CnlStats)]),
@@ -253,6 +255,11 @@ class Pipeline:
         s = WlsResult()
         self._check(self.L.lsmps_wls(self.h, C.byref(p), C.byref(s)), "wls")
         return as_dict(s)
+
+    def set_timing(self, flags):
+        """Which HIP events neighbor builds and WLS stages record
+        (TIMING_STAGES: their ms, TIMING_KERNELS: their kernel_ms)."""
+        self._check(self.L.lsmps_set_timing(self.h, flags), "set_timing")
 
     def step(self, re, max_neighbors, l0, precision=WLS_FP64, reg=1e-8, fp32_pivot_min=0.0, wls_re=None):
         """neighbors_classify(re, max_neighbors, l0) then wls(wls_re or re, l0,

@@ -43,7 +43,7 @@ def test_ctypes_binding_covers_the_abi():
     declared = set(declared_functions(os.path.join(ROOT, "include", "lsmps_hip.h")))
     assert declared == set(L.ABI), declared ^ set(L.ABI)
     lib = L.load_library()
-    assert lib.lsmps_abi_version() == 7
+    assert lib.lsmps_abi_version() == 8
 
 
 def test_generate_case_matches_reference(ref_configs):

@@ -259,6 +259,42 @@ def test_mixed_wls_redo_rows_equal_fp64(pipe):
             pipe.wls(re, l0, L.WLS_MIXED, fp32_pivot_min=bad)
 
 
+def test_timing_flags_leave_results_unchanged():
+    """lsmps_set_timing only decides which HIP events the neighbor builds and
+    WLS stages record: the same lists, types and WLS results with events off;
+    a field whose events are off reads 0, the others are timed."""
+    l0 = 0.01
+    re = 2.1 * l0
+    xyz, t = L.generate_case("wall_film", 8000, 4, l0)
+    say("timing flags", len(xyz))
+    out = {}
+    for flags in (L.TIMING_STAGES | L.TIMING_KERNELS, 0, L.TIMING_KERNELS, L.TIMING_STAGES):
+        p = L.Pipeline(0)
+        try:
+            p.upload(xyz, t)
+            p.set_field(L.C1_FIELD)
+            p.set_timing(flags)
+            cs, _, wr = p.step(re, 96, l0, L.WLS_MIXED)
+            cs2, _ = p.neighbors_classify(re, 96, l0)  # kept grid, the separate calls
+            wr2 = p.wls(re, l0, L.WLS_FP64)
+            for c in (cs, cs2):
+                assert (c["ms"] > 0) == bool(flags & L.TIMING_STAGES)
+                assert (c["kernel_ms"] > 0) == bool(flags & L.TIMING_KERNELS)
+            for w in (wr, wr2):
+                assert (w["ms"] > 0) == bool(flags & L.TIMING_STAGES)
+                assert (w["kernel_ms"] > 0) == bool(flags & L.TIMING_KERNELS)
+            out[flags] = (p.neighbor_lists(), p.types(), p.wls_results())
+            with pytest.raises(L.LsmpsError):
+                p.set_timing(4)
+        finally:
+            p.close()
+    ref = out[L.TIMING_STAGES | L.TIMING_KERNELS]
+    for flags, (lists, types, wls) in out.items():
+        assert all(np.array_equal(a, b) for a, b in zip(lists, ref[0]))
+        assert np.array_equal(types, ref[1])
+        assert all(a.tobytes() == b.tobytes() for a, b in zip(wls, ref[2]))
+
+
 def test_mixed_wls_scale_and_range():
     """The mixed WLS works in the caller's length unit: the same scene scaled
     by 2^8 reproduces the quadratic field x^2 + 2y - z as accurately as the
